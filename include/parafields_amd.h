@@ -1,0 +1,232 @@
+/*
+ * parafields_amd.h -- C ABI of the MI355X (gfx950) circulant-embedding
+ * Gaussian-random-field engine.
+ *
+ * The entry points replace the arithmetic that parafields-core's
+ * `parafields::Matrix` performs through FFTW-MPI (include/parafields/matrix.hh).
+ * They take plain pointers, sizes and POD structs; no C++ types cross the
+ * boundary and no exception escapes it. Every call returns a status code
+ * (PF_OK or a negative PF_E* value); pf_last_error() returns a message for the
+ * most recent failure on the calling thread.
+ *
+ * One plan = one covariance model on one grid (the state a reference `Matrix`
+ * holds: traits, lazily built embedding eigenvalues, field work buffers).
+ * One GPU holds the whole extended grid; calls on one plan are serialised by
+ * the plan; distinct plans may be used from distinct threads.
+ *
+ * Pointers: every data argument is accompanied by a pf_mem flag. PF_MEM_HOST
+ * pointers are staged through plan-owned device buffers; PF_MEM_DEVICE
+ * pointers must be device allocations on the plan's device and are used in
+ * place (the throughput path).
+ */
+#ifndef PARAFIELDS_AMD_H
+#define PARAFIELDS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference exception types they map to) ------------- */
+#define PF_OK 0
+#define PF_ENEG (-1)    /* NegativeEigenvalueError, include/parafields/exceptions.hh:8-15 */
+#define PF_EINVAL (-2)  /* std::runtime_error on bad configuration, matrix.hh:258-260 */
+#define PF_EHIP (-3)    /* device runtime failure */
+#define PF_ENOMEM (-4)  /* device allocation failure */
+#define PF_ESTATE (-5)  /* call not valid in the plan's current state */
+
+/* ---- enums ------------------------------------------------------------- */
+/* stochastic.covariance, covariance.hh:296-823 */
+enum pf_covariance {
+  PF_COV_EXPONENTIAL = 0,
+  PF_COV_GAUSSIAN = 1,
+  PF_COV_SPHERICAL = 2,
+  PF_COV_SEPARABLE_EXPONENTIAL = 3,
+  PF_COV_MATERN = 4,   /* general nu: own Bessel K_nu (reference needs GSL) */
+  PF_COV_MATERN32 = 5,
+  PF_COV_MATERN52 = 6,
+  PF_COV_DAMPED_OSCILLATION = 7,
+  PF_COV_GAMMA_EXPONENTIAL = 8,
+  PF_COV_CAUCHY = 9,
+  PF_COV_GENERALIZED_CAUCHY = 10,
+  PF_COV_CUBIC = 11,
+  PF_COV_WHITE_NOISE = 12,
+  PF_COV_CUSTOM = 13   /* host-evaluated values uploaded with pf_eigs_custom */
+};
+
+/* stochastic.anisotropy, matrix.hh:476-494 / covariance.hh:18-138 */
+enum pf_anisotropy { PF_ANISO_NONE = 0, PF_ANISO_AXIPARALLEL = 1, PF_ANISO_GEOMETRIC = 2 };
+
+/* MatrixBackend template argument of Matrix, matrix.hh:55-60 */
+enum pf_matrix_backend { PF_MATRIX_DCT = 0, PF_MATRIX_R2C = 1 };
+
+/* FieldBackend template argument of Matrix, matrix.hh:55-60 */
+enum pf_field_backend { PF_FIELD_R2C = 0, PF_FIELD_DCTDST = 1 };
+
+/* arithmetic of the per-realization transforms (eigenvalues are always FP64) */
+enum pf_precision { PF_FP64 = 0, PF_FP32 = 1 };
+
+/* randomField.transform, mutators.hh:126-163 */
+enum pf_transform {
+  PF_TRANSFORM_NONE = 0,
+  PF_TRANSFORM_LOGNORMAL = 1,
+  PF_TRANSFORM_FOLDEDNORMAL = 2,
+  PF_TRANSFORM_SIGN = 3,      /* param = threshold (reference: 0) */
+  PF_TRANSFORM_BOXCOX = 4     /* param = transform.boxCoxLambda */
+};
+
+enum pf_mem { PF_MEM_HOST = 0, PF_MEM_DEVICE = 1 };
+
+/* operator applied by pf_matvec */
+enum pf_matvec_op {
+  PF_OP_MULTIPLY = 0,        /* Matrix::operator*, matrix.hh:142-151 (C x / N_ext) */
+  PF_OP_ROOT = 1,            /* Matrix::multiplyRoot, matrix.hh:167-176 */
+  PF_OP_INVERSE_EMBED = 2    /* multiplyInverseExtended, matrix.hh:1587-1646 */
+};
+
+/* ---- descriptor: the configuration keys a reference Matrix reads ------- */
+typedef struct pf_desc {
+  int32_t dim;                 /* 2 or 3 */
+  int32_t cells[3];            /* grid.cells */
+  double extensions[3];        /* grid.extensions */
+  int32_t embedding_factor;    /* embedding.factor (default 2) */
+  int32_t periodic;            /* randomField.periodic (forces factor 1) */
+  int32_t covariance;          /* enum pf_covariance */
+  double variance;             /* stochastic.variance */
+  int32_t anisotropy;          /* enum pf_anisotropy */
+  double corr_length[9];       /* stochastic.corrLength: 1, dim or dim*dim values (row-major) */
+  double matern_nu;            /* stochastic.maternNu */
+  double exp_gamma;            /* stochastic.expGamma */
+  double cauchy_alpha;         /* stochastic.cauchyAlpha */
+  double cauchy_beta;          /* stochastic.cauchyBeta */
+  int32_t matrix_backend;      /* enum pf_matrix_backend */
+  int32_t field_backend;       /* enum pf_field_backend */
+  int32_t transposed;          /* fftw.transposed: -1 = reference default (dim > 1 with the
+                                  fieldtraits.hh:229-267 exceptions), 0 or 1 */
+  int32_t approximate;         /* embedding.approximate */
+  double threshold;            /* embedding.threshold (default 1e-14) */
+  int32_t precision;           /* enum pf_precision */
+  int32_t cg_iterations;       /* randomField.cgIterations (default 100) */
+  double nugget;               /* added to C(0) (custom-iso hook of the reference); 0 = none */
+} pf_desc;
+
+/* eigenvalue classification, matrix.hh:266-305 */
+typedef struct pf_eig_stats {
+  int64_t small;       /* 0 <= ... entries with threshold <= v < 1e-6 */
+  int64_t zero;        /* |v| < threshold */
+  int64_t negative;    /* v <= -threshold */
+  double smallest;     /* minimum entry */
+  int64_t entries;     /* entries classified (localMatrixSize after the transform) */
+} pf_eig_stats;
+
+/* sizes derived from the descriptor (fieldtraits.hh:217-314) */
+typedef struct pf_sizes {
+  int32_t dim;
+  int32_t cells[3];            /* n_i (1 for unused dims) */
+  int32_t extended[3];         /* N_i = factor * n_i */
+  int32_t transposed;          /* effective fftw.transposed */
+  int64_t domain_size;         /* prod n_i */
+  int64_t extended_size;       /* prod N_i */
+  int64_t eig_entries;         /* entries of the FP64 eigenvalue array */
+  int64_t noise_per_field;     /* N(0,1) samples one realization consumes */
+} pf_sizes;
+
+/* conjugate-gradient modes for pf_cg */
+enum pf_cg_mode {
+  PF_CG_REFERENCE = 0,   /* Matrix::multiplyInverse + innerCG, matrix.hh:189-206, :1315-1432 */
+  PF_CG_PLAIN = 1,       /* unpreconditioned CG, relative-residual stop */
+  PF_CG_EMBED_PCG = 2,   /* CG preconditioned by the embedding inverse, relative-residual stop */
+  PF_CG_LEARNED_PCG = 3  /* CG preconditioned by a learned radial spectral multiplier */
+};
+
+typedef struct pf_cg_opts {
+  int32_t mode;            /* enum pf_cg_mode */
+  int32_t max_iterations;  /* 0 = desc.cg_iterations */
+  double rel_tol;          /* stop at ||b - A x|| / ||b|| <= rel_tol (modes 1-3) */
+  const float* bin_log_multipliers; /* PF_CG_LEARNED_PCG: nbins host values */
+  int32_t nbins;
+} pf_cg_opts;
+
+typedef struct pf_cg_stats {
+  int32_t iterations;
+  double rel_residual;     /* FP64 true residual ||b - A x|| / ||b|| at exit */
+  double seconds;
+} pf_cg_stats;
+
+/* ---- plan lifecycle ------------------------------------------------------ */
+
+/* Replaces Matrix(traits) + update(), matrix.hh:98-133 and the size logic of
+ * RandomFieldTraits::update, fieldtraits.hh:217-314. Validates the
+ * configuration and allocates nothing large yet (eigenvalues are lazy, as in
+ * the reference). */
+int pf_plan_create(const pf_desc* desc, void** plan);
+int pf_plan_destroy(void* plan);
+int pf_plan_sizes(void* plan, pf_sizes* sizes);
+const char* pf_last_error(void);
+int pf_device_sync(void* plan);
+
+/* Replaces Matrix::fillTransformedMatrix(std::string), matrix.hh:223-318:
+ * covariance evaluation on the embedding grid, forward transform divided by
+ * N_ext, classification with clamping of negatives, and PF_ENEG when
+ * negatives exist and desc.approximate is 0 (stats are filled either way).
+ * Idempotent: a second call returns the cached statistics. */
+int pf_eigs(void* plan, pf_eig_stats* stats);
+
+/* Replaces Matrix::fillTransformedMatrix(Covariance&&) with a user callable
+ * (randomfield.hh:440-451, matrix.hh:261-264): `values` holds the covariance
+ * at every entry of the embedding grid the matrix backend evaluates
+ * (pf_sizes.eig_entries entries for PF_MATRIX_DCT; for PF_MATRIX_R2C the
+ * padded real array of 2*(N0/2+1)*N1*N2 entries), in the reference's
+ * x-fastest order. */
+int pf_eigs_custom(void* plan, const double* values, pf_eig_stats* stats);
+
+/* Copies the FP64 eigenvalues in the reference's final layout (DCT: the
+ * (N/2+1)^d array, 3D transposed as the reference stores it; R2C: the real
+ * part array of (N0/2+1)*N1*N2 entries). n = number of entries to copy. */
+int pf_eigs_copy(void* plan, double* out, int64_t n);
+
+/* Replaces Matrix::generateField (matrix.hh:330-421) followed by
+ * RandomField::bulkEvaluate's value transform (randomfield.hh:664-695,
+ * mutators.hh:126-163) for `batch` realizations.
+ * Noise source: if `noise` is non-NULL it holds batch * noise_per_field
+ * N(0,1) samples (always host FP64), consumed in the reference's order
+ * (the RNG `sample()` sequence of generateField); otherwise the device draws
+ * Philox4x32-10 normals keyed by `seed`, realization first_realization + b.
+ * Output: batch * domain_size values, x fastest, each realization contiguous;
+ * FP64 for PF_FP64 plans, FP32 for PF_FP32 plans. */
+int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_realization,
+                int32_t batch, int32_t transform, double transform_param, void* out,
+                int32_t out_mem);
+
+/* Replaces Matrix::operator* / multiplyRoot / multiplyInverseExtended
+ * (matrix.hh:142-176, :1443-1646) on `batch` vectors of domain_size values
+ * (element type as in pf_generate; x and y may alias). */
+int pf_matvec(void* plan, int32_t op, const void* x, void* y, int32_t batch, int32_t mem);
+
+/* Replaces Matrix::multiplyInverse (matrix.hh:189-206 with innerCG
+ * :1315-1432) in PF_CG_REFERENCE mode; other modes solve (C/N_ext) x = b to a
+ * relative residual. FP64 vectors of domain_size values. */
+int pf_cg(void* plan, const double* b, double* x, int32_t mem, const pf_cg_opts* opts,
+          pf_cg_stats* stats);
+
+/* Elementwise value transform (mutators.hh:126-163) of n values in place. */
+int pf_value_transform(void* plan, int32_t transform, double param, void* data, int64_t n,
+                       int32_t mem);
+
+/* Device Philox4x32-10 stream used by pf_generate when noise is NULL:
+ * writes n normals (FP64) of realization `realization` starting at sample
+ * index `first`. Exposed for statistical checks and parity tests. */
+int pf_philox_normals(void* plan, uint64_t seed, uint64_t realization, uint64_t first,
+                      int64_t n, double* out, int32_t mem);
+/* Raw Philox4x32-10 blocks: out[4*i..4*i+3] = philox(counter = (ctr0 + i,
+ * ctr1, ctr2, ctr3), key). */
+int pf_philox_raw(void* plan, const uint32_t ctr[4], const uint32_t key[2], int64_t nblocks,
+                  uint32_t* out, int32_t mem);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARAFIELDS_AMD_H */

@@ -1,0 +1,1361 @@
+// pf_oracle.cc -- FP64 CPU restatement of parafields-core's circulant-embedding
+// engine. TEST INFRASTRUCTURE ONLY: it is the checker that the HIP library is
+// compared against (tests, the smoke check and the CPU-baseline leg of the
+// benchmark). The product never links, loads or calls it.
+//
+// Every function follows the reference loop it restates (file:line relative
+// to the reference repository root, include/parafields/...). Arithmetic is
+// RF = double throughout, as in a reference build with RF = double. The
+// transforms that FFTW performs in the reference are computed with this
+// file's own mixed-radix FFT using FFTW's documented definitions:
+//   r2c   : X_k = sum_x x_x e^{-2 pi i k x / N}, k < N/2+1   (unnormalised)
+//   c2r   : c2c along all but the x axis, then a 1-D c2r along x that reads
+//           only Re of the DC and Nyquist entries                (unnormalised)
+//   REDFT00 (DCT-I), RODFT00 (DST-I): FFTW's definitions, unnormalised.
+//
+// Parity status of each function (what pins it) is listed in the tests that
+// exercise it (tests/test_oracle.py).
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/parafields_amd.h"
+
+namespace {
+
+using cd = std::complex<double>;
+using Index = long long;
+
+int g_threads = 1;
+
+// ----------------------------------------------------------------------------
+// Threading helper: split [0, n) over g_threads workers.
+// ----------------------------------------------------------------------------
+void parallel_for(Index n, const std::function<void(Index, Index)>& body)
+{
+  int nt = std::max(1, std::min<int>(g_threads, (int)std::min<Index>(n, 1 << 20)));
+  if (nt == 1 || n < 64) {
+    body(0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  Index chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    Index b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e)
+      break;
+    pool.emplace_back([&body, b, e] { body(b, e); });
+  }
+  for (auto& th : pool)
+    th.join();
+}
+
+// ----------------------------------------------------------------------------
+// Mixed-radix recursive FFT (decimation in time), unnormalised.
+// sign = -1: forward (e^{-i}), +1: backward (e^{+i}).
+// ----------------------------------------------------------------------------
+struct FFTPlan
+{
+  int n;
+  std::vector<cd> tw; // e^{-2 pi i j / n}, j < n
+  explicit FFTPlan(int n_)
+    : n(n_)
+    , tw(n_)
+  {
+    for (int j = 0; j < n; j++) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
+      tw[j] = cd((double)cosl(a), (double)sinl(a));
+    }
+  }
+};
+
+int smallest_factor(int n)
+{
+  if (n % 4 == 0)
+    return 4;
+  for (int p = 2; p * p <= n; p++)
+    if (n % p == 0)
+      return p;
+  return n;
+}
+
+void fft_rec(const cd* in, Index stride, cd* out, int n, int sign, const FFTPlan& plan,
+             int twstep)
+{
+  if (n == 1) {
+    out[0] = in[0];
+    return;
+  }
+  const int p = smallest_factor(n);
+  const int m = n / p;
+  for (int q = 0; q < p; q++)
+    fft_rec(in + q * stride, stride * p, out + q * m, m, sign, plan, twstep * p);
+  std::vector<cd> t(p), r(p);
+  for (int k = 0; k < m; k++) {
+    for (int q = 0; q < p; q++) {
+      cd w = plan.tw[(Index)q * k * twstep % plan.n];
+      if (sign > 0)
+        w = std::conj(w);
+      t[q] = w * out[q * m + k];
+    }
+    for (int s = 0; s < p; s++) {
+      cd acc = 0;
+      for (int q = 0; q < p; q++) {
+        cd w = plan.tw[(Index)((q * s) % p) * m * twstep % plan.n];
+        if (sign > 0)
+          w = std::conj(w);
+        acc += t[q] * w;
+      }
+      r[s] = acc;
+    }
+    for (int s = 0; s < p; s++)
+      out[k + m * s] = r[s];
+  }
+}
+
+void fft1d(cd* data, int n, int sign)
+{
+  static thread_local std::vector<std::pair<int, FFTPlan*>> cache;
+  FFTPlan* plan = nullptr;
+  for (auto& e : cache)
+    if (e.first == n)
+      plan = e.second;
+  if (!plan) {
+    plan = new FFTPlan(n);
+    cache.push_back({ n, plan });
+  }
+  std::vector<cd> in(data, data + n);
+  fft_rec(in.data(), 1, data, n, sign, *plan, 1);
+}
+
+// Apply a 1-D complex transform along axis `ax` of a complex array with
+// dims d[0..2] (x fastest). Only lines are transformed; the array is in place.
+void c2c_axis(cd* a, const Index d[3], int ax, int sign)
+{
+  Index stride = 1;
+  for (int i = 0; i < ax; i++)
+    stride *= d[i];
+  const int n = (int)d[ax];
+  const Index total = d[0] * d[1] * d[2];
+  const Index lines = total / n;
+  parallel_for(lines, [&](Index b, Index e) {
+    std::vector<cd> line(n);
+    for (Index l = b; l < e; l++) {
+      Index inner = l % stride, outer = l / stride;
+      Index base = inner + outer * stride * n;
+      for (int j = 0; j < n; j++)
+        line[j] = a[base + j * stride];
+      fft1d(line.data(), n, sign);
+      for (int j = 0; j < n; j++)
+        a[base + j * stride] = line[j];
+    }
+  });
+}
+
+// 1-D real-to-complex along x of real rows of length n0 (input pitch ip,
+// output pitch h = n0/2+1 complex), `rows` rows.
+void r2c_x(const double* in, Index ip, cd* out, int n0, Index rows)
+{
+  const int h = n0 / 2 + 1;
+  parallel_for(rows, [&](Index b, Index e) {
+    std::vector<cd> line(n0);
+    for (Index r = b; r < e; r++) {
+      for (int j = 0; j < n0; j++)
+        line[j] = in[r * ip + j];
+      fft1d(line.data(), n0, -1);
+      for (int k = 0; k < h; k++)
+        out[r * h + k] = line[k];
+    }
+  });
+}
+
+// 1-D complex-to-real along x (FFTW c2r semantics: Im of the DC and Nyquist
+// entries are ignored, the rest is Hermitian-extended), unnormalised.
+void c2r_x(const cd* in, double* out, Index op, int n0, Index rows)
+{
+  const int h = n0 / 2 + 1;
+  parallel_for(rows, [&](Index b, Index e) {
+    std::vector<cd> line(n0);
+    for (Index r = b; r < e; r++) {
+      const cd* z = in + r * h;
+      line[0] = cd(z[0].real(), 0.);
+      for (int k = 1; k < h; k++) {
+        line[k] = z[k];
+        if (n0 - k != k)
+          line[n0 - k] = std::conj(z[k]);
+      }
+      if (n0 % 2 == 0)
+        line[n0 / 2] = cd(z[n0 / 2].real(), 0.);
+      fft1d(line.data(), n0, +1);
+      for (int j = 0; j < n0; j++)
+        out[r * op + j] = line[j].real();
+    }
+  });
+}
+
+// FFTW REDFT00 along an axis of a real array (n = P points):
+// Y_k = X_0 + (-1)^k X_{n-1} + 2 sum_{j=1}^{n-2} X_j cos(pi j k / (n-1)),
+// computed as the DFT of the even extension of length 2(n-1).
+void redft00_axis(double* a, const Index d[3], int ax)
+{
+  Index stride = 1;
+  for (int i = 0; i < ax; i++)
+    stride *= d[i];
+  const int n = (int)d[ax];
+  const Index total = d[0] * d[1] * d[2];
+  const Index lines = total / n;
+  if (n == 1)
+    return;
+  const int N = 2 * (n - 1);
+  parallel_for(lines, [&](Index b, Index e) {
+    std::vector<cd> line(N);
+    for (Index l = b; l < e; l++) {
+      Index inner = l % stride, outer = l / stride;
+      Index base = inner + outer * stride * n;
+      for (int j = 0; j < n; j++)
+        line[j] = a[base + j * stride];
+      for (int j = 1; j < n - 1; j++)
+        line[N - j] = line[j];
+      fft1d(line.data(), N, -1);
+      for (int k = 0; k < n; k++)
+        a[base + k * stride] = line[k].real();
+    }
+  });
+}
+
+// FFTW RODFT00 along an axis of length n (interior points only):
+// Y_k = 2 sum_{j=0}^{n-1} X_j sin(pi (j+1)(k+1) / (n+1)),
+// computed from the odd extension of length 2(n+1).
+void rodft00_line(const double* x, double* y, int n)
+{
+  const int N = 2 * (n + 1);
+  std::vector<cd> line(N, 0.);
+  for (int j = 0; j < n; j++) {
+    line[j + 1] = x[j];
+    line[N - 1 - j] = -x[j];
+  }
+  fft1d(line.data(), N, -1);
+  // DFT of odd sequence: -2i sum x_j sin(...)  =>  Y_k = -Im(F_{k+1})
+  for (int k = 0; k < n; k++)
+    y[k] = -line[k + 1].imag();
+}
+
+// ----------------------------------------------------------------------------
+// Configuration-derived sizes, restating RandomFieldTraits::update,
+// fieldtraits.hh:217-314 (commSize = 1).
+// ----------------------------------------------------------------------------
+struct Geo
+{
+  int dim;
+  Index n[3], N[3], H;
+  double ext[3], h[3];
+  int factor;
+  bool transposed;
+  Index Next, domain;
+};
+
+Geo make_geo(const pf_desc& d)
+{
+  if (d.dim != 2 && d.dim != 3)
+    throw std::runtime_error("dim must be 2 or 3");
+  Geo g;
+  g.dim = d.dim;
+  g.factor = d.periodic ? 1 : d.embedding_factor;
+  if (g.factor < 1)
+    throw std::runtime_error("embedding.factor must be >= 1");
+  g.Next = 1;
+  g.domain = 1;
+  for (int i = 0; i < 3; i++) {
+    if (i < d.dim) {
+      g.n[i] = d.cells[i];
+      g.ext[i] = d.extensions[i];
+      g.h[i] = d.extensions[i] / d.cells[i];
+      g.N[i] = (Index)g.factor * d.cells[i];
+    } else {
+      g.n[i] = g.N[i] = 1;
+      g.ext[i] = g.h[i] = 1.;
+    }
+    g.Next *= g.N[i];
+    g.domain *= g.n[i];
+  }
+  g.H = g.N[0] / 2 + 1;
+  // fieldtraits.hh:229-267: default dim > 1; 2D with an R2C backend -> false.
+  if (d.transposed >= 0)
+    g.transposed = d.transposed != 0;
+  else if (d.dim == 2)
+    g.transposed = !(d.matrix_backend == PF_MATRIX_R2C || d.field_backend == PF_FIELD_R2C);
+  else
+    g.transposed = true;
+  return g;
+}
+
+// ----------------------------------------------------------------------------
+// Covariance functors, covariance.hh:296-823, and geometry, :18-138.
+// ----------------------------------------------------------------------------
+struct Geometry
+{
+  int dim, kind;
+  double m[9];
+  explicit Geometry(const pf_desc& d)
+  {
+    dim = d.dim;
+    kind = d.anisotropy;
+    if (kind == PF_ANISO_NONE || d.covariance == PF_COV_CUSTOM) {
+      // ScaledIdentityMatrix, covariance.hh:18-44 (custom-iso always uses it,
+      // matrix.hh:261-264)
+      kind = PF_ANISO_NONE;
+      m[0] = 1. / d.corr_length[0];
+    } else if (kind == PF_ANISO_AXIPARALLEL) {
+      // DiagonalMatrix, covariance.hh:49-74
+      for (int i = 0; i < dim; i++)
+        m[i] = 1. / d.corr_length[i];
+    } else if (kind == PF_ANISO_GEOMETRIC) {
+      // GeneralMatrix, covariance.hh:79-138: explicit cofactor inverse
+      const double* c = d.corr_length;
+      if (dim == 3) {
+        m[0] = c[4] * c[8] - c[5] * c[7];
+        m[1] = c[2] * c[7] - c[1] * c[8];
+        m[2] = c[1] * c[5] - c[2] * c[4];
+        m[3] = c[5] * c[6] - c[3] * c[8];
+        m[4] = c[0] * c[8] - c[2] * c[6];
+        m[5] = c[2] * c[3] - c[0] * c[5];
+        m[6] = c[3] * c[7] - c[4] * c[6];
+        m[7] = c[1] * c[6] - c[0] * c[7];
+        m[8] = c[0] * c[4] - c[1] * c[3];
+        const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) -
+                           c[1] * (c[3] * c[8] - c[5] * c[6]) +
+                           c[2] * (c[3] * c[7] - c[4] * c[6]);
+        for (int i = 0; i < 9; i++)
+          m[i] /= det;
+      } else {
+        m[0] = c[3];
+        m[1] = -c[1];
+        m[2] = -c[2];
+        m[3] = c[0];
+        const double det = c[0] * c[3] - c[1] * c[2];
+        for (int i = 0; i < 4; i++)
+          m[i] /= det;
+      }
+    } else
+      throw std::runtime_error("stochastic.anisotropy must be none, axiparallel or geometric");
+  }
+  void transform(const double* x, double* y) const
+  {
+    if (kind == PF_ANISO_NONE)
+      for (int i = 0; i < dim; i++)
+        y[i] = m[0] * x[i];
+    else if (kind == PF_ANISO_AXIPARALLEL)
+      for (int i = 0; i < dim; i++)
+        y[i] = m[i] * x[i];
+    else
+      for (int i = 0; i < dim; i++) {
+        y[i] = 0.;
+        for (int j = 0; j < dim; j++)
+          y[i] += m[i * dim + j] * x[j];
+      }
+  }
+};
+
+// Modified Bessel function K_nu(x), x > 0, nu >= 0: Temme's series for
+// |x| <= 2 and Steed's continued fraction CF2 for x > 2 (the method of
+// Numerical Recipes' bessik / Temme 1975), followed by forward recurrence.
+// Used for the general Matern family, which the reference evaluates with
+// GSL's gsl_sf_bessel_Knu (covariance.hh:539-555).
+double bessel_k(double nu, double x)
+{
+  const double EPS = 1e-16, FPMIN = 1e-300, PI = 3.14159265358979323846;
+  const int MAXIT = 100000;
+  int nl = (int)(nu + 0.5);
+  double xmu = nu - nl, xmu2 = xmu * xmu;
+  double xi = 1.0 / x, xi2 = 2.0 * xi;
+  double rkmu, rk1;
+  if (x < 2.0) {
+    double x2 = 0.5 * x, pimu = PI * xmu;
+    double fact = (std::fabs(pimu) < EPS ? 1.0 : pimu / std::sin(pimu));
+    double d = -std::log(x2), e = xmu * d;
+    double fact2 = (std::fabs(e) < EPS ? 1.0 : std::sinh(e) / e);
+    // gamma1 = (1/Gamma(1-xmu) - 1/Gamma(1+xmu)) / (2 xmu), gamma2 = (..+..)/2
+    double gampl = 1.0 / std::tgamma(1.0 + xmu), gammi = 1.0 / std::tgamma(1.0 - xmu);
+    // gamma1 -> -(c2 + c4 mu^2 + c6 mu^4 + c8 mu^6) near mu = 0, with c_k the
+    // Taylor coefficients of 1/Gamma(z) (Abramowitz & Stegun 6.1.34)
+    double gam1 = (std::fabs(xmu) < 1e-2)
+                    ? -(0.5772156649015329 - 0.0420026350340952 * xmu2 -
+                        0.0421977345555443 * xmu2 * xmu2 + 0.0072189432466630 * xmu2 * xmu2 * xmu2)
+                    : (gammi - gampl) / (2.0 * xmu);
+    double gam2 = 0.5 * (gammi + gampl);
+    double ff = fact * (gam1 * std::cosh(e) + gam2 * fact2 * d);
+    double sum = ff;
+    e = std::exp(e);
+    double p = 0.5 * e / gampl, q = 0.5 / (e * gammi), c = 1.0;
+    d = x2 * x2;
+    double sum1 = p;
+    int i;
+    for (i = 1; i <= MAXIT; i++) {
+      ff = (i * ff + p + q) / (i * i - xmu2);
+      c *= (d / i);
+      p /= (i - xmu);
+      q /= (i + xmu);
+      double del = c * ff;
+      sum += del;
+      double del1 = c * (p - i * ff);
+      sum1 += del1;
+      if (std::fabs(del) < std::fabs(sum) * EPS)
+        break;
+    }
+    rkmu = sum;
+    rk1 = sum1 * xi2;
+  } else {
+    double b = 2.0 * (1.0 + x), d = 1.0 / b, h = d, delh = d;
+    double q1 = 0.0, q2 = 1.0, a1 = 0.25 - xmu2, q = a1, c = a1, a = -a1;
+    double s = 1.0 + q * delh;
+    int i;
+    for (i = 2; i <= MAXIT; i++) {
+      a -= 2 * (i - 1);
+      c = -a * c / i;
+      double qnew = (q1 - b * q2) / a;
+      q1 = q2;
+      q2 = qnew;
+      q += c * qnew;
+      b += 2.0;
+      d = 1.0 / (b + a * d);
+      delh = (b * d - 1.0) * delh;
+      h += delh;
+      double dels = q * delh;
+      s += dels;
+      if (std::fabs(dels / s) < EPS)
+        break;
+    }
+    h = a1 * h;
+    rkmu = std::sqrt(PI / (2.0 * x)) * std::exp(-x) / s;
+    rk1 = rkmu * (xmu + x + 0.5 - h) * xi;
+  }
+  (void)FPMIN;
+  for (int i = 1; i <= nl; i++) {
+    double rktemp = (xmu + i) * xi2 * rk1 + rkmu;
+    rkmu = rk1;
+    rk1 = rktemp;
+  }
+  return rkmu;
+}
+
+struct Covariance
+{
+  pf_desc d;
+  explicit Covariance(const pf_desc& d_)
+    : d(d_)
+  {
+    if (d.covariance == PF_COV_GAMMA_EXPONENTIAL && (d.exp_gamma < 0. || d.exp_gamma > 2.))
+      throw std::runtime_error("exponent gamma has to be between 0 and 2");
+    if (d.covariance == PF_COV_GENERALIZED_CAUCHY &&
+        (d.cauchy_alpha <= 0. || d.cauchy_alpha > 2. || d.cauchy_beta <= 0.))
+      throw std::runtime_error("generalized Cauchy parameters out of range");
+    if (d.covariance == PF_COV_MATERN && d.matern_nu < 0.)
+      throw std::runtime_error("matern nu has to be positive");
+    if (d.covariance == PF_COV_CUBIC && d.dim == 3)
+      throw std::runtime_error("cubic covariance only applicable in 1D or 2D");
+    if (d.covariance < 0 || d.covariance > PF_COV_CUSTOM)
+      throw std::runtime_error("covariance structure not known");
+  }
+  double operator()(const double* x) const
+  {
+    const int dim = d.dim;
+    const double var = d.variance;
+    double sum = 0.;
+    for (int i = 0; i < dim; i++)
+      sum += x[i] * x[i];
+    const double h = std::sqrt(sum);
+    double v = 0.;
+    switch (d.covariance) {
+      case PF_COV_SPHERICAL:
+        if (h > 1.)
+          v = 0.;
+        else if (dim == 3)
+          v = var * (1. - 1.5 * h + 0.5 * std::pow(h, 3));
+        else
+          v = var * (1. - 2. / 3.14159265358979323846 *
+                            (h * std::sqrt(1. - std::pow(h, 2)) + std::asin(h)));
+        break;
+      case PF_COV_EXPONENTIAL:
+      case PF_COV_CUSTOM: // custom-iso nugget model: exponential + nugget
+        v = var * std::exp(-h);
+        break;
+      case PF_COV_GAMMA_EXPONENTIAL:
+        v = var * std::exp(-std::pow(h, d.exp_gamma));
+        break;
+      case PF_COV_GAUSSIAN:
+        v = var * std::exp(-h * h);
+        break;
+      case PF_COV_SEPARABLE_EXPONENTIAL: {
+        double s = 0.;
+        for (int i = 0; i < dim; i++)
+          s += std::fabs(x[i]);
+        v = var * std::exp(-s);
+        break;
+      }
+      case PF_COV_MATERN: {
+        const double nu = d.matern_nu;
+        if (h < 1e-10)
+          v = var;
+        else {
+          const double s2n = std::sqrt(2. * nu);
+          v = var * std::pow(2., 1. - nu) / std::tgamma(nu) * std::pow(s2n * h, nu) *
+              bessel_k(nu, s2n * h);
+        }
+        break;
+      }
+      case PF_COV_MATERN32:
+        v = var * (1. + std::sqrt(3.) * h) * std::exp(-std::sqrt(3.) * h);
+        break;
+      case PF_COV_MATERN52:
+        v = var * (1. + std::sqrt(5.) * h + 5. / 3. * h * h) * std::exp(-std::sqrt(5.) * h);
+        break;
+      case PF_COV_DAMPED_OSCILLATION:
+        if (dim == 3)
+          v = var * std::exp(-h) * std::cos(h / std::sqrt(3.));
+        else
+          v = var * std::exp(-h) * std::cos(h);
+        break;
+      case PF_COV_CAUCHY:
+        v = var * std::pow(1. + std::pow(h, 2), -3);
+        break;
+      case PF_COV_GENERALIZED_CAUCHY:
+        v = var * std::pow(1. + std::pow(h, d.cauchy_alpha), -d.cauchy_beta);
+        break;
+      case PF_COV_CUBIC:
+        if (h > 1.)
+          v = 0.;
+        else
+          v = var * (1. - 7. * std::pow(h, 2) + 8.75 * std::pow(h, 3) - 3.5 * std::pow(h, 5) +
+                     0.75 * std::pow(h, 7));
+        break;
+      case PF_COV_WHITE_NOISE: {
+        v = var;
+        for (int i = 0; i < dim; i++)
+          if (std::fabs(x[i]) > 1e-10)
+            v = 0.;
+        break;
+      }
+      default:
+        throw std::runtime_error("covariance structure not known");
+    }
+    return v;
+  }
+};
+
+// Entry value of the embedding grid at integer position idx:
+// Matrix::computeMatrixEntriesWithMirroring, matrix.hh:615-638.
+double embedded_cov(const Geo& g, const Geometry& geo, const Covariance& cov, const pf_desc& d,
+                    const Index idx[3])
+{
+  double coord[3] = { 0, 0, 0 }, tc[3] = { 0, 0, 0 };
+  for (int i = 0; i < g.dim; i++) {
+    coord[i] = (double)idx[i] * g.h[i];
+    if (coord[i] > 0.5 * g.ext[i] * (double)g.factor)
+      coord[i] -= g.ext[i] * (double)g.factor;
+  }
+  geo.transform(coord, tc);
+  double v = cov(tc);
+  if (d.nugget != 0.) {
+    bool origin = true;
+    for (int i = 0; i < g.dim; i++)
+      if (tc[i] != 0.)
+        origin = false;
+    if (origin)
+      v += d.nugget;
+  }
+  return v;
+}
+
+// ----------------------------------------------------------------------------
+// Eigenvalues. Result kept in a logical (natural x-fastest) layout:
+//   DCT: P0*P1*P2 reals, P_i = N_i/2+1.
+//   R2C: H*N1*N2 reals (Re part after the reference's clamping).
+// ----------------------------------------------------------------------------
+struct Eigs
+{
+  std::vector<double> v;
+  pf_eig_stats st;
+};
+
+// Classification loop, matrix.hh:266-291, over `count` reals visited in the
+// reference's storage order through `at(i)` (a reference to the value).
+template<typename At>
+void classify(Index count, double threshold, At at, pf_eig_stats& st)
+{
+  st.small = st.zero = st.negative = 0;
+  st.smallest = std::numeric_limits<double>::max();
+  st.entries = count;
+  for (Index i = 0; i < count; i++) {
+    double& value = at(i);
+    if (value < st.smallest)
+      st.smallest = value;
+    if (value < 1e-6) {
+      if (value < threshold) {
+        if (value > -threshold)
+          st.zero++;
+        else
+          st.negative++;
+      } else
+        st.small++;
+    }
+    if (value < 0.)
+      value = 0.;
+  }
+}
+
+// DCTMatrixBackend (dctmatrixbackend.hh:240-272) filled through
+// computeMatrixEntriesWithMirroring; `custom` (if non-null) supplies the
+// entries in x-fastest order instead.
+Eigs eigs_dct(const pf_desc& d, const Geo& g, const double* custom)
+{
+  Geometry geo(d);
+  Covariance cov(d);
+  Index P[3];
+  for (int i = 0; i < 3; i++)
+    P[i] = (i < g.dim) ? g.N[i] / 2 + 1 : 1;
+  const Index total = P[0] * P[1] * P[2];
+  Eigs e;
+  e.v.resize(total);
+  parallel_for(total, [&](Index b, Index en) {
+    for (Index f = b; f < en; f++) {
+      Index idx[3] = { f % P[0], (f / P[0]) % P[1], f / (P[0] * P[1]) };
+      e.v[f] = custom ? custom[f] : embedded_cov(g, geo, cov, d, idx);
+    }
+  });
+  for (int ax = 0; ax < g.dim; ax++)
+    redft00_axis(e.v.data(), P, ax);
+  for (auto& x : e.v)
+    x /= (double)g.Next;
+  // classification is order independent for the DCT array
+  classify(total, d.threshold, [&](Index i) -> double& { return e.v[i]; }, e.st);
+  return e;
+}
+
+
+// R2CMatrixBackend (r2cmatrixbackend.hh:254-285, finalize :412-421) with the
+// reference's classification over the interleaved buffer: after the forward
+// transform localMatrixSize() is the complex count (r2cmatrixbackend.hh:150-156)
+// but get/set index the buffer as reals (:378-400), so matrix.hh:270-291
+// checks and clamps Re and Im of the first half of the complex entries only.
+Eigs eigs_r2c(const pf_desc& d, const Geo& g, const double* custom)
+{
+  Geometry geo(d);
+  Covariance cov(d);
+  const Index H = g.H, N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const Index rows = N1 * N2;
+  std::vector<double> real(2 * H * rows);
+  parallel_for(rows, [&](Index b, Index en) {
+    for (Index r = b; r < en; r++) {
+      Index idx[3] = { 0, r % N1, r / N1 };
+      for (Index x = 0; x < 2 * H; x++) {
+        idx[0] = x;
+        real[r * 2 * H + x] = custom ? custom[r * 2 * H + x] : embedded_cov(g, geo, cov, d, idx);
+      }
+    }
+  });
+  std::vector<cd> spec(H * rows);
+  r2c_x(real.data(), 2 * H, spec.data(), (int)N0, rows);
+  Index dims[3] = { H, N1, N2 };
+  for (int ax = 1; ax < g.dim; ax++)
+    c2c_axis(spec.data(), dims, ax, -1);
+  for (auto& c : spec)
+    c /= (double)g.Next;
+  const bool tr = g.transposed && g.dim == 3;
+  // storage index (FFTW_MPI_TRANSPOSED_OUT in 3D: kx fastest, then kz, then ky)
+  // -> logical natural index kx + H (ky + N1 kz)
+  auto storage_to_logical = [&](Index s) -> Index {
+    if (!tr)
+      return s;
+    Index kx = s % H, kz = (s / H) % N2, ky = s / (H * N2);
+    return kx + H * (ky + N1 * kz);
+  };
+  const Index count = H * rows;
+  Eigs e;
+  classify(
+    count, d.threshold,
+    [&](Index i) -> double& {
+      double* p = reinterpret_cast<double*>(&spec[storage_to_logical(i / 2)]);
+      return p[i % 2];
+    },
+    e.st);
+  e.v.resize(count);
+  for (Index c = 0; c < count; c++)
+    e.v[c] = spec[c].real();
+  return e;
+}
+
+Eigs compute_eigs(const pf_desc& d, const Geo& g, const double* custom)
+{
+  if (d.matrix_backend == PF_MATRIX_DCT)
+    return eigs_dct(d, g, custom);
+  if (d.matrix_backend == PF_MATRIX_R2C)
+    return eigs_r2c(d, g, custom);
+  throw std::runtime_error("unknown matrix backend");
+}
+
+// Eigenvalue at logical spectral position (k0,k1,k2) of the R2C half spectrum
+// (DCTMatrixBackend::eval folds k >= P to N-k, dctmatrixbackend.hh:337-345;
+// R2CMatrixBackend::eval reads the same position, r2cmatrixbackend.hh:337-342).
+double eig_at(const pf_desc& d, const Geo& g, const Eigs& e, Index k0, Index k1, Index k2)
+{
+  if (d.matrix_backend == PF_MATRIX_DCT) {
+    Index P0 = g.N[0] / 2 + 1, P1 = g.N[1] / 2 + 1, P2 = g.N[2] / 2 + 1;
+    if (g.dim < 3)
+      P2 = 1;
+    Index a = k0 >= P0 ? g.N[0] - k0 : k0;
+    Index b = k1 >= P1 ? g.N[1] - k1 : k1;
+    Index c = (g.dim == 3 && k2 >= P2) ? g.N[2] - k2 : k2;
+    return e.v[a + P0 * (b + P1 * c)];
+  }
+  return e.v[k0 + g.H * (k1 + g.N[1] * k2)];
+}
+
+// Storage index -> logical (k0,k1,k2) of the R2C half spectrum: the order in
+// which generateField visits entries (matrix.hh:374-404; 3D transposed
+// layout after transposeIfNeeded, r2cfieldbackend.hh:190-199).
+void spec_storage_to_k(const Geo& g, Index s, Index k[3])
+{
+  const Index H = g.H;
+  if (g.dim == 3 && g.transposed) {
+    k[0] = s % H;
+    k[2] = (s / H) % g.N[2];
+    k[1] = s / (H * g.N[2]);
+  } else {
+    k[0] = s % H;
+    k[1] = (s / H) % g.N[1];
+    k[2] = s / (H * g.N[1]);
+  }
+}
+
+// Backward transform of the R2C field backend (r2cfieldbackend.hh:250-274)
+// followed by extendedFieldToField (:423-441).
+void r2c_backward_extract(const Geo& g, std::vector<cd>& spec, double* field)
+{
+  Index dims[3] = { g.H, g.N[1], g.N[2] };
+  for (int ax = g.dim - 1; ax >= 1; ax--)
+    c2c_axis(spec.data(), dims, ax, +1);
+  const Index rows = g.N[1] * g.N[2];
+  std::vector<double> real(g.N[0] * rows);
+  c2r_x(spec.data(), real.data(), g.N[0], (int)g.N[0], rows);
+  for (Index z = 0; z < g.n[2]; z++)
+    for (Index y = 0; y < g.n[1]; y++)
+      for (Index x = 0; x < g.n[0]; x++)
+        field[x + g.n[0] * (y + g.n[1] * z)] = real[x + g.N[0] * (y + g.N[1] * z)];
+}
+
+// Matrix::generateField, general version (matrix.hh:374-421) with
+// R2CFieldBackend::set (r2cfieldbackend.hh:305-325).
+void generate_r2c(const pf_desc& d, const Geo& g, const Eigs& e, const double* noise,
+                  double* field)
+{
+  static const double sqrtTwo = std::sqrt(2.);
+  const Index count = g.H * g.N[1] * g.N[2];
+  std::vector<cd> spec(count);
+  for (Index s = 0; s < count; s++) {
+    Index k[3];
+    spec_storage_to_k(g, s, k);
+    const double lambda = std::sqrt(eig_at(d, g, e, k[0], k[1], k[2]));
+    const double rand1 = noise[2 * s];
+    const double rand2 = noise[2 * s + 1];
+    bool allMultiple = true;
+    for (int i = 0; i < g.dim; i++)
+      if ((2 * k[i]) % g.N[i] != 0)
+        allMultiple = false;
+    const Index c = k[0] + g.H * (k[1] + g.N[1] * k[2]);
+    if (allMultiple)
+      spec[c] = cd(lambda * rand1, 0.);
+    else
+      spec[c] = cd(lambda / sqrtTwo * rand1, lambda / sqrtTwo * rand2);
+  }
+  r2c_backward_extract(g, spec, field);
+}
+
+// Storage permutation of the DCT/DST arrays: transposeIfNeeded swaps the two
+// last dims (dctdstfieldbackend.hh:868-882).
+void dct_storage_to_k(const Geo& g, const Index P[3], Index s, Index k[3])
+{
+  int perm[3] = { 0, 1, 2 };
+  if (g.transposed)
+    std::swap(perm[g.dim - 2], perm[g.dim - 1]);
+  Index r = s;
+  for (int j = 0; j < 3; j++) {
+    Index sz = P[perm[j]];
+    k[perm[j]] = r % sz;
+    r /= sz;
+  }
+}
+
+// Field extraction walk of metaForLoopFromExtended (dctdstfieldbackend.hh:905-930):
+// x runs over n0 entries then skips 1, each y row skips P0 after n1 rows.
+Index dct_extract_index(const Geo& g, const Index P[3], Index x, Index y, Index z)
+{
+  if (g.dim == 2)
+    return x + y * (g.n[0] + 1);
+  return x + y * (g.n[0] + 1) + z * (g.n[1] * (g.n[0] + 1) + P[0]);
+}
+
+// Matrix::generateField, DCT/DST version (matrix.hh:342-372) with
+// DCTDSTFieldBackend::set (dctdstfieldbackend.hh:418-438), the r2r backward
+// transforms (:326-404; REDFT00 on even axes, RODFT00 on the interior of odd
+// axes with the zero boundary reinserted, :1041-1140) and the additive
+// extraction (:631-648).
+void generate_dctdst(const pf_desc& d, const Geo& g, const Eigs& e, const double* noise,
+                     double* field)
+{
+  if (d.matrix_backend != PF_MATRIX_DCT)
+    throw std::runtime_error("DCTDSTFieldBackend requires DCTMatrixBackend");
+  Index P[3];
+  for (int i = 0; i < 3; i++)
+    P[i] = (i < g.dim) ? g.N[i] / 2 + 1 : 1;
+  const Index count = P[0] * P[1] * P[2];
+  const double scale = 1. / std::sqrt((double)(1 << g.dim));
+  for (Index i = 0; i < g.domain; i++)
+    field[i] = 0.;
+  std::vector<double> a(count);
+  for (unsigned type = 0; type < (1u << g.dim); type++) {
+    bool odd[3] = { false, false, false };
+    for (int i = 0; i < g.dim; i++)
+      odd[i] = (type >> i) & 1;
+    for (Index s = 0; s < count; s++) {
+      Index k[3];
+      dct_storage_to_k(g, P, s, k);
+      const Index c = k[0] + P[0] * (k[1] + P[1] * k[2]);
+      const double lambda = std::sqrt(e.v[c]);
+      const double rand = noise[type * count + s];
+      unsigned border = 0;
+      bool zero = false;
+      for (int i = 0; i < g.dim; i++)
+        if (k[i] == 0 || k[i] == P[i] - 1) {
+          if (odd[i]) {
+            zero = true;
+            break;
+          }
+          border++;
+        }
+      if (zero) {
+        a[c] = 0.;
+        continue;
+      }
+      double v = scale * lambda * rand;
+      if (border > 0)
+        v *= std::sqrt((double)(1 << border));
+      a[c] = v;
+    }
+    for (int ax = 0; ax < g.dim; ax++) {
+      if (!odd[ax]) {
+        redft00_axis(a.data(), P, ax);
+        continue;
+      }
+      Index stride = 1;
+      for (int i = 0; i < ax; i++)
+        stride *= P[i];
+      const int n = (int)P[ax];
+      const Index lines = count / n;
+      parallel_for(lines, [&](Index b, Index en) {
+        std::vector<double> in(n), out(n);
+        for (Index l = b; l < en; l++) {
+          Index inner = l % stride, outer = l / stride;
+          Index base = inner + outer * stride * n;
+          for (int j = 0; j < n - 2; j++)
+            in[j] = a[base + (j + 1) * stride];
+          rodft00_line(in.data(), out.data(), n - 2);
+          a[base] = 0.;
+          a[base + (n - 1) * stride] = 0.;
+          for (int j = 0; j < n - 2; j++)
+            a[base + (j + 1) * stride] = out[j];
+        }
+      });
+    }
+    for (Index z = 0; z < g.n[2]; z++)
+      for (Index y = 0; y < g.n[1]; y++)
+        for (Index x = 0; x < g.n[0]; x++)
+          field[x + g.n[0] * (y + g.n[1] * z)] += a[dct_extract_index(g, P, x, y, z)];
+  }
+}
+
+// Matrix::multiplyExtended / multiplyRootExtended / multiplyInverseExtended,
+// general version (matrix.hh:1443-1646): zero-pad (r2cfieldbackend.hh:352-368),
+// r2c / N_ext (:210-239), mult (:337-341), c2r, restrict.
+void matvec_r2c(const pf_desc& d, const Geo& g, const Eigs& e, int op, const double* x,
+                double* y)
+{
+  const Index rows = g.N[1] * g.N[2];
+  std::vector<double> real(g.N[0] * rows, 0.);
+  for (Index z = 0; z < g.n[2]; z++)
+    for (Index yy = 0; yy < g.n[1]; yy++)
+      for (Index xx = 0; xx < g.n[0]; xx++)
+        real[xx + g.N[0] * (yy + g.N[1] * z)] = x[xx + g.n[0] * (yy + g.n[1] * z)];
+  std::vector<cd> spec(g.H * rows);
+  r2c_x(real.data(), g.N[0], spec.data(), (int)g.N[0], rows);
+  Index dims[3] = { g.H, g.N[1], g.N[2] };
+  for (int ax = 1; ax < g.dim; ax++)
+    c2c_axis(spec.data(), dims, ax, -1);
+  for (auto& c : spec)
+    c /= (double)g.Next;
+  for (Index k2 = 0; k2 < g.N[2]; k2++)
+    for (Index k1 = 0; k1 < g.N[1]; k1++)
+      for (Index k0 = 0; k0 < g.H; k0++) {
+        const double m = eig_at(d, g, e, k0, k1, k2);
+        const double f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+        spec[k0 + g.H * (k1 + g.N[1] * k2)] *= f;
+      }
+  r2c_backward_extract(g, spec, y);
+}
+
+struct Engine
+{
+  pf_desc d;
+  Geo g;
+  Eigs e;
+  int status;
+};
+
+Engine* get_engine(const pf_desc* d, const double* custom = nullptr)
+{
+  static thread_local Engine* cached = nullptr;
+  if (!custom && cached && std::memcmp(&cached->d, d, sizeof(pf_desc)) == 0)
+    return cached;
+  Engine* en = new Engine;
+  en->d = *d;
+  en->g = make_geo(*d);
+  if (d->field_backend == PF_FIELD_R2C && d->dim == 2 && en->g.transposed)
+    throw std::runtime_error("R2CFieldBackend supports transposed output only for dim > 2");
+  en->e = compute_eigs(*d, en->g, custom);
+  en->status = (en->e.st.negative > 0 && !d->approximate) ? PF_ENEG : PF_OK;
+  if (custom)
+    return en; // caller owns
+  delete cached;
+  cached = en;
+  return en;
+}
+
+void matvec(const Engine& en, int op, const double* x, double* y)
+{
+  if (en.d.field_backend != PF_FIELD_R2C)
+    throw std::runtime_error("matvec implemented for the R2C field backend");
+  matvec_r2c(en.d, en.g, en.e, op, x, y);
+}
+
+double dot(const std::vector<double>& a, const std::vector<double>& b)
+{
+  double s = 0.;
+  for (size_t i = 0; i < a.size(); i++)
+    s += a[i] * b[i];
+  return s;
+}
+
+// Matrix::innerCG, matrix.hh:1315-1432, restated line by line (commSize 1).
+int inner_cg(const Engine& en, std::vector<double>& iter, const std::vector<double>& solution,
+             bool precondition, unsigned cgIterations)
+{
+  const size_t n = iter.size();
+  std::vector<double> tempSolution = solution;
+  std::vector<double> matrixTimesSolution(n), matrixTimesIter(n), residual(n), precResidual(n),
+    direction(n), matrixTimesDirection(n);
+  double scalarProd, scalarProd2, alphaDenominator, alpha, beta;
+  matvec(en, PF_OP_MULTIPLY, tempSolution.data(), matrixTimesSolution.data()); // unused product
+  matvec(en, PF_OP_MULTIPLY, iter.data(), matrixTimesIter.data());
+  for (size_t i = 0; i < n; i++)
+    residual[i] = solution[i] - matrixTimesIter[i];
+  if (precondition)
+    matvec(en, PF_OP_INVERSE_EMBED, residual.data(), precResidual.data());
+  else
+    precResidual = residual;
+  direction = precResidual;
+  bool converged = false;
+  scalarProd = dot(precResidual, residual);
+  scalarProd2 = dot(residual, residual);
+  if (std::sqrt(std::abs(scalarProd2)) < 1e-6)
+    converged = true;
+  double firstValue = 0.;
+  for (size_t i = 0; i < n; i++)
+    firstValue += iter[i] * (0.5 * matrixTimesIter[i] - solution[i]);
+  unsigned count = 0;
+  while (!converged && count < cgIterations) {
+    matvec(en, PF_OP_MULTIPLY, direction.data(), matrixTimesDirection.data());
+    alphaDenominator = dot(direction, matrixTimesDirection);
+    alpha = scalarProd / alphaDenominator;
+    double oldValue = 0.;
+    for (size_t i = 0; i < n; i++)
+      oldValue += iter[i] * (0.5 * matrixTimesIter[i] - solution[i]);
+    for (size_t i = 0; i < n; i++) {
+      iter[i] += alpha * direction[i];
+      matrixTimesIter[i] += alpha * matrixTimesDirection[i];
+    }
+    double value = 0.;
+    for (size_t i = 0; i < n; i++)
+      value += iter[i] * (0.5 * matrixTimesIter[i] - solution[i]);
+    for (size_t i = 0; i < n; i++)
+      residual[i] = solution[i] - matrixTimesIter[i];
+    if (precondition)
+      matvec(en, PF_OP_INVERSE_EMBED, residual.data(), precResidual.data());
+    else
+      precResidual = residual;
+    beta = 1. / scalarProd;
+    scalarProd = dot(precResidual, residual);
+    beta *= scalarProd;
+    for (size_t i = 0; i < n; i++)
+      direction[i] = precResidual[i] + beta * direction[i];
+    if (value != firstValue) {
+      if (std::abs(value - oldValue) / std::abs(value - firstValue) < 1e-16)
+        converged = true;
+    }
+    count++;
+  }
+  return (int)count;
+}
+
+// Textbook (P)CG on (C/N_ext) x = b with a relative-residual stop, the
+// residual recomputed from the FP64 matvec every iteration. `prec` applies
+// M^{-1}; x starts at 0.
+int pcg(const Engine& en, const std::vector<double>& b, std::vector<double>& x,
+        const std::function<void(const std::vector<double>&, std::vector<double>&)>* prec,
+        int maxit, double rel_tol, double* relres)
+{
+  const size_t n = b.size();
+  x.assign(n, 0.);
+  std::vector<double> r = b, z(n), p(n), Ap(n);
+  const double bnorm = std::sqrt(dot(b, b));
+  if (bnorm == 0.) {
+    *relres = 0.;
+    return 0;
+  }
+  if (prec)
+    (*prec)(r, z);
+  else
+    z = r;
+  p = z;
+  double rz = dot(r, z);
+  int it = 0;
+  double rr = std::sqrt(dot(r, r)) / bnorm;
+  while (rr > rel_tol && it < maxit) {
+    matvec(en, PF_OP_MULTIPLY, p.data(), Ap.data());
+    const double alpha = rz / dot(p, Ap);
+    for (size_t i = 0; i < n; i++)
+      x[i] += alpha * p[i];
+    // true residual r = b - A x
+    std::vector<double> Ax(n);
+    matvec(en, PF_OP_MULTIPLY, x.data(), Ax.data());
+    for (size_t i = 0; i < n; i++)
+      r[i] = b[i] - Ax[i];
+    if (prec)
+      (*prec)(r, z);
+    else
+      z = r;
+    const double rzn = dot(r, z);
+    const double beta = rzn / rz;
+    rz = rzn;
+    for (size_t i = 0; i < n; i++)
+      p[i] = z[i] + beta * p[i];
+    it++;
+    rr = std::sqrt(dot(r, r)) / bnorm;
+  }
+  *relres = rr;
+  return it;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11 "Parallel random numbers: as easy as
+// 1, 2, 3"), the counter-based generator of the device throughput path.
+// ---------------------------------------------------------------------------
+void philox4x32_10(uint32_t c[4], const uint32_t key[2])
+{
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+// Normal sample `s` of realization `real`: block s/4 of the Philox stream
+// keyed by the 64-bit seed; uniforms u = (x + 0.5) 2^-32; Box-Muller on the
+// pairs (u0,u1) and (u2,u3): r = sqrt(-2 ln u_a), t = 2 pi u_b,
+// n_a = r cos t, n_b = r sin t.
+void philox_normals(uint64_t seed, uint64_t real, uint64_t first, Index n, double* out)
+{
+  const uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+  const double TWO_PI = 6.283185307179586476925286766559;
+  for (Index i = 0; i < n; i++) {
+    const uint64_t s = first + (uint64_t)i;
+    const uint64_t blk = s >> 2;
+    uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
+                      (uint32_t)(real >> 32) };
+    philox4x32_10(c, key);
+    const int lane = (int)(s & 3);
+    const int pa = lane & 2;
+    const double ua = ((double)c[pa] + 0.5) * (1.0 / 4294967296.0);
+    const double ub = ((double)c[pa + 1] + 0.5) * (1.0 / 4294967296.0);
+    const double r = std::sqrt(-2.0 * std::log(ua));
+    const double t = TWO_PI * ub;
+    out[i] = (lane & 1) ? r * std::sin(t) : r * std::cos(t);
+  }
+}
+
+// ValueTransform, mutators.hh:35-163.
+void value_transform(int kind, double param, double* v, Index n)
+{
+  for (Index i = 0; i < n; i++) {
+    double& x = v[i];
+    switch (kind) {
+      case PF_TRANSFORM_NONE:
+        break;
+      case PF_TRANSFORM_LOGNORMAL:
+        x = std::exp(x);
+        break;
+      case PF_TRANSFORM_FOLDEDNORMAL:
+        x = std::abs(x);
+        break;
+      case PF_TRANSFORM_SIGN:
+        x = (x < param) ? -1. : 1.;
+        break;
+      case PF_TRANSFORM_BOXCOX:
+        x = std::pow(x * param + 1, -param);
+        break;
+      default:
+        throw std::runtime_error("transform type not known");
+    }
+  }
+}
+
+thread_local std::string g_err;
+
+} // namespace
+
+#define PFO_TRY(...)                                                                               \
+  try {                                                                                            \
+    __VA_ARGS__;                                                                                   \
+  } catch (const std::exception& ex) {                                                             \
+    g_err = ex.what();                                                                             \
+    return PF_EINVAL;                                                                              \
+  }
+
+extern "C" {
+
+const char* pfo_last_error(void) { return g_err.c_str(); }
+
+int pfo_set_threads(int n)
+{
+  g_threads = n < 1 ? 1 : n;
+  return PF_OK;
+}
+
+int pfo_sizes(const pf_desc* d, pf_sizes* s)
+{
+  PFO_TRY({
+    Geo g = make_geo(*d);
+    s->dim = g.dim;
+    for (int i = 0; i < 3; i++) {
+      s->cells[i] = (int)g.n[i];
+      s->extended[i] = (int)g.N[i];
+    }
+    s->transposed = g.transposed;
+    s->domain_size = g.domain;
+    s->extended_size = g.Next;
+    Index P = 1;
+    for (int i = 0; i < g.dim; i++)
+      P *= g.N[i] / 2 + 1;
+    const Index half = g.H * g.N[1] * g.N[2];
+    s->eig_entries = d->matrix_backend == PF_MATRIX_DCT ? P : half;
+    s->noise_per_field = d->field_backend == PF_FIELD_DCTDST ? (P << g.dim) : 2 * half;
+  });
+  return PF_OK;
+}
+
+// Eigenvalues (fillTransformedMatrix). `out` (optional) receives them in the
+// reference's storage layout. Returns PF_ENEG when negatives exist and
+// approximate is off (stats filled either way).
+int pfo_eigs(const pf_desc* d, const double* custom, double* out, pf_eig_stats* st)
+{
+  PFO_TRY({
+    Engine* en = get_engine(d, custom);
+    if (st)
+      *st = en->e.st;
+    if (out) {
+      const Geo& g = en->g;
+      if (d->matrix_backend == PF_MATRIX_DCT) {
+        Index P[3];
+        for (int i = 0; i < 3; i++)
+          P[i] = (i < g.dim) ? g.N[i] / 2 + 1 : 1;
+        const Index count = P[0] * P[1] * P[2];
+        for (Index s = 0; s < count; s++) {
+          Index k[3];
+          dct_storage_to_k(g, P, s, k);
+          out[s] = en->e.v[k[0] + P[0] * (k[1] + P[1] * k[2])];
+        }
+      } else {
+        const Index count = g.H * g.N[1] * g.N[2];
+        for (Index s = 0; s < count; s++) {
+          Index k[3];
+          spec_storage_to_k(g, s, k);
+          out[s] = en->e.v[k[0] + g.H * (k[1] + g.N[1] * k[2])];
+        }
+      }
+    }
+    int status = en->status;
+    if (custom)
+      delete en;
+    return status;
+  });
+}
+
+// One realization: noise consumed in generateField's order, untransformed.
+int pfo_generate(const pf_desc* d, const double* noise, double* field)
+{
+  PFO_TRY({
+    Engine* en = get_engine(d);
+    if (en->status != PF_OK) {
+      g_err = "negative eigenvalues in covariance matrix";
+      return en->status;
+    }
+    if (d->field_backend == PF_FIELD_DCTDST)
+      generate_dctdst(en->d, en->g, en->e, noise, field);
+    else
+      generate_r2c(en->d, en->g, en->e, noise, field);
+  });
+  return PF_OK;
+}
+
+int pfo_matvec(const pf_desc* d, int op, const double* x, double* y)
+{
+  PFO_TRY({
+    Engine* en = get_engine(d);
+    if (en->status != PF_OK) {
+      g_err = "negative eigenvalues in covariance matrix";
+      return en->status;
+    }
+    matvec(*en, op, x, y);
+  });
+  return PF_OK;
+}
+
+// Matrix::multiplyInverse, matrix.hh:189-206.
+int pfo_multiply_inverse(const pf_desc* d, const double* b, double* x, int* iterations)
+{
+  PFO_TRY({
+    Engine* en = get_engine(d);
+    if (en->status != PF_OK) {
+      g_err = "negative eigenvalues in covariance matrix";
+      return en->status;
+    }
+    const Index n = en->g.domain;
+    std::vector<double> input(b, b + n), output(b, b + n);
+    bool fieldZero = true;
+    for (Index i = 0; i < n; i++)
+      if (std::abs(input[i]) > 1e-10)
+        fieldZero = false;
+    int it = 0;
+    if (!fieldZero) {
+      std::vector<double> tmp(n);
+      matvec(*en, PF_OP_INVERSE_EMBED, output.data(), tmp.data());
+      output = tmp;
+      it = inner_cg(*en, output, input, true, (unsigned)d->cg_iterations);
+    }
+    std::copy(output.begin(), output.end(), x);
+    if (iterations)
+      *iterations = it;
+  });
+  return PF_OK;
+}
+
+// Relative-residual (P)CG: mode PF_CG_PLAIN or PF_CG_EMBED_PCG.
+int pfo_pcg(const pf_desc* d, int mode, int maxit, double rel_tol, const double* b, double* x,
+            int* iterations, double* relres)
+{
+  PFO_TRY({
+    Engine* en = get_engine(d);
+    if (en->status != PF_OK) {
+      g_err = "negative eigenvalues in covariance matrix";
+      return en->status;
+    }
+    const Index n = en->g.domain;
+    std::vector<double> bb(b, b + n), xx;
+    std::function<void(const std::vector<double>&, std::vector<double>&)> prec =
+      [&](const std::vector<double>& r, std::vector<double>& z) {
+        z.resize(r.size());
+        matvec(*en, PF_OP_INVERSE_EMBED, r.data(), z.data());
+      };
+    int it = pcg(*en, bb, xx, mode == PF_CG_EMBED_PCG ? &prec : nullptr, maxit, rel_tol, relres);
+    std::copy(xx.begin(), xx.end(), x);
+    if (iterations)
+      *iterations = it;
+  });
+  return PF_OK;
+}
+
+int pfo_value_transform(int kind, double param, double* v, long long n)
+{
+  PFO_TRY(value_transform(kind, param, v, n));
+  return PF_OK;
+}
+
+// CppRNGBackend (cpprngbackend.hh:57-100): std::default_random_engine seeded
+// with seed (+ rank 0, randomfield.hh:525-526), std::normal_distribution<double>.
+int pfo_noise_cpp(unsigned seed, long long n, double* out)
+{
+  std::default_random_engine generator;
+  std::normal_distribution<double> normalDist(0., 1.);
+  generator.seed(seed);
+  for (long long i = 0; i < n; i++)
+    out[i] = normalDist(generator);
+  return PF_OK;
+}
+
+int pfo_noise_philox(uint64_t seed, uint64_t realization, uint64_t first, long long n,
+                     double* out)
+{
+  philox_normals(seed, realization, first, n, out);
+  return PF_OK;
+}
+
+int pfo_philox_raw(const uint32_t ctr[4], const uint32_t key[2], long long nblocks,
+                   uint32_t* out)
+{
+  for (long long i = 0; i < nblocks; i++) {
+    uint32_t c[4] = { ctr[0] + (uint32_t)i, ctr[1], ctr[2], ctr[3] };
+    philox4x32_10(c, key);
+    for (int j = 0; j < 4; j++)
+      out[4 * i + j] = c[j];
+  }
+  return PF_OK;
+}
+
+// Covariance functor (with geometry) at `n` points of raw coordinates.
+int pfo_cov_eval(const pf_desc* d, const double* x, long long n, double* out)
+{
+  PFO_TRY({
+    Geometry geo(*d);
+    Covariance cov(*d);
+    for (long long i = 0; i < n; i++) {
+      double t[3] = { 0, 0, 0 };
+      geo.transform(x + i * d->dim, t);
+      out[i] = cov(t);
+    }
+  });
+  return PF_OK;
+}
+
+double pfo_bessel_k(double nu, double x) { return bessel_k(nu, x); }
+
+} // extern "C"

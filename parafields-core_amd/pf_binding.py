@@ -1,0 +1,327 @@
+"""ctypes binding of the C ABI declared in include/parafields_amd.h.
+
+This is the Python-side caller of libparafields_amd.so (the HIP library). It
+does no arithmetic of its own: every operation is a call into the library,
+and loading fails loudly when the library has not been built.
+
+The configuration mirrors the reference's ParameterTree keys
+(include/parafields/fieldtraits.hh:169-209, matrix.hh:223-318); `Desc`
+defaults follow the reference defaults (embedding.factor 2,
+embedding.threshold 1e-14, randomField.cgIterations 100, fftw.transposed
+default).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libparafields_amd.so")
+
+PF_OK, PF_ENEG, PF_EINVAL, PF_EHIP, PF_ENOMEM, PF_ESTATE = 0, -1, -2, -3, -4, -5
+
+COVARIANCES = {
+    "exponential": 0, "gaussian": 1, "spherical": 2, "separableExponential": 3,
+    "matern": 4, "matern32": 5, "matern52": 6, "dampedOscillation": 7,
+    "gammaExponential": 8, "cauchy": 9, "generalizedCauchy": 10, "cubic": 11,
+    "whiteNoise": 12, "custom-iso": 13,
+}
+ANISOTROPY = {"none": 0, "axiparallel": 1, "geometric": 2}
+MATRIX_BACKENDS = {"dct": 0, "r2c": 1}
+FIELD_BACKENDS = {"r2c": 0, "dctdst": 1}
+PRECISION = {"fp64": 0, "fp32": 1}
+TRANSFORMS = {"none": 0, "logNormal": 1, "foldedNormal": 2, "sign": 3, "boxCox": 4}
+OPS = {"multiply": 0, "root": 1, "inverse_embed": 2}
+CG_MODES = {"reference": 0, "plain": 1, "embed_pcg": 2, "learned_pcg": 3}
+MEM_HOST, MEM_DEVICE = 0, 1
+
+
+class NegativeEigenvalueError(RuntimeError):
+    """Counterpart of parafields::NegativeEigenvalueError (exceptions.hh:8-15)."""
+
+
+class PfDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("cells", ctypes.c_int32 * 3),
+        ("extensions", ctypes.c_double * 3),
+        ("embedding_factor", ctypes.c_int32),
+        ("periodic", ctypes.c_int32),
+        ("covariance", ctypes.c_int32),
+        ("variance", ctypes.c_double),
+        ("anisotropy", ctypes.c_int32),
+        ("corr_length", ctypes.c_double * 9),
+        ("matern_nu", ctypes.c_double),
+        ("exp_gamma", ctypes.c_double),
+        ("cauchy_alpha", ctypes.c_double),
+        ("cauchy_beta", ctypes.c_double),
+        ("matrix_backend", ctypes.c_int32),
+        ("field_backend", ctypes.c_int32),
+        ("transposed", ctypes.c_int32),
+        ("approximate", ctypes.c_int32),
+        ("threshold", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("cg_iterations", ctypes.c_int32),
+        ("nugget", ctypes.c_double),
+    ]
+
+
+class PfEigStats(ctypes.Structure):
+    _fields_ = [
+        ("small", ctypes.c_int64),
+        ("zero", ctypes.c_int64),
+        ("negative", ctypes.c_int64),
+        ("smallest", ctypes.c_double),
+        ("entries", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PfSizes(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("cells", ctypes.c_int32 * 3),
+        ("extended", ctypes.c_int32 * 3),
+        ("transposed", ctypes.c_int32),
+        ("domain_size", ctypes.c_int64),
+        ("extended_size", ctypes.c_int64),
+        ("eig_entries", ctypes.c_int64),
+        ("noise_per_field", ctypes.c_int64),
+    ]
+
+
+class PfCgOpts(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("rel_tol", ctypes.c_double),
+        ("bin_log_multipliers", ctypes.POINTER(ctypes.c_float)),
+        ("nbins", ctypes.c_int32),
+    ]
+
+
+class PfCgStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("rel_residual", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[float]] = None,
+              covariance: str = "exponential", variance: float = 1.0,
+              corr_length: Sequence[float] | float = 0.05, anisotropy: str = "none",
+              factor: int = 2, periodic: bool = False, matrix: Optional[str] = None,
+              field: str = "r2c", transposed: int = -1, approximate: bool = False,
+              threshold: float = 1e-14, precision: str = "fp64", cg_iterations: int = 100,
+              nugget: float = 0.0, matern_nu: float = 1.0, exp_gamma: float = 1.0,
+              cauchy_alpha: float = 1.0, cauchy_beta: float = 1.0) -> PfDesc:
+    """Descriptor with the reference's defaults. `matrix=None` picks what
+    RandomField would (randomfield.hh:153-164): DCT (DefaultIsoMatrix) for
+    none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy."""
+    d = PfDesc()
+    d.dim = dim
+    for i in range(3):
+        d.cells[i] = int(cells[i]) if i < dim else 1
+        d.extensions[i] = float(extensions[i]) if (extensions is not None and i < dim) else 1.0
+    d.embedding_factor = factor
+    d.periodic = int(periodic)
+    d.covariance = COVARIANCES[covariance]
+    d.variance = variance
+    d.anisotropy = ANISOTROPY[anisotropy]
+    cl = [corr_length] if np.isscalar(corr_length) else list(corr_length)
+    for i, v in enumerate(cl):
+        d.corr_length[i] = float(v)
+    d.matern_nu, d.exp_gamma = matern_nu, exp_gamma
+    d.cauchy_alpha, d.cauchy_beta = cauchy_alpha, cauchy_beta
+    if matrix is None:
+        matrix = "r2c" if (anisotropy == "geometric") else "dct"
+    d.matrix_backend = MATRIX_BACKENDS[matrix]
+    d.field_backend = FIELD_BACKENDS[field]
+    d.transposed = transposed
+    d.approximate = int(approximate)
+    d.threshold = threshold
+    d.precision = PRECISION[precision]
+    d.cg_iterations = cg_iterations
+    d.nugget = nugget
+    return d
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libparafields_amd.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    vp, i32, i64, u64, dp = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
+                             ctypes.c_double)
+    sig = {
+        "pf_plan_create": (i32, [ctypes.POINTER(PfDesc), ctypes.POINTER(vp)]),
+        "pf_plan_destroy": (i32, [vp]),
+        "pf_plan_sizes": (i32, [vp, ctypes.POINTER(PfSizes)]),
+        "pf_last_error": (ctypes.c_char_p, []),
+        "pf_device_sync": (i32, [vp]),
+        "pf_eigs": (i32, [vp, ctypes.POINTER(PfEigStats)]),
+        "pf_eigs_custom": (i32, [vp, vp, ctypes.POINTER(PfEigStats)]),
+        "pf_eigs_copy": (i32, [vp, vp, i64]),
+        "pf_generate": (i32, [vp, vp, u64, u64, i32, i32, dp, vp, i32]),
+        "pf_matvec": (i32, [vp, i32, vp, vp, i32, i32]),
+        "pf_cg": (i32, [vp, vp, vp, i32, ctypes.POINTER(PfCgOpts), ctypes.POINTER(PfCgStats)]),
+        "pf_value_transform": (i32, [vp, i32, dp, vp, i64, i32]),
+        "pf_philox_normals": (i32, [vp, u64, u64, u64, i64, vp, i32]),
+        "pf_philox_raw": (i32, [vp, vp, vp, i64, vp, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names every caller may bind (must all be exported by the library)."""
+    return ["pf_plan_create", "pf_plan_destroy", "pf_plan_sizes", "pf_last_error",
+            "pf_device_sync", "pf_eigs", "pf_eigs_custom", "pf_eigs_copy", "pf_generate",
+            "pf_matvec", "pf_cg", "pf_value_transform", "pf_philox_normals", "pf_philox_raw"]
+
+
+def check(status: int):
+    if status == PF_OK:
+        return
+    msg = load().pf_last_error().decode(errors="replace")
+    if status == PF_ENEG:
+        raise NegativeEigenvalueError(msg)
+    raise RuntimeError(f"parafields_amd error {status}: {msg}")
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor (device memory)
+
+
+def _mem(a) -> int:
+    return MEM_HOST if isinstance(a, np.ndarray) else MEM_DEVICE
+
+
+class Plan:
+    """One covariance model on one grid: the state a reference Matrix holds."""
+
+    def __init__(self, desc: PfDesc):
+        self.lib = load()
+        self.desc = desc
+        h = ctypes.c_void_p()
+        check(self.lib.pf_plan_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+        s = PfSizes()
+        check(self.lib.pf_plan_sizes(self.h, ctypes.byref(s)))
+        self.sizes = s
+        self.domain = int(s.domain_size)
+        self.dtype = np.float64 if desc.precision == 0 else np.float32
+
+    def close(self):
+        if self.h:
+            self.lib.pf_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def eigs(self) -> dict:
+        st = PfEigStats()
+        check(self.lib.pf_eigs(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def eigs_stats_nothrow(self):
+        st = PfEigStats()
+        status = self.lib.pf_eigs(self.h, ctypes.byref(st))
+        return status, st.as_dict()
+
+    def eigs_custom(self, values: np.ndarray) -> dict:
+        st = PfEigStats()
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(self.lib.pf_eigs_custom(self.h, v.ctypes.data, ctypes.byref(st)))
+        return st.as_dict()
+
+    def eig_values(self) -> np.ndarray:
+        out = np.empty(int(self.sizes.eig_entries), dtype=np.float64)
+        check(self.lib.pf_eigs_copy(self.h, out.ctypes.data, out.size))
+        return out
+
+    def generate(self, batch: int = 1, noise: Optional[np.ndarray] = None, seed: int = 0,
+                 first: int = 0, transform: str = "none", param: float = 0.0, out=None):
+        if out is None:
+            out = np.empty((batch, self.domain), dtype=self.dtype)
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            assert nz.size == batch * int(self.sizes.noise_per_field)
+        check(self.lib.pf_generate(self.h, None if nz is None else nz.ctypes.data, seed, first,
+                                   batch, TRANSFORMS[transform], param, _ptr(out), _mem(out)))
+        return out
+
+    def matvec(self, x, op: str = "multiply", batch: int = 1, out=None):
+        if out is None:
+            out = np.empty_like(x)
+        check(self.lib.pf_matvec(self.h, OPS[op], _ptr(x), _ptr(out), batch, _mem(x)))
+        return out
+
+    def cg(self, b, mode: str = "reference", max_iterations: int = 0, rel_tol: float = 1e-8,
+           bin_log_multipliers: Optional[np.ndarray] = None, x=None):
+        if x is None:
+            x = np.empty_like(b)
+        o = PfCgOpts()
+        o.mode = CG_MODES[mode]
+        o.max_iterations = max_iterations
+        o.rel_tol = rel_tol
+        keep = None
+        if bin_log_multipliers is not None:
+            keep = np.ascontiguousarray(bin_log_multipliers, dtype=np.float32)
+            o.bin_log_multipliers = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            o.nbins = keep.size
+        st = PfCgStats()
+        check(self.lib.pf_cg(self.h, _ptr(b), _ptr(x), _mem(b), ctypes.byref(o),
+                             ctypes.byref(st)))
+        return x, {"iterations": st.iterations, "rel_residual": st.rel_residual,
+                   "seconds": st.seconds}
+
+    def value_transform(self, data, transform: str, param: float = 0.0):
+        check(self.lib.pf_value_transform(self.h, TRANSFORMS[transform], param, _ptr(data),
+                                          data.size if isinstance(data, np.ndarray)
+                                          else data.numel(), _mem(data)))
+        return data
+
+    def philox_normals(self, seed: int, realization: int, first: int, n: int, out=None):
+        if out is None:
+            out = np.empty(n, dtype=np.float64)
+        check(self.lib.pf_philox_normals(self.h, seed, realization, first, n, _ptr(out),
+                                         _mem(out)))
+        return out
+
+    def philox_raw(self, ctr, key, nblocks: int):
+        c = np.ascontiguousarray(ctr, dtype=np.uint32)
+        k = np.ascontiguousarray(key, dtype=np.uint32)
+        out = np.empty(4 * nblocks, dtype=np.uint32)
+        check(self.lib.pf_philox_raw(self.h, c.ctypes.data, k.ctypes.data, nblocks,
+                                     out.ctypes.data, MEM_HOST))
+        return out
+
+    def sync(self):
+        check(self.lib.pf_device_sync(self.h))

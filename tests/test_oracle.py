@@ -1,0 +1,383 @@
+"""Pins the FP64 CPU oracle (oracle/pf_oracle.cc) before it is trusted as the
+checker of the HIP path. Each check is independent of the oracle's own code:
+textbook closed forms, scipy/numpy library routines (fftn, dctn/dstn type 1,
+eigvalsh, solve, kv), brute-force dense matrices, and statistical identities
+of the generated fields."""
+import numpy as np
+import pytest
+from scipy import fft as sfft
+from scipy import special
+
+import np_helpers as nh
+import oracle_py as orc
+import pf_binding as pfb
+
+
+def desc(dim, cells, **kw):
+    kw.setdefault("extensions", [1.0] * dim)
+    return pfb.make_desc(dim, cells, **kw)
+
+
+# ---------------------------------------------------------------- noise --
+
+def test_cpp_noise_golden_seed42():
+    # libstdc++ default_random_engine(42) + normal_distribution<double>; values
+    # measured for the reference's CppRNGBackend (cpprngbackend.hh:57-100).
+    v = orc.noise_cpp(42, 2)
+    assert v[0] == -1.7141127395876619
+    assert v[1] == 0.17805682714513033
+
+
+def test_philox_known_answers():
+    # Random123 philox4x32_10 known-answer vectors (Salmon et al. 2011).
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        got = orc.philox_raw(ctr, key, 1)
+        assert tuple(int(x) for x in got) == want
+
+
+def test_philox_normals_moments():
+    x = orc.noise_philox(7, 3, 0, 400000)
+    assert abs(x.mean()) < 5 / np.sqrt(x.size)
+    assert abs(x.var() - 1) < 6 * np.sqrt(2 / x.size)
+    # pairs from one Box-Muller draw are uncorrelated
+    assert abs(np.mean(x[0::2] * x[1::2])) < 6 / np.sqrt(x.size / 2)
+    # counter layout: a later window equals the tail of a longer draw
+    y = orc.noise_philox(7, 3, 5, 11)
+    assert np.array_equal(y, x[5:16])
+    # realizations are distinct streams
+    z = orc.noise_philox(7, 4, 0, 16)
+    assert not np.allclose(z, x[:16])
+
+
+# ------------------------------------------------------------ covariance --
+
+@pytest.mark.parametrize("nu", [0.3, 0.5, 1.0, 1.5, 2.5, 3.7])
+def test_bessel_k_vs_scipy(nu):
+    for x in [1e-3, 0.1, 0.9, 1.99, 2.01, 5.0, 30.0]:
+        want = special.kv(nu, x)
+        assert abs(orc.bessel_k(nu, x) - want) <= 1e-12 * abs(want)
+
+
+@pytest.mark.parametrize("kind", ["exponential", "gaussian", "spherical", "matern32",
+                                  "matern52", "matern", "cauchy", "generalizedCauchy",
+                                  "gammaExponential", "dampedOscillation", "cubic",
+                                  "separableExponential"])
+@pytest.mark.parametrize("aniso", ["none", "axiparallel", "geometric"])
+def test_covariance_closed_forms(kind, aniso):
+    dim = 2
+    rng = np.random.default_rng(1)
+    corr = {"none": [0.3], "axiparallel": [0.3, 0.1],
+            "geometric": [0.3, 0.05, -0.02, 0.1]}[aniso]
+    kw = dict(matern_nu=1.7, exp_gamma=1.3, cauchy_alpha=1.5, cauchy_beta=0.8)
+    d = desc(dim, [8, 8], covariance=kind, variance=2.0, corr_length=corr, anisotropy=aniso,
+             **kw)
+    pts = rng.uniform(-0.5, 0.5, size=(200, dim))
+    got = orc.cov_eval(d, pts)
+    want = nh.cov_at(pts, kind, aniso, corr, var=2.0, nu=1.7, gamma=1.3, alpha=1.5, beta=0.8)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+
+
+def test_covariance_3d_geometric_and_spherical():
+    rng = np.random.default_rng(2)
+    A = [0.2, 0.03, 0.01, -0.02, 0.1, 0.0, 0.01, 0.02, 0.05]
+    for kind in ["spherical", "matern52"]:
+        d = desc(3, [4, 4, 4], covariance=kind, corr_length=A, anisotropy="geometric")
+        pts = rng.uniform(-0.3, 0.3, size=(100, 3))
+        np.testing.assert_allclose(orc.cov_eval(d, pts), nh.cov_at(pts, kind, "geometric", A),
+                                   rtol=1e-12, atol=1e-14)
+
+
+def test_cubic_rejected_in_3d():
+    d = desc(3, [4, 4, 4], covariance="cubic", corr_length=0.2)
+    with pytest.raises(RuntimeError):
+        orc.eigs(d)
+
+
+# ----------------------------------------------------------- eigenvalues --
+
+def to_natural_dct(v, d):
+    s = orc.sizes(d)
+    P = [s.extended[i] // 2 + 1 for i in range(s.dim)]
+    if s.dim == 2:
+        arr = v.reshape(P[1], P[0])
+        return arr.T.T if not s.transposed else v.reshape(P[0], P[1]).T
+    if s.transposed:
+        return v.reshape(P[1], P[2], P[0]).transpose(1, 0, 2)
+    return v.reshape(P[2], P[1], P[0])
+
+
+@pytest.mark.parametrize("dim,cells,kind,aniso,corr", [
+    (2, [8, 6], "exponential", "none", [0.2]),
+    (2, [16, 16], "gaussian", "axiparallel", [0.1, 0.05]),
+    (3, [4, 6, 4], "matern32", "axiparallel", [0.1, 0.1, 0.05]),
+    (3, [4, 4, 4], "spherical", "none", [0.3]),
+])
+def test_eigs_dct_vs_library(dim, cells, kind, aniso, corr):
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, approximate=True)
+    v, st, _ = orc.eigs(d, allow_negative=True)
+    nat = to_natural_dct(v, d)
+    full = nh.full_eigs(cells, [1.0] * dim, kind, aniso, corr)
+    assert np.abs(full.imag).max() < 1e-12
+    P = [c + 1 for c in cells]
+    half = full.real[tuple(slice(0, p) for p in P[::-1])]
+    # scipy's DCT-I is FFTW's REDFT00
+    g = nh.wrapped_grid(cells, [1.0] * dim)
+    c = nh.cov_at(g.reshape(-1, dim), kind, aniso, corr).reshape(g.shape[:-1])
+    dct = sfft.dctn(c[tuple(slice(0, p) for p in P[::-1])], type=1) / c.size
+    expect = np.where(half < 0, 0.0, half)  # clamping of negatives (matrix.hh:289-290)
+    np.testing.assert_allclose(nat, expect, rtol=0, atol=1e-13 * np.abs(half).max())
+    np.testing.assert_allclose(np.where(dct < 0, 0, dct), expect, atol=1e-13 * np.abs(half).max())
+    # classification counts (matrix.hh:270-291) from the unclamped values
+    thr = 1e-14
+    assert st["negative"] == int(np.sum(half <= -thr))
+    assert st["zero"] == int(np.sum(np.abs(half) < thr))
+    assert st["small"] == int(np.sum((half >= thr) & (half < 1e-6)))
+    assert st["smallest"] == pytest.approx(half.min(), abs=1e-13 * np.abs(half).max())
+
+
+def test_eigs_are_circulant_spectrum():
+    # Eigenvalues of the dense circulant embedding matrix (eigvalsh) equal
+    # N_ext times the oracle's eigenvalues, expanded by even symmetry.
+    cells, corr = [4, 3], [0.4]
+    d = desc(2, cells, covariance="exponential", corr_length=corr)
+    v, _, _ = orc.eigs(d)
+    P = [c + 1 for c in cells]
+    N = [2 * c for c in cells]
+    half = v.reshape(P[1], P[0])
+    full = np.empty((N[1], N[0]))
+    for ky in range(N[1]):
+        for kx in range(N[0]):
+            full[ky, kx] = half[ky if ky < P[1] else N[1] - ky, kx if kx < P[0] else N[0] - kx]
+    g = nh.wrapped_grid(cells, [1.0, 1.0])
+    pts = g.reshape(-1, 2)
+    # circulant: entry (i, j) = c(wrap(x_i - x_j)) on the periodic extended grid
+    Lx, Ly = 2.0, 2.0
+    dlt = pts[:, None, :] - pts[None, :, :]
+    dlt[..., 0] = np.where(dlt[..., 0] > 0.5 * Lx, dlt[..., 0] - Lx, dlt[..., 0])
+    dlt[..., 0] = np.where(dlt[..., 0] < -0.5 * Lx, dlt[..., 0] + Lx, dlt[..., 0])
+    dlt[..., 1] = np.where(dlt[..., 1] > 0.5 * Ly, dlt[..., 1] - Ly, dlt[..., 1])
+    dlt[..., 1] = np.where(dlt[..., 1] < -0.5 * Ly, dlt[..., 1] + Ly, dlt[..., 1])
+    C = nh.cov_at(dlt.reshape(-1, 2), "exponential", "none", corr).reshape(len(pts), len(pts))
+    ev = np.linalg.eigvalsh(C)
+    np.testing.assert_allclose(np.sort(full.ravel() * full.size), ev, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,cells,corr", [
+    (2, [8, 6], [0.2, 0.05, -0.03, 0.1]),
+    (3, [4, 4, 6], [0.2, 0.02, 0.0, 0.01, 0.1, 0.02, 0.0, 0.01, 0.08]),
+])
+def test_eigs_r2c_vs_library(dim, cells, corr):
+    d = desc(dim, cells, covariance="gaussian", corr_length=corr, anisotropy="geometric",
+             approximate=True)
+    assert d.matrix_backend == pfb.MATRIX_BACKENDS["r2c"]
+    v, st, _ = orc.eigs(d, allow_negative=True)
+    s = orc.sizes(d)
+    N = [2 * c for c in cells]
+    H = N[0] // 2 + 1
+    spec = nh.full_eigs(cells, [1.0] * dim, "gaussian", "geometric", corr)[..., :H]
+    # storage order of the reference (3D: transposed, kx fastest then kz then ky)
+    store = spec if dim == 2 else spec.transpose(1, 0, 2)
+    store = np.ascontiguousarray(store)
+    # interleaved classification over the first H*N1*N2 reals (matrix.hh:270-291)
+    reals = store.view(np.float64).ravel().copy()
+    count = store.size
+    head = reals[:count]
+    thr = 1e-14
+    assert st["negative"] == int(np.sum(head <= -thr))
+    assert st["zero"] == int(np.sum(np.abs(head) < thr))
+    assert st["small"] == int(np.sum((head >= thr) & (head < 1e-6)))
+    head[head < 0] = 0.0
+    reals[:count] = head
+    want = reals.view(np.complex128).real
+    np.testing.assert_allclose(v, want, atol=1e-13 * np.abs(want).max())
+    assert s.transposed == (dim == 3)
+
+
+def test_negative_eigenvalues_raise_unless_approximate():
+    # Gaussian with a long correlation length and factor 2 embeds badly.
+    d = desc(2, [16, 16], covariance="gaussian", corr_length=0.5)
+    with pytest.raises(pfb.NegativeEigenvalueError):
+        orc.generate(d, np.zeros(orc.sizes(d).noise_per_field))
+    d.approximate = 1
+    v, st, status = orc.eigs(d)
+    assert status == 0 and st["negative"] > 0 and v.min() >= 0
+
+
+def test_reference_smoke_matrix_does_not_throw():
+    # test/fieldtest.cc:26-57: 2D 16x16 and 3D 8^3, corrLength 0.05, three families.
+    for dim, cells in [(2, [16, 16]), (3, [8, 8, 8])]:
+        for kind in ["exponential", "gaussian", "spherical"]:
+            d = desc(dim, cells, covariance=kind, corr_length=0.05)
+            n = orc.sizes(d).noise_per_field
+            f = orc.generate(d, orc.noise_cpp(1, n))
+            assert np.all(np.isfinite(f))
+
+
+# ------------------------------------------------------------ generation --
+
+def linear_map(d, nsamp):
+    """Field as a linear map of the noise: columns = fields of unit noise."""
+    cols = []
+    for j in range(nsamp):
+        z = np.zeros(nsamp)
+        z[j] = 1.0
+        cols.append(orc.generate(d, z))
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,matrix", [
+    (2, [4, 3], "exponential", [0.3], "none", "dct"),
+    (2, [4, 4], "gaussian", [0.2, 0.05, 0.0, 0.15], "geometric", "r2c"),
+    (3, [2, 3, 2], "exponential", [0.3], "none", "dct"),
+    (3, [2, 2, 2], "matern52", [0.3, 0.2, 0.1], "axiparallel", "r2c"),
+])
+def test_generate_r2c_covariance_identity(dim, cells, kind, corr, aniso, matrix):
+    """L L^T = C - D, where D is the variance lost on the kx in {0, N0/2}
+    planes because their +-k entries get independent noise (the reference's
+    c2r keeps only the Hermitian part of those planes)."""
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, matrix=matrix,
+             approximate=True)
+    s = orc.sizes(d)
+    L = linear_map(d, int(s.noise_per_field))
+    C = nh.dense_cov(cells, [1.0] * dim, kind, aniso, corr)
+    full = nh.full_eigs(cells, [1.0] * dim, kind, aniso, corr).real
+    full = np.maximum(full, 0.0) if matrix == "dct" else full
+    N = [2 * c for c in cells]
+    x = nh.cell_coords(cells, [1.0] * dim) / (np.array([1.0] * dim) / np.array(cells))  # int idx
+    D = np.zeros_like(C)
+    it = np.ndindex(*full.shape)
+    for kk in it:
+        k = kk[::-1]  # (kx, ky[, kz])
+        if k[0] not in (0, N[0] // 2):
+            continue
+        if all((2 * k[i]) % N[i] == 0 for i in range(dim)):
+            continue
+        ph = 2 * np.pi * sum(k[i] * x[:, i] / N[i] for i in range(dim))
+        D += 0.5 * full[kk] * np.cos(ph[:, None] - ph[None, :])
+    if matrix == "r2c":
+        # R2C eigen: full spectrum as stored (no clamping needed: approximate with
+        # the interleaved clamp would only touch negative values)
+        assert full.min() > -1e-12
+    np.testing.assert_allclose(L @ L.T, C - D, atol=1e-12)
+
+
+def test_generate_single_mode_noise_order_3d():
+    """One unit sample at the xi_1 slot of storage entry s produces
+    w * a * cos(2 pi k.x / N), with k the reference's transposed 3D order
+    (kx fastest, then kz, then ky; matrix.hh:374-404, r2cfieldbackend.hh:190-199)."""
+    cells = [2, 3, 4]
+    d = desc(3, cells, covariance="exponential", corr_length=0.3)
+    s = orc.sizes(d)
+    N = [2 * c for c in cells]
+    H = N[0] // 2 + 1
+    full = nh.full_eigs(cells, [1.0] * 3, "exponential", "none", [0.3]).real
+    x = np.stack(np.meshgrid(*[np.arange(c) for c in cells[::-1]], indexing="ij")[::-1],
+                 axis=-1).reshape(-1, 3)
+    for sidx in [0, 1, 2, 7, H + 1, H * N[2] + 3, H * N[2] * 2 + H * 3 + 2, 2 * H * N[2] - 1]:
+        kx, kz, ky = sidx % H, (sidx // H) % N[2], sidx // (H * N[2])
+        m = full[kz, ky, kx]
+        selfconj = all((2 * k) % n == 0 for k, n in zip((kx, ky, kz), N))
+        w = 1.0 if kx in (0, N[0] // 2) else 2.0
+        ph = 2 * np.pi * (kx * x[:, 0] / N[0] + ky * x[:, 1] / N[1] + kz * x[:, 2] / N[2])
+        for slot in (0, 1):
+            z = np.zeros(s.noise_per_field)
+            z[2 * sidx + slot] = 1.0
+            f = orc.generate(d, z)
+            if selfconj:
+                want = np.sqrt(m) * np.cos(ph) if slot == 0 else 0 * ph
+            else:
+                a = np.sqrt(m) / np.sqrt(2)
+                want = w * a * (np.cos(ph) if slot == 0 else -np.sin(ph))
+            np.testing.assert_allclose(f, want, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,tr", [
+    (2, [3, 4], "exponential", [0.3], "none", -1),
+    (2, [4, 4], "gaussian", [0.2, 0.1], "axiparallel", 0),
+    (3, [2, 3, 2], "exponential", [0.3], "none", -1),
+    (3, [3, 2, 2], "matern32", [0.2, 0.1, 0.3], "axiparallel", 0),
+])
+def test_generate_dctdst_covariance_identity(dim, cells, kind, corr, aniso, tr):
+    """The DCT/DST backend reproduces the embedded covariance exactly:
+    L L^T = C on the original grid."""
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, field="dctdst",
+             matrix="dct", transposed=tr, approximate=True)
+    s = orc.sizes(d)
+    L = linear_map(d, int(s.noise_per_field))
+    C = nh.dense_cov(cells, [1.0] * dim, kind, aniso, corr)
+    np.testing.assert_allclose(L @ L.T, C, atol=1e-12)
+
+
+# ------------------------------------------------------ matvec and CG ----
+
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,matrix", [
+    (2, [6, 5], "exponential", [0.2], "none", "dct"),
+    (2, [5, 4], "gaussian", [0.1, 0.02, 0.0, 0.08], "geometric", "r2c"),
+    (3, [3, 4, 3], "matern32", [0.2, 0.1, 0.1], "axiparallel", "dct"),
+])
+def test_matvec_is_dense_covariance_over_next(dim, cells, kind, corr, aniso, matrix):
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, matrix=matrix,
+             approximate=True)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0] * dim, kind, aniso, corr)
+    x = np.random.default_rng(3).standard_normal(C.shape[0])
+    np.testing.assert_allclose(orc.matvec(d, x), C @ x / s.extended_size, atol=1e-14)
+
+
+def test_root_and_inverse_are_exact_when_periodic():
+    d = desc(2, [8, 6], covariance="exponential", corr_length=0.1, periodic=True)
+    x = np.random.default_rng(4).standard_normal(48)
+    y = orc.matvec(d, x)
+    np.testing.assert_allclose(orc.matvec(d, orc.matvec(d, x, "root"), "root"), y, atol=1e-13)
+    np.testing.assert_allclose(orc.matvec(d, y, "inverse_embed"), x, atol=1e-10)
+
+
+def test_multiply_inverse_solves_dense_system():
+    cells = [6, 6]
+    d = desc(2, cells, covariance="exponential", corr_length=0.2)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0, 1.0], "exponential", "none", [0.2]) / s.extended_size
+    b = np.random.default_rng(5).standard_normal(36)
+    x, it = orc.multiply_inverse(d, b)
+    want = np.linalg.solve(C, b)
+    assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-6
+    assert 0 < it <= 100
+    # zero right-hand side returns the input unchanged (matrix.hh:193-196)
+    z = np.full(36, 1e-11)
+    x0, it0 = orc.multiply_inverse(d, z)
+    assert it0 == 0 and np.array_equal(x0, z)
+
+
+@pytest.mark.parametrize("mode", ["plain", "embed_pcg"])
+def test_pcg_solves_dense_system(mode):
+    cells = [6, 5, 4]
+    d = desc(3, cells, covariance="exponential", corr_length=0.15, nugget=1e-3)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0] * 3, "exponential", "none", [0.15])
+    C = (C + 1e-3 * np.eye(C.shape[0])) / s.extended_size
+    b = np.random.default_rng(6).standard_normal(C.shape[0])
+    x, it, rr = orc.pcg(d, b, mode=mode, maxit=500, rel_tol=1e-10)
+    assert rr <= 1e-10
+    want = np.linalg.solve(C, b)
+    assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-7
+
+
+# ----------------------------------------------------- value transforms --
+
+def test_value_transforms():
+    v = np.linspace(-2, 2, 41)
+    np.testing.assert_array_equal(orc.value_transform(v, "none"), v)
+    np.testing.assert_allclose(orc.value_transform(v, "logNormal"), np.exp(v), rtol=1e-15)
+    np.testing.assert_array_equal(orc.value_transform(v, "foldedNormal"), np.abs(v))
+    np.testing.assert_array_equal(orc.value_transform(v, "sign"), np.where(v < 0, -1.0, 1.0))
+    np.testing.assert_array_equal(orc.value_transform(v, "sign", 0.5),
+                                  np.where(v < 0.5, -1.0, 1.0))
+    lam = 0.3
+    np.testing.assert_allclose(orc.value_transform(v, "boxCox", lam),
+                               np.power(v * lam + 1, -lam), rtol=1e-14)
