@@ -225,6 +225,16 @@ int pf_philox_normals(void* plan, uint64_t seed, uint64_t realization, uint64_t 
 int pf_philox_raw(void* plan, const uint32_t ctr[4], const uint32_t key[2], int64_t nblocks,
                   uint32_t* out, int32_t mem);
 
+/* Observability (the analogue of the reference's randomField.verbose
+ * prints, fieldtraits.hh:180, matrix.hh:301-305): when on, every kernel
+ * launch of a call is bracketed by HIP events on the plan's stream and
+ * pf_last_timings returns the per-launch device times (ms) of the last call
+ * (names ';'-separated). Also switched on by the environment variable
+ * PF_PROFILE. No counterpart in the reference. */
+int pf_set_profiling(void* plan, int32_t on);
+int pf_last_timings(void* plan, char* names, int32_t names_len, double* ms, int32_t max_n,
+                    int32_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,7 +101,16 @@ void fft_rec(const cd* in, Index stride, cd* out, int n, int sign, const FFTPlan
   const int m = n / p;
   for (int q = 0; q < p; q++)
     fft_rec(in + q * stride, stride * p, out + q * m, m, sign, plan, twstep * p);
-  std::vector<cd> t(p), r(p);
+  cd tbuf[16], rbuf[16];
+  std::vector<cd> tv, rv;
+  cd* t = tbuf;
+  cd* r = rbuf;
+  if (p > 16) {
+    tv.resize(p);
+    rv.resize(p);
+    t = tv.data();
+    r = rv.data();
+  }
   for (int k = 0; k < m; k++) {
     for (int q = 0; q < p; q++) {
       cd w = plan.tw[(Index)q * k * twstep % plan.n];

@@ -1,0 +1,1272 @@
+// pf_api.hip -- plan objects and the extern "C" ABI of libparafields_amd.so.
+//
+// A plan is the device-side state of one reference `Matrix`
+// (include/parafields/matrix.hh:55-462): the configuration, the lazily built
+// FP64 embedding eigenvalues, the per-precision sqrt table and the work
+// buffers of the field transforms. All work of a plan runs on its own HIP
+// stream; calls on a plan are serialised by a mutex; every call returns after
+// its device work has completed.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pf_kernels.hpp"
+
+using namespace pfamd;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct PfError : std::runtime_error
+{
+  int code;
+  PfError(int c, const std::string& m)
+    : std::runtime_error(m)
+    , code(c)
+  {
+  }
+};
+
+#define HIPCHECK(x)                                                                                \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess)                                                                          \
+      throw PfError(PF_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));                      \
+  } while (0)
+
+struct DevBuf
+{
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b)
+  {
+    if (b <= bytes)
+      return;
+    release();
+    if (hipMalloc(&p, b) != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      throw PfError(PF_ENOMEM, "device allocation of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+  }
+  void release()
+  {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename X>
+  X* as() const
+  {
+    return reinterpret_cast<X*>(p);
+  }
+  ~DevBuf() { release(); }
+};
+
+Sched make_sched(int n)
+{
+  Sched s;
+  s.n = n;
+  s.nst = 0;
+  int m = n, p2 = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    p2++;
+  }
+  std::vector<int> r;
+  int n8 = p2 / 3, rem = p2 % 3;
+  if (rem == 1 && n8 > 0) {
+    n8--;
+    r.push_back(4);
+    r.push_back(4);
+  } else if (rem == 1)
+    r.push_back(2);
+  else if (rem == 2)
+    r.push_back(4);
+  for (int i = 0; i < n8; i++)
+    r.insert(r.begin(), 8);
+  for (int q : { 3, 5, 7 })
+    while (m % q == 0) {
+      m /= q;
+      r.push_back(q);
+    }
+  if (m != 1)
+    throw PfError(PF_EINVAL, "transform length " + std::to_string(n) +
+                               " has a prime factor above 7 (supported: 2, 3, 5, 7)");
+  if ((int)r.size() > kMaxStages)
+    throw PfError(PF_EINVAL, "transform length too long");
+  for (size_t i = 0; i < r.size(); i++)
+    s.rad[i] = r[i];
+  s.nst = (int)r.size();
+  return s;
+}
+
+template <typename T>
+constexpr int maxe()
+{
+  return sizeof(T) == 4 ? 32 : 16;
+}
+
+struct Cfg
+{
+  int L, threads;
+  size_t lds;
+};
+
+// Tile of L lines of length n (+ a twiddle table of ntab entries) within 64 KB
+// of LDS, one thread per MAXE elements.
+size_t lds_budget()
+{
+  static size_t budget = 0;
+  if (budget == 0) {
+    int dev = 0, v = 65536;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    budget = std::min<size_t>((size_t)v, 65536);
+    if (const char* e = getenv("PF_LDS_BUDGET"))
+      budget = std::min<size_t>((size_t)v, (size_t)atol(e));
+  }
+  return budget;
+}
+
+template <typename T>
+Cfg make_cfg(int n, int64_t nlines, int ntab)
+{
+  const size_t cs = sizeof(cplx<T>);
+  int L = sizeof(T) == 4 ? 16 : 8;
+  while (L > 1 && ((size_t)L * n + ntab) * cs > lds_budget())
+    L /= 2;
+  if (((size_t)L * n + ntab) * cs > lds_budget())
+    throw PfError(PF_EINVAL, "line length " + std::to_string(n) + " exceeds the LDS budget");
+  while (L > 1 && L / 2 >= nlines)
+    L /= 2;
+  const int me = maxe<T>();
+  int threads = (L * n + me - 1) / me;
+  threads = ((threads + 63) / 64) * 64;
+  if (threads < 64)
+    threads = 64;
+  if (threads > 1024)
+    throw PfError(PF_EINVAL, "line length " + std::to_string(n) + " too long for one workgroup");
+  Cfg c;
+  c.L = L;
+  c.threads = threads;
+  c.lds = (size_t)(ntab + (size_t)L * n) * cs;
+  return c;
+}
+
+inline unsigned nblocks(int64_t lines, int L)
+{
+  return (unsigned)((lines + L - 1) / L);
+}
+
+struct Geo
+{
+  int dim, factor, transposed;
+  int n[3], N[3], H, P[3];
+  double h[3], ext[3];
+  int64_t Next, domain, eig_entries, noise_per_field, half;
+};
+
+struct Timing
+{
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
+};
+
+struct Plan
+{
+  pf_desc d;
+  Geo g;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  bool eig_valid = false;
+  int eig_status = PF_OK;
+  pf_eig_stats st{};
+  DevBuf eig;       // FP64 eigenvalues (logical layout)
+  DevBuf sq32, sq64; // sqrt tables
+  DevBuf w1, w2, w3, noise, stage_in, stage_out, partial, scratch;
+  std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tw; // (precision bytes, n) -> table
+  bool profiling = false;
+  std::vector<std::pair<std::string, double>> last_timings;
+  std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> pending;
+
+  ~Plan()
+  {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(std::get<1>(p));
+      (void)hipEventDestroy(std::get<2>(p));
+    }
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  const cplx<T>* table(int n)
+  {
+    auto key = std::make_pair((int)sizeof(T), n);
+    auto it = tw.find(key);
+    if (it != tw.end())
+      return it->second->template as<cplx<T>>();
+    std::vector<cplx<T>> h(n);
+    for (int j = 0; j < n; j++) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
+      h[j] = { (T)cosl(a), (T)sinl(a) };
+    }
+    auto b = std::make_unique<DevBuf>();
+    b->ensure(n * sizeof(cplx<T>));
+    HIPCHECK(hipMemcpyAsync(b->p, h.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const cplx<T>* ptr = b->template as<cplx<T>>();
+    tw[key] = std::move(b);
+    return ptr;
+  }
+
+  // optional per-launch timing with events on the plan's stream
+  void mark_begin(const char* name, hipEvent_t& a, hipEvent_t& b)
+  {
+    if (!profiling)
+      return;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipEventRecord(a, stream));
+    pending.emplace_back(name, a, b);
+  }
+  void mark_end(hipEvent_t b)
+  {
+    if (!profiling)
+      return;
+    HIPCHECK(hipEventRecord(b, stream));
+  }
+  void collect_timings()
+  {
+    last_timings.clear();
+    for (auto& p : pending) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, std::get<1>(p), std::get<2>(p)));
+      last_timings.emplace_back(std::get<0>(p), (double)ms);
+      (void)hipEventDestroy(std::get<1>(p));
+      (void)hipEventDestroy(std::get<2>(p));
+    }
+    pending.clear();
+  }
+  void sync()
+  {
+    HIPCHECK(hipStreamSynchronize(stream));
+    collect_timings();
+  }
+};
+
+struct Scope
+{
+  Plan* p;
+  hipEvent_t a{}, b{};
+  Scope(Plan* p_, const char* name)
+    : p(p_)
+  {
+    p->mark_begin(name, a, b);
+  }
+  ~Scope() noexcept(false) { p->mark_end(b); }
+};
+
+void check_launch()
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw PfError(PF_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+Geo make_geo(const pf_desc& d)
+{
+  Geo g{};
+  if (d.dim != 2 && d.dim != 3)
+    throw PfError(PF_EINVAL, "dim must be 2 or 3");
+  g.dim = d.dim;
+  g.factor = d.periodic ? 1 : d.embedding_factor;
+  if (g.factor < 1)
+    throw PfError(PF_EINVAL, "embedding.factor must be >= 1");
+  g.Next = 1;
+  g.domain = 1;
+  for (int i = 0; i < 3; i++) {
+    if (i < d.dim) {
+      if (d.cells[i] < 1)
+        throw PfError(PF_EINVAL, "grid.cells must be positive");
+      g.n[i] = d.cells[i];
+      g.N[i] = g.factor * d.cells[i];
+      g.ext[i] = d.extensions[i];
+      g.h[i] = d.extensions[i] / d.cells[i];
+    } else {
+      g.n[i] = g.N[i] = 1;
+      g.ext[i] = g.h[i] = 1.;
+    }
+    g.P[i] = (i < d.dim) ? g.N[i] / 2 + 1 : 1;
+    g.Next *= g.N[i];
+    g.domain *= g.n[i];
+  }
+  if (g.N[0] % 2 != 0)
+    throw PfError(PF_EINVAL, "the extended x size must be even");
+  g.H = g.N[0] / 2 + 1;
+  if (d.transposed >= 0)
+    g.transposed = d.transposed != 0;
+  else if (d.dim == 2)
+    g.transposed = !(d.matrix_backend == PF_MATRIX_R2C || d.field_backend == PF_FIELD_R2C);
+  else
+    g.transposed = 1;
+  if (d.field_backend == PF_FIELD_R2C && d.dim == 2 && g.transposed)
+    throw PfError(PF_EINVAL, "R2CFieldBackend supports transposed output only for dim > 2");
+  if (d.matrix_backend != PF_MATRIX_DCT && d.matrix_backend != PF_MATRIX_R2C)
+    throw PfError(PF_EINVAL, "unknown matrix backend");
+  if (d.field_backend != PF_FIELD_R2C && d.field_backend != PF_FIELD_DCTDST)
+    throw PfError(PF_EINVAL, "unknown field backend");
+  if (d.field_backend == PF_FIELD_DCTDST && d.matrix_backend != PF_MATRIX_DCT)
+    throw PfError(PF_EINVAL, "DCTDSTFieldBackend requires DCTMatrixBackend");
+  if (d.covariance < 0 || d.covariance > PF_COV_CUSTOM)
+    throw PfError(PF_EINVAL, "covariance structure not known");
+  if (d.covariance == PF_COV_CUBIC && d.dim == 3)
+    throw PfError(PF_EINVAL, "cubic covariance only applicable in 1D or 2D");
+  if (d.covariance == PF_COV_GAMMA_EXPONENTIAL && (d.exp_gamma < 0. || d.exp_gamma > 2.))
+    throw PfError(PF_EINVAL, "exponent gamma has to be between 0 and 2");
+  if (d.covariance == PF_COV_GENERALIZED_CAUCHY &&
+      (d.cauchy_alpha <= 0. || d.cauchy_alpha > 2. || d.cauchy_beta <= 0.))
+    throw PfError(PF_EINVAL, "generalized Cauchy parameters out of range");
+  if (d.covariance == PF_COV_MATERN && d.matern_nu < 0.)
+    throw PfError(PF_EINVAL, "matern nu has to be positive");
+  if (d.anisotropy < 0 || d.anisotropy > 2)
+    throw PfError(PF_EINVAL, "stochastic.anisotropy must be none, axiparallel or geometric");
+  if (d.precision != PF_FP64 && d.precision != PF_FP32)
+    throw PfError(PF_EINVAL, "precision must be PF_FP64 or PF_FP32");
+  const int64_t half = (int64_t)g.H * g.N[1] * g.N[2];
+  const int64_t P = (int64_t)g.P[0] * g.P[1] * g.P[2];
+  g.half = half;
+  g.eig_entries = d.matrix_backend == PF_MATRIX_DCT ? P : half;
+  g.noise_per_field = d.field_backend == PF_FIELD_DCTDST ? (P << g.dim) : 2 * half;
+  // every transform length must factor into 2, 3, 5, 7
+  for (int i = 0; i < d.dim; i++)
+    (void)make_sched(g.N[i]);
+  (void)make_sched(g.N[0] / 2);
+  return g;
+}
+
+CovParams make_cov(const pf_desc& d, const Geo& g)
+{
+  CovParams p{};
+  p.dim = d.dim;
+  p.covariance = d.covariance;
+  p.variance = d.variance;
+  p.aniso = d.anisotropy;
+  if (p.aniso == PF_ANISO_NONE || d.covariance == PF_COV_CUSTOM) {
+    p.aniso = PF_ANISO_NONE;
+    p.m[0] = 1. / d.corr_length[0];
+  } else if (p.aniso == PF_ANISO_AXIPARALLEL) {
+    for (int i = 0; i < d.dim; i++)
+      p.m[i] = 1. / d.corr_length[i];
+  } else {
+    const double* c = d.corr_length;
+    if (d.dim == 3) {
+      p.m[0] = c[4] * c[8] - c[5] * c[7];
+      p.m[1] = c[2] * c[7] - c[1] * c[8];
+      p.m[2] = c[1] * c[5] - c[2] * c[4];
+      p.m[3] = c[5] * c[6] - c[3] * c[8];
+      p.m[4] = c[0] * c[8] - c[2] * c[6];
+      p.m[5] = c[2] * c[3] - c[0] * c[5];
+      p.m[6] = c[3] * c[7] - c[4] * c[6];
+      p.m[7] = c[1] * c[6] - c[0] * c[7];
+      p.m[8] = c[0] * c[4] - c[1] * c[3];
+      const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
+                         c[2] * (c[3] * c[7] - c[4] * c[6]);
+      for (int i = 0; i < 9; i++)
+        p.m[i] /= det;
+    } else {
+      p.m[0] = c[3];
+      p.m[1] = -c[1];
+      p.m[2] = -c[2];
+      p.m[3] = c[0];
+      const double det = c[0] * c[3] - c[1] * c[2];
+      for (int i = 0; i < 4; i++)
+        p.m[i] /= det;
+    }
+  }
+  p.nu = d.matern_nu;
+  p.sqrt2nu = std::sqrt(2. * d.matern_nu);
+  p.matern_c = d.covariance == PF_COV_MATERN ? std::pow(2., 1. - d.matern_nu) / std::tgamma(d.matern_nu)
+                                              : 0.;
+  p.gamma = d.exp_gamma;
+  p.alpha = d.cauchy_alpha;
+  p.beta = d.cauchy_beta;
+  p.nugget = d.nugget;
+  for (int i = 0; i < 3; i++) {
+    p.h[i] = g.h[i];
+    p.wrap_half[i] = 0.5 * g.ext[i] * (double)g.factor;
+    p.wrap_len[i] = g.ext[i] * (double)g.factor;
+  }
+  p.factor = g.factor;
+  return p;
+}
+
+GenGeom gen_geom(const Plan& P)
+{
+  const Geo& g = P.g;
+  GenGeom gg;
+  gg.dim = g.dim;
+  gg.transposed = g.transposed;
+  gg.N0 = g.N[0];
+  gg.N1 = g.N[1];
+  gg.N2 = g.N[2];
+  gg.H = g.H;
+  gg.n0 = g.n[0];
+  gg.n1 = g.n[1];
+  gg.n2 = g.n[2];
+  gg.P0 = g.P[0];
+  gg.P1 = g.P[1];
+  gg.P2 = g.P[2];
+  gg.dct = P.d.matrix_backend == PF_MATRIX_DCT;
+  gg.noise_stride = g.noise_per_field;
+  return gg;
+}
+
+unsigned grid1d(int64_t n, int threads = 256)
+{
+  int64_t b = (n + threads - 1) / threads;
+  if (b > 65536)
+    b = 65536;
+  if (b < 1)
+    b = 1;
+  return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------
+// Eigenvalues: Matrix::fillTransformedMatrix, matrix.hh:223-318.
+// ---------------------------------------------------------------------------
+pf_eig_stats reduce_partials(Plan& P, int nb)
+{
+  std::vector<ClassifyPartial> h(nb);
+  HIPCHECK(hipMemcpyAsync(h.data(), P.partial.p, nb * sizeof(ClassifyPartial),
+                          hipMemcpyDeviceToHost, P.stream));
+  HIPCHECK(hipStreamSynchronize(P.stream));
+  pf_eig_stats s{};
+  s.smallest = 1.7976931348623157e308;
+  for (auto& c : h) {
+    s.small += c.small;
+    s.zero += c.zero;
+    s.negative += c.negative;
+    s.smallest = std::min(s.smallest, c.smallest);
+  }
+  return s;
+}
+
+void compute_eigs(Plan& P, const double* custom_host)
+{
+  const Geo& g = P.g;
+  const pf_desc& d = P.d;
+  CovParams cp = make_cov(d, g);
+  const double inv_next = 1.0 / (double)g.Next;
+  const int nbp = 1024;
+  P.partial.ensure(nbp * sizeof(ClassifyPartial));
+  if (d.matrix_backend == PF_MATRIX_DCT) {
+    const int64_t total = (int64_t)g.P[0] * g.P[1] * g.P[2];
+    P.eig.ensure(total * sizeof(double));
+    double* a = P.eig.as<double>();
+    if (custom_host)
+      HIPCHECK(hipMemcpyAsync(a, custom_host, total * sizeof(double), hipMemcpyHostToDevice,
+                              P.stream));
+    else {
+      Scope sc(&P, "eig_cov");
+      k_eval_cov_dct<<<grid1d(total), 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+      check_launch();
+    }
+    int64_t stride = 1;
+    for (int ax = 0; ax < g.dim; ax++) {
+      const int Pn = g.P[ax], Nf = g.N[ax];
+      const int64_t nlines = total / Pn;
+      if (Pn > 1) {
+        Sched s = make_sched(Nf);
+        Cfg c = make_cfg<double>(Nf, nlines, Nf);
+        Scope sc(&P, "eig_redft00");
+        if (ax == 0)
+          k_redft00_axis<16, true><<<nblocks(nlines, c.L), c.threads, c.lds, P.stream>>>(
+            a, s, P.table<double>(Nf), Pn, stride, nlines, c.L);
+        else
+          k_redft00_axis<16, false><<<nblocks(nlines, c.L), c.threads, c.lds, P.stream>>>(
+            a, s, P.table<double>(Nf), Pn, stride, nlines, c.L);
+        check_launch();
+      }
+      stride *= Pn;
+    }
+    const unsigned nb = std::min<unsigned>(nbp, grid1d(total));
+    {
+      Scope sc(&P, "eig_classify");
+      k_scale_classify_dct<<<nb, 256, 0, P.stream>>>(a, total, inv_next, d.threshold,
+                                                    P.partial.as<ClassifyPartial>());
+      check_launch();
+    }
+    P.st = reduce_partials(P, nb);
+    P.st.entries = total;
+  } else {
+    const int64_t H = g.H, N0 = g.N[0], rows = (int64_t)g.N[1] * g.N[2];
+    const int64_t count = H * rows;
+    P.eig.ensure(count * sizeof(double));
+    P.scratch.ensure(std::max<size_t>(N0 * rows * sizeof(double), 0) +
+                     count * sizeof(cplx<double>));
+    double* real = P.scratch.as<double>();
+    cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + N0 * rows);
+    if (custom_host) {
+      // custom values arrive as the padded 2H x N1 x N2 real array
+      std::vector<double> compact(N0 * rows);
+      for (int64_t r = 0; r < rows; r++)
+        std::memcpy(&compact[r * N0], custom_host + r * 2 * H, N0 * sizeof(double));
+      HIPCHECK(hipMemcpyAsync(real, compact.data(), compact.size() * sizeof(double),
+                              hipMemcpyHostToDevice, P.stream));
+      HIPCHECK(hipStreamSynchronize(P.stream));
+    } else {
+      Scope sc(&P, "eig_cov");
+      k_eval_cov_full<<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      check_launch();
+    }
+    {
+      Sched s = make_sched((int)N0);
+      Cfg c = make_cfg<double>((int)N0, rows, (int)N0);
+      Scope sc(&P, "eig_r2c_x");
+      k_r2c_rows_full<16><<<nblocks(rows, c.L), c.threads, c.lds, P.stream>>>(
+        real, spec, s, P.table<double>((int)N0), rows, c.L);
+      check_launch();
+    }
+    int64_t stride = H;
+    for (int ax = 1; ax < g.dim; ax++) {
+      const int Nf = g.N[ax];
+      const int64_t nlines = count / Nf;
+      Sched s = make_sched(Nf);
+      Cfg c = make_cfg<double>(Nf, nlines, Nf);
+      Scope sc(&P, "eig_c2c");
+      k_c2c_strided<double, 16, false><<<dim3(nblocks(nlines, c.L), 1), c.threads, c.lds,
+                                         P.stream>>>(spec, s, P.table<double>(Nf), stride, nlines,
+                                                     c.L, 0);
+      check_launch();
+      stride *= Nf;
+    }
+    k_scale_complex<<<grid1d(count), 256, 0, P.stream>>>(spec, count, inv_next);
+    check_launch();
+    const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
+    k_classify_r2c<<<nb, 256, 0, P.stream>>>(spec, count, g.H, g.N[1], g.N[2],
+                                             (g.dim == 3 && g.transposed) ? 1 : 0, d.threshold,
+                                             P.partial.as<ClassifyPartial>());
+    check_launch();
+    P.st = reduce_partials(P, nb);
+    P.st.entries = count;
+    k_real_part<<<grid1d(count), 256, 0, P.stream>>>(spec, P.eig.as<double>(), count);
+    check_launch();
+    P.scratch.release();
+  }
+  // sqrt tables for generation
+  const int64_t ne = g.eig_entries;
+  if (d.precision == PF_FP32) {
+    P.sq32.ensure(ne * sizeof(float));
+    k_sqrt_table<float><<<grid1d(ne), 256, 0, P.stream>>>(P.eig.as<double>(), P.sq32.as<float>(), ne);
+  } else {
+    P.sq64.ensure(ne * sizeof(double));
+    k_sqrt_table<double>
+      <<<grid1d(ne), 256, 0, P.stream>>>(P.eig.as<double>(), P.sq64.as<double>(), ne);
+  }
+  check_launch();
+  P.sync();
+  P.eig_valid = true;
+  P.eig_status = (P.st.negative > 0 && !d.approximate) ? PF_ENEG : PF_OK;
+}
+
+void ensure_eigs(Plan& P)
+{
+  if (!P.eig_valid)
+    compute_eigs(P, nullptr);
+  if (P.eig_status != PF_OK)
+    throw PfError(PF_ENEG, "negative eigenvalues in covariance matrix");
+}
+
+// ---------------------------------------------------------------------------
+// Generation (R2C field backend)
+// ---------------------------------------------------------------------------
+template <typename T>
+void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
+                  int transform, double param, T* out)
+{
+  const Geo& g = P.g;
+  GenGeom gg = gen_geom(P);
+  const T* sq = sizeof(T) == 4 ? (const T*)P.sq32.p : (const T*)P.sq64.p;
+  const int dim = g.dim;
+  const int Nlast = dim == 3 ? g.N[2] : g.N[1];
+  const int nlast = dim == 3 ? g.n[2] : g.n[1];
+  const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
+  const size_t b1 = (size_t)nlinesG * nlast * sizeof(cplx<T>);
+  const size_t b2 = dim == 3 ? (size_t)g.H * g.n[1] * g.n[2] * sizeof(cplx<T>) : 0;
+  P.w1.ensure(b1 * batch);
+  if (dim == 3)
+    P.w2.ensure(b2 * batch);
+  constexpr int ME = maxe<T>();
+  {
+    Sched s = make_sched(Nlast);
+    Cfg c = make_cfg<T>(Nlast, nlinesG, Nlast);
+    dim3 grid(nblocks(nlinesG, c.L), batch);
+    Scope sc(&P, "gen_last");
+    if (dnoise)
+      k_gen_last<T, ME, 0><<<grid, c.threads, c.lds, P.stream>>>(
+        gg, sq, dnoise, seed, first, P.w1.as<cplx<T>>(), s, P.table<T>(Nlast), c.L);
+    else
+      k_gen_last<T, ME, 1><<<grid, c.threads, c.lds, P.stream>>>(
+        gg, sq, nullptr, seed, first, P.w1.as<cplx<T>>(), s, P.table<T>(Nlast), c.L);
+    check_launch();
+  }
+  const cplx<T>* xin = P.w1.as<cplx<T>>();
+  if (dim == 3) {
+    const int64_t nl = (int64_t)g.H * g.n[2];
+    Sched s = make_sched(g.N[1]);
+    Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+    dim3 grid(nblocks(nl, c.L), batch);
+    Scope sc(&P, "gen_mid");
+    k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, P.stream>>>(
+      P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
+      P.table<T>(g.N[1]), c.L);
+    check_launch();
+    xin = P.w2.as<cplx<T>>();
+  }
+  {
+    const int M = g.N[0] / 2;
+    const int64_t rows = (int64_t)g.n[1] * g.n[2];
+    Sched s = make_sched(M);
+    Cfg c = make_cfg<T>(M, rows, g.N[0]);
+    dim3 grid(nblocks(rows, c.L), batch);
+    Scope sc(&P, "gen_c2r_x");
+    k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, P.stream>>>(xin, out, rows, g.H, g.n[0], s,
+                                                           P.table<T>(g.N[0]), c.L, transform,
+                                                           (T)param);
+    check_launch();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Matvec (R2C field backend): multiplyExtended & co, matrix.hh:1443-1646
+// ---------------------------------------------------------------------------
+template <typename T>
+void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
+{
+  const Geo& g = P.g;
+  GenGeom gg = gen_geom(P);
+  const int dim = g.dim;
+  constexpr int ME = maxe<T>();
+  const int M = g.N[0] / 2;
+  const int64_t rows = (int64_t)g.n[1] * g.n[2];
+  const size_t c1 = (size_t)rows * g.H * sizeof(cplx<T>);
+  const size_t c2 = dim == 3 ? (size_t)g.H * g.N[1] * g.n[2] * sizeof(cplx<T>) : 0;
+  P.w1.ensure(c1 * batch);
+  if (dim == 3)
+    P.w2.ensure(c2 * batch);
+  {
+    Sched s = make_sched(M);
+    Cfg c = make_cfg<T>(M, rows, g.N[0]);
+    Scope sc(&P, "mv_r2c_x");
+    k_r2c_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
+      x, P.w1.as<cplx<T>>(), rows, g.H, g.n[0], s, P.table<T>(g.N[0]), c.L);
+    check_launch();
+  }
+  cplx<T>* C = P.w1.as<cplx<T>>();
+  if (dim == 3) {
+    const int64_t nl = (int64_t)g.H * g.n[2];
+    Sched s = make_sched(g.N[1]);
+    Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+    Scope sc(&P, "mv_fwd_mid");
+    k_pass_mid<T, ME, false><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
+      P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], s,
+      P.table<T>(g.N[1]), c.L);
+    check_launch();
+    C = P.w2.as<cplx<T>>();
+  }
+  {
+    const int Nlast = dim == 3 ? g.N[2] : g.N[1];
+    const int64_t nlines = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
+    Sched s = make_sched(Nlast);
+    Cfg c = make_cfg<T>(Nlast, nlines, Nlast);
+    Scope sc(&P, "mv_fused_mult");
+    k_fused_mult<T, ME><<<dim3(nblocks(nlines, c.L), batch), c.threads, c.lds, P.stream>>>(
+      C, gg, P.eig.as<double>(), op, 1.0 / (double)g.Next, s, P.table<T>(Nlast), c.L);
+    check_launch();
+  }
+  if (dim == 3) {
+    const int64_t nl = (int64_t)g.H * g.n[2];
+    Sched s = make_sched(g.N[1]);
+    Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+    Scope sc(&P, "mv_inv_mid");
+    k_pass_mid<T, ME, true><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
+      P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
+      P.table<T>(g.N[1]), c.L);
+    check_launch();
+  }
+  {
+    Sched s = make_sched(M);
+    Cfg c = make_cfg<T>(M, rows, g.N[0]);
+    Scope sc(&P, "mv_c2r_x");
+    k_c2r_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
+      P.w1.as<cplx<T>>(), y, rows, g.H, g.n[0], s, P.table<T>(g.N[0]), c.L, PF_TRANSFORM_NONE,
+      T(0));
+    check_launch();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Conjugate gradients (FP64 vectors and dot products on the device)
+// ---------------------------------------------------------------------------
+struct Vec
+{
+  DevBuf b;
+  double* p(int64_t n)
+  {
+    b.ensure(n * sizeof(double));
+    return b.as<double>();
+  }
+};
+
+double dot(Plan& P, const double* a, const double* b, int64_t n)
+{
+  const unsigned nb = std::min<unsigned>(1024, grid1d(n));
+  P.partial.ensure(1024 * sizeof(ClassifyPartial));
+  k_dot_partial<<<nb, 256, 0, P.stream>>>(a, b, n, P.partial.as<double>());
+  check_launch();
+  std::vector<double> h(nb);
+  HIPCHECK(hipMemcpyAsync(h.data(), P.partial.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                          P.stream));
+  HIPCHECK(hipStreamSynchronize(P.stream));
+  double s = 0.;
+  for (double v : h)
+    s += v;
+  return s;
+}
+
+double energy(Plan& P, const double* it, const double* ait, const double* b, int64_t n)
+{
+  const unsigned nb = std::min<unsigned>(1024, grid1d(n));
+  P.partial.ensure(1024 * sizeof(ClassifyPartial));
+  k_energy_partial<<<nb, 256, 0, P.stream>>>(it, ait, b, n, P.partial.as<double>());
+  check_launch();
+  std::vector<double> h(nb);
+  HIPCHECK(hipMemcpyAsync(h.data(), P.partial.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                          P.stream));
+  HIPCHECK(hipStreamSynchronize(P.stream));
+  double s = 0.;
+  for (double v : h)
+    s += v;
+  return s;
+}
+
+void matvec64(Plan& P, int op, const double* x, double* y)
+{
+  if (P.d.field_backend != PF_FIELD_R2C)
+    throw PfError(PF_EINVAL, "matvec is implemented for the R2C field backend");
+  matvec_r2c<double>(P, op, x, y, 1);
+}
+
+// Matrix::innerCG, matrix.hh:1315-1432, on device vectors.
+int inner_cg(Plan& P, double* iter, const double* solution, int64_t n, unsigned cgIterations)
+{
+  Vec vMS, vMI, vR, vPR, vD, vMD;
+  double* matrixTimesSolution = vMS.p(n);
+  double* matrixTimesIter = vMI.p(n);
+  double* residual = vR.p(n);
+  double* precResidual = vPR.p(n);
+  double* direction = vD.p(n);
+  double* matrixTimesDirection = vMD.p(n);
+  const unsigned gb = grid1d(n);
+  matvec64(P, PF_OP_MULTIPLY, solution, matrixTimesSolution); // unused product (matrix.hh:1329)
+  matvec64(P, PF_OP_MULTIPLY, iter, matrixTimesIter);
+  k_sub<<<gb, 256, 0, P.stream>>>(solution, matrixTimesIter, residual, n);
+  matvec64(P, PF_OP_INVERSE_EMBED, residual, precResidual);
+  HIPCHECK(hipMemcpyAsync(direction, precResidual, n * sizeof(double), hipMemcpyDeviceToDevice,
+                          P.stream));
+  bool converged = false;
+  double scalarProd = dot(P, precResidual, residual, n);
+  double scalarProd2 = dot(P, residual, residual, n);
+  if (std::sqrt(std::abs(scalarProd2)) < 1e-6)
+    converged = true;
+  const double firstValue = energy(P, iter, matrixTimesIter, solution, n);
+  unsigned count = 0;
+  while (!converged && count < cgIterations) {
+    matvec64(P, PF_OP_MULTIPLY, direction, matrixTimesDirection);
+    const double alphaDenominator = dot(P, direction, matrixTimesDirection, n);
+    const double alpha = scalarProd / alphaDenominator;
+    const double oldValue = energy(P, iter, matrixTimesIter, solution, n);
+    k_axpy<<<gb, 256, 0, P.stream>>>(alpha, direction, iter, n);
+    k_axpy<<<gb, 256, 0, P.stream>>>(alpha, matrixTimesDirection, matrixTimesIter, n);
+    const double value = energy(P, iter, matrixTimesIter, solution, n);
+    k_sub<<<gb, 256, 0, P.stream>>>(solution, matrixTimesIter, residual, n);
+    matvec64(P, PF_OP_INVERSE_EMBED, residual, precResidual);
+    double beta = 1. / scalarProd;
+    scalarProd = dot(P, precResidual, residual, n);
+    beta *= scalarProd;
+    k_xpby<<<gb, 256, 0, P.stream>>>(precResidual, beta, direction, n);
+    check_launch();
+    if (value != firstValue) {
+      if (std::abs(value - oldValue) / std::abs(value - firstValue) < 1e-16)
+        converged = true;
+    }
+    count++;
+  }
+  return (int)count;
+}
+
+// (P)CG with a relative-residual stop; the residual is the FP64 true residual
+// b - A x recomputed from a matvec of x every iteration.
+int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, double rel_tol,
+        double* relres)
+{
+  Vec vR, vZ, vP, vAp, vAx;
+  double* r = vR.p(n);
+  double* z = vZ.p(n);
+  double* p = vP.p(n);
+  double* Ap = vAp.p(n);
+  double* Ax = vAx.p(n);
+  const unsigned gb = grid1d(n);
+  HIPCHECK(hipMemsetAsync(x, 0, n * sizeof(double), P.stream));
+  HIPCHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
+  const double bnorm = std::sqrt(dot(P, b, b, n));
+  if (bnorm == 0.) {
+    *relres = 0.;
+    return 0;
+  }
+  auto prec = [&](const double* rr, double* zz) {
+    if (mode == PF_CG_EMBED_PCG)
+      matvec64(P, PF_OP_INVERSE_EMBED, rr, zz);
+    else
+      HIPCHECK(hipMemcpyAsync(zz, rr, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
+  };
+  prec(r, z);
+  HIPCHECK(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
+  double rz = dot(P, r, z, n);
+  int it = 0;
+  double rr = std::sqrt(dot(P, r, r, n)) / bnorm;
+  while (rr > rel_tol && it < maxit) {
+    matvec64(P, PF_OP_MULTIPLY, p, Ap);
+    const double alpha = rz / dot(P, p, Ap, n);
+    k_axpy<<<gb, 256, 0, P.stream>>>(alpha, p, x, n);
+    matvec64(P, PF_OP_MULTIPLY, x, Ax);
+    k_sub<<<gb, 256, 0, P.stream>>>(b, Ax, r, n);
+    check_launch();
+    prec(r, z);
+    const double rzn = dot(P, r, z, n);
+    const double beta = rzn / rz;
+    rz = rzn;
+    k_xpby<<<gb, 256, 0, P.stream>>>(z, beta, p, n);
+    check_launch();
+    it++;
+    rr = std::sqrt(dot(P, r, r, n)) / bnorm;
+  }
+  *relres = rr;
+  return it;
+}
+
+#define PF_GUARD(...)                                                                              \
+  try {                                                                                            \
+    __VA_ARGS__                                                                                    \
+  } catch (const PfError& e) {                                                                     \
+    g_err = e.what();                                                                              \
+    return e.code;                                                                                 \
+  } catch (const std::exception& e) {                                                              \
+    g_err = e.what();                                                                              \
+    return PF_EINVAL;                                                                              \
+  }
+
+size_t elem_size(const Plan& P) { return P.d.precision == PF_FP32 ? 4 : 8; }
+
+} // namespace
+
+extern "C" {
+
+const char* pf_last_error(void) { return g_err.c_str(); }
+
+int pf_plan_create(const pf_desc* desc, void** plan)
+{
+  PF_GUARD({
+    if (!desc || !plan)
+      throw PfError(PF_EINVAL, "null argument");
+    auto P = std::make_unique<Plan>();
+    P->d = *desc;
+    P->g = make_geo(*desc);
+    HIPCHECK(hipGetDevice(&P->device));
+    HIPCHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    if (getenv("PF_PROFILE"))
+      P->profiling = true;
+    *plan = P.release();
+    return PF_OK;
+  })
+}
+
+int pf_plan_destroy(void* plan)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    if (P) {
+      (void)hipStreamSynchronize(P->stream);
+      delete P;
+    }
+    return PF_OK;
+  })
+}
+
+int pf_plan_sizes(void* plan, pf_sizes* s)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    const Geo& g = P->g;
+    s->dim = g.dim;
+    for (int i = 0; i < 3; i++) {
+      s->cells[i] = g.n[i];
+      s->extended[i] = g.N[i];
+    }
+    s->transposed = g.transposed;
+    s->domain_size = g.domain;
+    s->extended_size = g.Next;
+    s->eig_entries = g.eig_entries;
+    s->noise_per_field = g.noise_per_field;
+    return PF_OK;
+  })
+}
+
+int pf_device_sync(void* plan)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    P->sync();
+    return PF_OK;
+  })
+}
+
+int pf_eigs(void* plan, pf_eig_stats* stats)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (!P->eig_valid)
+      compute_eigs(*P, nullptr);
+    if (stats)
+      *stats = P->st;
+    if (P->eig_status != PF_OK)
+      g_err = "negative eigenvalues in covariance matrix";
+    return P->eig_status;
+  })
+}
+
+int pf_eigs_custom(void* plan, const double* values, pf_eig_stats* stats)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    compute_eigs(*P, values);
+    if (stats)
+      *stats = P->st;
+    if (P->eig_status != PF_OK)
+      g_err = "negative eigenvalues in covariance matrix";
+    return P->eig_status;
+  })
+}
+
+int pf_eigs_copy(void* plan, double* out, int64_t n)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (!P->eig_valid)
+      compute_eigs(*P, nullptr);
+    const Geo& g = P->g;
+    if (n > g.eig_entries)
+      throw PfError(PF_EINVAL, "n exceeds the number of eigenvalues");
+    std::vector<double> logical(g.eig_entries);
+    HIPCHECK(hipMemcpy(logical.data(), P->eig.p, g.eig_entries * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    // reference storage order (3D transposed: x, z, y)
+    const bool tr = g.transposed;
+    if (P->d.matrix_backend == PF_MATRIX_DCT) {
+      int perm[3] = { 0, 1, 2 };
+      if (tr)
+        std::swap(perm[g.dim - 2], perm[g.dim - 1]);
+      for (int64_t s = 0; s < n; s++) {
+        int64_t k[3], r = s;
+        for (int j = 0; j < 3; j++) {
+          k[perm[j]] = r % g.P[perm[j]];
+          r /= g.P[perm[j]];
+        }
+        out[s] = logical[k[0] + g.P[0] * (k[1] + (int64_t)g.P[1] * k[2])];
+      }
+    } else {
+      for (int64_t s = 0; s < n; s++) {
+        int64_t kx = s % g.H, k1, k2;
+        if (g.dim == 3 && tr) {
+          k2 = (s / g.H) % g.N[2];
+          k1 = s / ((int64_t)g.H * g.N[2]);
+        } else {
+          k1 = (s / g.H) % g.N[1];
+          k2 = s / ((int64_t)g.H * g.N[1]);
+        }
+        out[s] = logical[kx + g.H * (k1 + (int64_t)g.N[1] * k2)];
+      }
+    }
+    return PF_OK;
+  })
+}
+
+int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_realization,
+                int32_t batch, int32_t transform, double transform_param, void* out,
+                int32_t out_mem)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (batch < 1)
+      throw PfError(PF_EINVAL, "batch must be >= 1");
+    if (transform < 0 || transform > PF_TRANSFORM_BOXCOX)
+      throw PfError(PF_EINVAL, "transform type not known");
+    if (P->d.field_backend != PF_FIELD_R2C)
+      throw PfError(PF_EINVAL, "field backend not supported by pf_generate yet");
+    ensure_eigs(*P);
+    const Geo& g = P->g;
+    const double* dn = nullptr;
+    if (noise) {
+      const size_t nb = (size_t)batch * g.noise_per_field * sizeof(double);
+      P->noise.ensure(nb);
+      HIPCHECK(hipMemcpyAsync(P->noise.p, noise, nb, hipMemcpyHostToDevice, P->stream));
+      dn = P->noise.as<double>();
+    }
+    const size_t es = elem_size(*P);
+    const size_t outb = (size_t)batch * g.domain * es;
+    void* dout = out;
+    if (out_mem == PF_MEM_HOST) {
+      P->stage_out.ensure(outb);
+      dout = P->stage_out.p;
+    }
+    if (P->d.precision == PF_FP32)
+      generate_r2c<float>(*P, dn, seed, first_realization, batch, transform, transform_param,
+                          (float*)dout);
+    else
+      generate_r2c<double>(*P, dn, seed, first_realization, batch, transform, transform_param,
+                           (double*)dout);
+    if (out_mem == PF_MEM_HOST)
+      HIPCHECK(hipMemcpyAsync(out, dout, outb, hipMemcpyDeviceToHost, P->stream));
+    P->sync();
+    return PF_OK;
+  })
+}
+
+int pf_matvec(void* plan, int32_t op, const void* x, void* y, int32_t batch, int32_t mem)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (op < 0 || op > PF_OP_INVERSE_EMBED)
+      throw PfError(PF_EINVAL, "unknown matvec operator");
+    if (P->d.field_backend != PF_FIELD_R2C)
+      throw PfError(PF_EINVAL, "matvec is implemented for the R2C field backend");
+    ensure_eigs(*P);
+    const Geo& g = P->g;
+    const size_t es = elem_size(*P);
+    const size_t nb = (size_t)batch * g.domain * es;
+    const void* dx = x;
+    void* dy = y;
+    if (mem == PF_MEM_HOST) {
+      P->stage_in.ensure(nb);
+      P->stage_out.ensure(nb);
+      HIPCHECK(hipMemcpyAsync(P->stage_in.p, x, nb, hipMemcpyHostToDevice, P->stream));
+      dx = P->stage_in.p;
+      dy = P->stage_out.p;
+    }
+    if (P->d.precision == PF_FP32)
+      matvec_r2c<float>(*P, op, (const float*)dx, (float*)dy, batch);
+    else
+      matvec_r2c<double>(*P, op, (const double*)dx, (double*)dy, batch);
+    if (mem == PF_MEM_HOST)
+      HIPCHECK(hipMemcpyAsync(y, dy, nb, hipMemcpyDeviceToHost, P->stream));
+    P->sync();
+    return PF_OK;
+  })
+}
+
+int pf_cg(void* plan, const double* b, double* x, int32_t mem, const pf_cg_opts* opts,
+          pf_cg_stats* stats)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (P->d.field_backend != PF_FIELD_R2C)
+      throw PfError(PF_EINVAL, "multiplyInverse is implemented for the R2C field backend");
+    ensure_eigs(*P);
+    const int64_t n = P->g.domain;
+    const int mode = opts ? opts->mode : PF_CG_REFERENCE;
+    const int maxit = (opts && opts->max_iterations > 0) ? opts->max_iterations
+                                                         : P->d.cg_iterations;
+    const double tol = opts ? opts->rel_tol : 1e-8;
+    if (mode == PF_CG_LEARNED_PCG)
+      throw PfError(PF_EINVAL, "learned preconditioner not available in this build");
+    // the CG runs in FP64 whatever the plan's field precision
+    if (P->d.precision != PF_FP64)
+      throw PfError(PF_EINVAL, "pf_cg needs a PF_FP64 plan (FP64 matvecs and dot products)");
+    Vec vb, vx;
+    double* db = vb.p(n);
+    double* dx = vx.p(n);
+    HIPCHECK(hipMemcpyAsync(db, b, n * sizeof(double),
+                            mem == PF_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                            P->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    int it = 0;
+    double rr = 0.;
+    if (mode == PF_CG_REFERENCE) {
+      // Matrix::multiplyInverse, matrix.hh:189-206
+      std::vector<double> hb(n);
+      HIPCHECK(hipMemcpyAsync(hb.data(), db, n * sizeof(double), hipMemcpyDeviceToHost,
+                              P->stream));
+      HIPCHECK(hipStreamSynchronize(P->stream));
+      bool fieldZero = true;
+      for (int64_t i = 0; i < n; i++)
+        if (std::abs(hb[i]) > 1e-10)
+          fieldZero = false;
+      if (fieldZero)
+        HIPCHECK(hipMemcpyAsync(dx, db, n * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
+      else {
+        matvec64(*P, PF_OP_INVERSE_EMBED, db, dx);
+        it = inner_cg(*P, dx, db, n, (unsigned)maxit);
+      }
+      Vec vax, vr;
+      double* ax = vax.p(n);
+      double* r = vr.p(n);
+      matvec64(*P, PF_OP_MULTIPLY, dx, ax);
+      k_sub<<<grid1d(n), 256, 0, P->stream>>>(db, ax, r, n);
+      const double bn = std::sqrt(dot(*P, db, db, n));
+      rr = bn > 0 ? std::sqrt(dot(*P, r, r, n)) / bn : 0.;
+    } else {
+      it = pcg(*P, db, dx, n, mode, maxit, tol, &rr);
+    }
+    P->sync();
+    auto t1 = std::chrono::steady_clock::now();
+    HIPCHECK(hipMemcpyAsync(x, dx, n * sizeof(double),
+                            mem == PF_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                            P->stream));
+    P->sync();
+    if (stats) {
+      stats->iterations = it;
+      stats->rel_residual = rr;
+      stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return PF_OK;
+  })
+}
+
+int pf_value_transform(void* plan, int32_t transform, double param, void* data, int64_t n,
+                       int32_t mem)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (transform < 0 || transform > PF_TRANSFORM_BOXCOX)
+      throw PfError(PF_EINVAL, "transform type not known");
+    const size_t es = elem_size(*P);
+    void* d = data;
+    if (mem == PF_MEM_HOST) {
+      P->stage_in.ensure(n * es);
+      HIPCHECK(hipMemcpyAsync(P->stage_in.p, data, n * es, hipMemcpyHostToDevice, P->stream));
+      d = P->stage_in.p;
+    }
+    if (es == 4)
+      k_value_transform<float><<<grid1d(n), 256, 0, P->stream>>>((float*)d, n, transform,
+                                                                 (float)param);
+    else
+      k_value_transform<double><<<grid1d(n), 256, 0, P->stream>>>((double*)d, n, transform,
+                                                                  param);
+    check_launch();
+    if (mem == PF_MEM_HOST)
+      HIPCHECK(hipMemcpyAsync(data, d, n * es, hipMemcpyDeviceToHost, P->stream));
+    P->sync();
+    return PF_OK;
+  })
+}
+
+int pf_philox_normals(void* plan, uint64_t seed, uint64_t realization, uint64_t first, int64_t n,
+                      double* out, int32_t mem)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    double* d = out;
+    if (mem == PF_MEM_HOST) {
+      P->stage_out.ensure(n * sizeof(double));
+      d = P->stage_out.as<double>();
+    }
+    k_philox_normals<<<grid1d(n), 256, 0, P->stream>>>(seed, realization, first, n, d);
+    check_launch();
+    if (mem == PF_MEM_HOST)
+      HIPCHECK(hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    P->sync();
+    return PF_OK;
+  })
+}
+
+int pf_philox_raw(void* plan, const uint32_t ctr[4], const uint32_t key[2], int64_t nblocks,
+                  uint32_t* out, int32_t mem)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    uint32_t* d = out;
+    const size_t nb = (size_t)nblocks * 4 * sizeof(uint32_t);
+    if (mem == PF_MEM_HOST) {
+      P->stage_out.ensure(nb);
+      d = P->stage_out.as<uint32_t>();
+    }
+    k_philox_raw<<<grid1d(nblocks), 256, 0, P->stream>>>(ctr[0], ctr[1], ctr[2], ctr[3], key[0],
+                                                         key[1], nblocks, d);
+    check_launch();
+    if (mem == PF_MEM_HOST)
+      HIPCHECK(hipMemcpyAsync(out, d, nb, hipMemcpyDeviceToHost, P->stream));
+    P->sync();
+    return PF_OK;
+  })
+}
+
+// Per-launch device times (ms) of the last call when profiling is enabled
+// (environment PF_PROFILE or pf_set_profiling): names and durations.
+int pf_set_profiling(void* plan, int32_t on)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    P->profiling = on != 0;
+    return PF_OK;
+  })
+}
+
+int pf_last_timings(void* plan, char* names, int32_t names_len, double* ms, int32_t max_n,
+                    int32_t* n_out)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    std::string all;
+    int n = 0;
+    for (auto& t : P->last_timings) {
+      if (n < max_n)
+        ms[n] = t.second;
+      all += t.first;
+      all += ';';
+      n++;
+    }
+    if (names && names_len > 0) {
+      std::strncpy(names, all.c_str(), names_len - 1);
+      names[names_len - 1] = 0;
+    }
+    *n_out = n;
+    return PF_OK;
+  })
+}
+
+} // extern "C"

@@ -1,0 +1,344 @@
+// pf_device.hpp -- device-side building blocks of the MI355X circulant-
+// embedding engine: complex arithmetic, the Philox4x32-10 normal stream,
+// covariance families and coordinate geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/parafields_amd.h"
+
+namespace pfamd {
+
+template <typename T>
+struct cplx
+{
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b)
+{
+  return { a.x + b.x, a.y + b.y };
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b)
+{
+  return { a.x - b.x, a.y - b.y };
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+  return { a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x };
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a)
+{
+  return { a.x, -a.y };
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s)
+{
+  return { a.x * s, a.y * s };
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). The normal stream of a
+// realization: sample s lives in block s/4 (counter = (block, realization)),
+// key = 64-bit seed; uniforms u = (x + 0.5) 2^-32; Box-Muller on (u0,u1) and
+// (u2,u3): n_a = sqrt(-2 ln u_a) cos(2 pi u_b), n_b = ... sin(2 pi u_b).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c[0];
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2];
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0;
+    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+// Pair of normals (xi_1, xi_2) = samples (2e, 2e+1) of the stream: one
+// Box-Muller pair of block e/2.
+template <typename T>
+struct BoxMuller;
+
+template <>
+struct BoxMuller<double>
+{
+  static __device__ __forceinline__ void pair(uint32_t xa, uint32_t xb, double& na, double& nb)
+  {
+    const double ua = ((double)xa + 0.5) * (1.0 / 4294967296.0);
+    const double ub = ((double)xb + 0.5) * (1.0 / 4294967296.0);
+    const double r = sqrt(-2.0 * log(ua));
+    double s, c;
+    sincospi(2.0 * ub, &s, &c);
+    na = r * c;
+    nb = r * s;
+  }
+};
+
+template <>
+struct BoxMuller<float>
+{
+  static __device__ __forceinline__ void pair(uint32_t xa, uint32_t xb, float& na, float& nb)
+  {
+    // u in (0, 1]: float of the 32-bit integer scaled by 2^-32 plus half an ulp
+    const float ua = fmaf((float)xa, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+    const float ub = fmaf((float)xb, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+    // ln(u) = log2(u) * ln 2 with the hardware log2
+    const float r = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(ua));
+    // hardware sin/cos take their argument in revolutions: sin(2 pi ub)
+    const float s = __builtin_amdgcn_sinf(ub);
+    const float c = __builtin_amdgcn_cosf(ub);
+    na = r * c;
+    nb = r * s;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ void philox_pair(uint64_t seed, uint64_t real, uint64_t entry, T& xi1,
+                                            T& xi2)
+{
+  const uint64_t blk = entry >> 1;
+  uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real, (uint32_t)(real >> 32) };
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  if (entry & 1)
+    BoxMuller<T>::pair(c[2], c[3], xi1, xi2);
+  else
+    BoxMuller<T>::pair(c[0], c[1], xi1, xi2);
+}
+
+// Single normal of sample index s.
+template <typename T>
+__device__ __forceinline__ T philox_normal(uint64_t seed, uint64_t real, uint64_t s)
+{
+  T a, b;
+  philox_pair<T>(seed, real, s >> 1, a, b);
+  return (s & 1) ? b : a;
+}
+
+// ---------------------------------------------------------------------------
+// Covariance model on the device (FP64), covariance.hh:18-823.
+// ---------------------------------------------------------------------------
+struct CovParams
+{
+  int dim;
+  int covariance;
+  int aniso;         // effective geometry kind
+  double variance;
+  double m[9];       // 1/l, 1/l_i or A^{-1} (row-major)
+  double nu, sqrt2nu, matern_c; // Matern: 2^{1-nu}/Gamma(nu)
+  double gamma, alpha, beta;
+  double nugget;
+  // embedding grid geometry (matrix.hh:615-638)
+  double h[3], wrap_half[3], wrap_len[3];
+  int factor;
+};
+
+// Modified Bessel K_nu(x) (Temme series for x < 2, Steed's CF2 otherwise,
+// forward recurrence in nu) -- the same method as the oracle's bessel_k.
+__device__ inline double bessel_k(double nu, double x)
+{
+  const double EPS = 1e-16, PI = 3.14159265358979323846;
+  int nl = (int)(nu + 0.5);
+  double xmu = nu - nl, xmu2 = xmu * xmu;
+  double xi = 1.0 / x, xi2 = 2.0 * xi;
+  double rkmu, rk1;
+  if (x < 2.0) {
+    double x2 = 0.5 * x, pimu = PI * xmu;
+    double fact = (fabs(pimu) < EPS ? 1.0 : pimu / sin(pimu));
+    double d = -log(x2), e = xmu * d;
+    double fact2 = (fabs(e) < EPS ? 1.0 : sinh(e) / e);
+    double gampl = 1.0 / tgamma(1.0 + xmu), gammi = 1.0 / tgamma(1.0 - xmu);
+    double gam1 = (fabs(xmu) < 1e-2)
+                    ? -(0.5772156649015329 - 0.0420026350340952 * xmu2 -
+                        0.0421977345555443 * xmu2 * xmu2 + 0.0072189432466630 * xmu2 * xmu2 * xmu2)
+                    : (gammi - gampl) / (2.0 * xmu);
+    double gam2 = 0.5 * (gammi + gampl);
+    double ff = fact * (gam1 * cosh(e) + gam2 * fact2 * d);
+    double sum = ff;
+    e = exp(e);
+    double p = 0.5 * e / gampl, q = 0.5 / (e * gammi), c = 1.0;
+    d = x2 * x2;
+    double sum1 = p;
+    for (int i = 1; i <= 10000; i++) {
+      ff = (i * ff + p + q) / (i * i - xmu2);
+      c *= (d / i);
+      p /= (i - xmu);
+      q /= (i + xmu);
+      double del = c * ff;
+      sum += del;
+      double del1 = c * (p - i * ff);
+      sum1 += del1;
+      if (fabs(del) < fabs(sum) * EPS)
+        break;
+    }
+    rkmu = sum;
+    rk1 = sum1 * xi2;
+  } else {
+    double b = 2.0 * (1.0 + x), d = 1.0 / b, h = d, delh = d;
+    double q1 = 0.0, q2 = 1.0, a1 = 0.25 - xmu2, q = a1, c = a1, a = -a1;
+    double s = 1.0 + q * delh;
+    for (int i = 2; i <= 10000; i++) {
+      a -= 2 * (i - 1);
+      c = -a * c / i;
+      double qnew = (q1 - b * q2) / a;
+      q1 = q2;
+      q2 = qnew;
+      q += c * qnew;
+      b += 2.0;
+      d = 1.0 / (b + a * d);
+      delh = (b * d - 1.0) * delh;
+      h += delh;
+      double dels = q * delh;
+      s += dels;
+      if (fabs(dels / s) < EPS)
+        break;
+    }
+    h = a1 * h;
+    rkmu = sqrt(PI / (2.0 * x)) * exp(-x) / s;
+    rk1 = rkmu * (xmu + x + 0.5 - h) * xi;
+  }
+  for (int i = 1; i <= nl; i++) {
+    double rktemp = (xmu + i) * xi2 * rk1 + rkmu;
+    rkmu = rk1;
+    rk1 = rktemp;
+  }
+  return rkmu;
+}
+
+// Covariance at the embedding grid position idx (integer cell offsets):
+// coordinates idx*h wrapped by the classical rule, geometry transform,
+// covariance functor (+ nugget at the origin).
+__device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
+{
+#pragma clang fp contract(off)
+  double coord[3] = { 0., 0., 0. }, t[3] = { 0., 0., 0. };
+  for (int i = 0; i < p.dim; i++) {
+    coord[i] = (double)idx[i] * p.h[i];
+    if (coord[i] > p.wrap_half[i])
+      coord[i] -= p.wrap_len[i];
+  }
+  if (p.aniso == PF_ANISO_NONE) {
+    for (int i = 0; i < p.dim; i++)
+      t[i] = p.m[0] * coord[i];
+  } else if (p.aniso == PF_ANISO_AXIPARALLEL) {
+    for (int i = 0; i < p.dim; i++)
+      t[i] = p.m[i] * coord[i];
+  } else {
+    for (int i = 0; i < p.dim; i++) {
+      t[i] = 0.;
+      for (int j = 0; j < p.dim; j++)
+        t[i] += p.m[i * p.dim + j] * coord[j];
+    }
+  }
+  double sum = 0.;
+  for (int i = 0; i < p.dim; i++)
+    sum += t[i] * t[i];
+  const double hh = sqrt(sum);
+  const double var = p.variance;
+  double v = 0.;
+  switch (p.covariance) {
+    case PF_COV_SPHERICAL:
+      if (hh > 1.)
+        v = 0.;
+      else if (p.dim == 3)
+        v = var * (1. - 1.5 * hh + 0.5 * pow(hh, 3.));
+      else
+        v = var * (1. - 2. / 3.14159265358979323846 * (hh * sqrt(1. - pow(hh, 2.)) + asin(hh)));
+      break;
+    case PF_COV_EXPONENTIAL:
+    case PF_COV_CUSTOM:
+      v = var * exp(-hh);
+      break;
+    case PF_COV_GAMMA_EXPONENTIAL:
+      v = var * exp(-pow(hh, p.gamma));
+      break;
+    case PF_COV_GAUSSIAN:
+      v = var * exp(-hh * hh);
+      break;
+    case PF_COV_SEPARABLE_EXPONENTIAL: {
+      double s = 0.;
+      for (int i = 0; i < p.dim; i++)
+        s += fabs(t[i]);
+      v = var * exp(-s);
+      break;
+    }
+    case PF_COV_MATERN:
+      if (hh < 1e-10)
+        v = var;
+      else
+        v = var * p.matern_c * pow(p.sqrt2nu * hh, p.nu) * bessel_k(p.nu, p.sqrt2nu * hh);
+      break;
+    case PF_COV_MATERN32:
+      v = var * (1. + sqrt(3.) * hh) * exp(-sqrt(3.) * hh);
+      break;
+    case PF_COV_MATERN52:
+      v = var * (1. + sqrt(5.) * hh + 5. / 3. * hh * hh) * exp(-sqrt(5.) * hh);
+      break;
+    case PF_COV_DAMPED_OSCILLATION:
+      v = (p.dim == 3) ? var * exp(-hh) * cos(hh / sqrt(3.)) : var * exp(-hh) * cos(hh);
+      break;
+    case PF_COV_CAUCHY:
+      v = var * pow(1. + pow(hh, 2.), -3.);
+      break;
+    case PF_COV_GENERALIZED_CAUCHY:
+      v = var * pow(1. + pow(hh, p.alpha), -p.beta);
+      break;
+    case PF_COV_CUBIC:
+      v = (hh > 1.) ? 0.
+                    : var * (1. - 7. * pow(hh, 2.) + 8.75 * pow(hh, 3.) - 3.5 * pow(hh, 5.) +
+                             0.75 * pow(hh, 7.));
+      break;
+    case PF_COV_WHITE_NOISE:
+      v = var;
+      for (int i = 0; i < p.dim; i++)
+        if (fabs(t[i]) > 1e-10)
+          v = 0.;
+      break;
+    default:
+      v = 0.;
+  }
+  if (p.nugget != 0.) {
+    bool origin = true;
+    for (int i = 0; i < p.dim; i++)
+      if (t[i] != 0.)
+        origin = false;
+    if (origin)
+      v += p.nugget;
+  }
+  return v;
+}
+
+// Value transforms of mutators.hh:35-163 (sign threshold is a parameter,
+// 0 in the reference).
+template <typename T>
+__device__ __forceinline__ T value_transform(int kind, T v, T param)
+{
+  switch (kind) {
+    case PF_TRANSFORM_LOGNORMAL:
+      return exp(v);
+    case PF_TRANSFORM_FOLDEDNORMAL:
+      return fabs(v);
+    case PF_TRANSFORM_SIGN:
+      return (v < param) ? T(-1) : T(1);
+    case PF_TRANSFORM_BOXCOX:
+      return pow(v * param + T(1), -param);
+    default:
+      return v;
+  }
+}
+
+} // namespace pfamd

@@ -1,0 +1,666 @@
+// pf_kernels.hpp -- the HIP kernels of the engine (gfx950).
+//
+// Layout in HBM (x fastest everywhere, realization b outermost):
+//   eigenvalues   FP64, DCT backend [P2][P1][P0] (P_i = N_i/2+1),
+//                 R2C backend [N2][N1][H] (H = N0/2+1, real part)
+//   sqrt table    T, same layout as the eigenvalues
+//   spectrum B1   complex T [n_last][N_mid][H]     (after the last-axis pass)
+//   spectrum B2   complex T [n2][n1][H]            (3D, after the y pass)
+//   field         T [n2][n1][n0]
+#pragma once
+
+#include "pf_fft.hpp"
+
+namespace pfamd {
+
+// ---------------------------------------------------------------------------
+// Eigenvalues
+// ---------------------------------------------------------------------------
+
+// Covariance on the DCT half grid [P2][P1][P0] (dctmatrixbackend.hh:164-170 via
+// matrix.hh:615-638).
+__global__ void k_eval_cov_dct(CovParams p, int P0, int P1, int P2, double* out)
+{
+  const int64_t total = (int64_t)P0 * P1 * P2;
+  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int64_t idx[3] = { f % P0, (f / P0) % P1, f / ((int64_t)P0 * P1) };
+    out[f] = embedded_cov(p, idx);
+  }
+}
+
+// Covariance on the full real grid [N2][N1][N0] (r2cmatrixbackend.hh:150-170;
+// the r2c padding entries are never read by the transform, so they are not
+// stored).
+__global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out)
+{
+  const int64_t total = (int64_t)N0 * N1 * N2;
+  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int64_t idx[3] = { f % N0, (f / N0) % N1, f / ((int64_t)N0 * N1) };
+    out[f] = embedded_cov(p, idx);
+  }
+}
+
+// REDFT00 (FFTW's DCT-I) along one axis of a real FP64 array, computed as the
+// DFT of the even extension of length Nfft = 2(P-1); in place. Lines of the
+// axis: stride `stride`, P points. LM = true for the contiguous axis.
+template <int MAXE, bool LM>
+__global__ void k_redft00_axis(double* a, Sched s, const cplx<double>* tw, int P, int64_t stride,
+                               int64_t nlines, int L)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = s.n;
+  cplx<double>* twl = reinterpret_cast<cplx<double>*>(smem);
+  cplx<double>* buf = twl + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    twl[i] = tw[i];
+  const int64_t line0 = (int64_t)blockIdx.x * L;
+  const int se = LM ? 1 : L, sl = LM ? N : 1;
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    int l, e;
+    if (LM) {
+      l = idx / N;
+      e = idx % N;
+    } else {
+      l = idx % L;
+      e = idx / L;
+    }
+    const int64_t u = line0 + l;
+    double v = 0.;
+    if (u < nlines) {
+      const int64_t inner = u % stride, outer = u / stride;
+      const int64_t base = inner + outer * stride * P;
+      const int src = e < P ? e : N - e;
+      v = a[base + src * stride];
+    }
+    buf[e * se + l * sl] = { v, 0. };
+  }
+  __syncthreads();
+  fft_lds<double, MAXE, LM, false>(buf, twl, 1, s, L, se, sl);
+  for (int idx = threadIdx.x; idx < L * P; idx += blockDim.x) {
+    int l, e;
+    if (LM) {
+      l = idx / P;
+      e = idx % P;
+    } else {
+      l = idx % L;
+      e = idx / L;
+    }
+    const int64_t u = line0 + l;
+    if (u < nlines) {
+      const int64_t inner = u % stride, outer = u / stride;
+      const int64_t base = inner + outer * stride * P;
+      a[base + e * stride] = buf[e * se + l * sl].x;
+    }
+  }
+}
+
+// Real rows of length N0 -> first H = N0/2+1 complex outputs of the forward
+// DFT (full complex FFT of the real row), FP64.
+template <int MAXE>
+__global__ void k_r2c_rows_full(const double* in, cplx<double>* out, Sched s,
+                                const cplx<double>* tw, int64_t rows, int L)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = s.n, H = N / 2 + 1;
+  cplx<double>* twl = reinterpret_cast<cplx<double>*>(smem);
+  cplx<double>* buf = twl + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    twl[i] = tw[i];
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx / N, e = idx % N;
+    const int64_t r = row0 + l;
+    buf[l * N + e] = { r < rows ? in[r * N + e] : 0., 0. };
+  }
+  __syncthreads();
+  fft_lds<double, MAXE, true, false>(buf, twl, 1, s, L, 1, N);
+  for (int idx = threadIdx.x; idx < L * H; idx += blockDim.x) {
+    const int l = idx / H, e = idx % H;
+    const int64_t r = row0 + l;
+    if (r < rows)
+      out[r * H + e] = buf[l * N + e];
+  }
+}
+
+// Complex c2c along a strided axis, in place (lines of n points at
+// `stride`), unnormalised. Works for any T.
+template <typename T, int MAXE, bool INV>
+__global__ void k_c2c_strided(cplx<T>* a, Sched s, const cplx<T>* tw, int64_t stride,
+                              int64_t nlines, int L, int64_t bstride)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = s.n;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* buf = twl + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    twl[i] = tw[i];
+  a += blockIdx.y * bstride;
+  const int64_t line0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx % L, e = idx / L;
+    const int64_t u = line0 + l;
+    cplx<T> v = { 0, 0 };
+    if (u < nlines) {
+      const int64_t inner = u % stride, outer = u / stride;
+      v = a[inner + outer * stride * N + e * stride];
+    }
+    buf[e * L + l] = v;
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, false, INV>(buf, twl, 1, s, L, L, 1);
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx % L, e = idx / L;
+    const int64_t u = line0 + l;
+    if (u < nlines) {
+      const int64_t inner = u % stride, outer = u / stride;
+      a[inner + outer * stride * N + e * stride] = buf[e * L + l];
+    }
+  }
+}
+
+struct ClassifyPartial
+{
+  long long small, zero, negative;
+  double smallest;
+};
+
+__device__ __forceinline__ void classify_one(double& v, double thr, ClassifyPartial& c)
+{
+  if (v < c.smallest)
+    c.smallest = v;
+  if (v < 1e-6) {
+    if (v < thr) {
+      if (v > -thr)
+        c.zero++;
+      else
+        c.negative++;
+    } else
+      c.small++;
+  }
+  if (v < 0.)
+    v = 0.;
+}
+
+__device__ void block_reduce_write(ClassifyPartial c, ClassifyPartial* out)
+{
+  __shared__ long long s_small[1024 / 64], s_zero[1024 / 64], s_neg[1024 / 64];
+  __shared__ double s_min[1024 / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    c.small += __shfl_down(c.small, off);
+    c.zero += __shfl_down(c.zero, off);
+    c.negative += __shfl_down(c.negative, off);
+    c.smallest = fmin(c.smallest, __shfl_down(c.smallest, off));
+  }
+  const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+    s_small[wid] = c.small;
+    s_zero[wid] = c.zero;
+    s_neg[wid] = c.negative;
+    s_min[wid] = c.smallest;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ClassifyPartial r = { 0, 0, 0, 1.7976931348623157e308 };
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++) {
+      r.small += s_small[w];
+      r.zero += s_zero[w];
+      r.negative += s_neg[w];
+      r.smallest = fmin(r.smallest, s_min[w]);
+    }
+    out[blockIdx.x] = r;
+  }
+}
+
+// Divide by N_ext and classify/clamp every entry (DCT backend; order free),
+// matrix.hh:266-291 with dctmatrixbackend.hh:240-272.
+__global__ void k_scale_classify_dct(double* a, int64_t n, double inv_next, double thr,
+                                     ClassifyPartial* partial)
+{
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = a[i] * inv_next;
+    classify_one(v, thr, c);
+    a[i] = v;
+  }
+  block_reduce_write(c, partial);
+}
+
+// R2C backend: divide the complex spectrum [N2][N1][H] by N_ext.
+__global__ void k_scale_complex(cplx<double>* a, int64_t n, double inv_next)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    a[i].x *= inv_next;
+    a[i].y *= inv_next;
+  }
+}
+
+// R2C backend classification over the first `count` reals of the interleaved
+// complex buffer in the reference's storage order (r2cmatrixbackend.hh:378-400
+// read through matrix.hh:270-291): real i is component i%2 of storage entry
+// i/2; 3D transposed storage is (kx, kz, ky).
+__global__ void k_classify_r2c(cplx<double>* spec, int64_t count, int H, int N1, int N2,
+                               int transposed3d, double thr, ClassifyPartial* partial)
+{
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i >> 1;
+    int64_t logical = s;
+    if (transposed3d) {
+      const int64_t kx = s % H, kz = (s / H) % N2, ky = s / ((int64_t)H * N2);
+      logical = kx + (int64_t)H * (ky + (int64_t)N1 * kz);
+    }
+    double* p = reinterpret_cast<double*>(&spec[logical]) + (i & 1);
+    double v = *p;
+    classify_one(v, thr, c);
+    *p = v;
+  }
+  block_reduce_write(c, partial);
+}
+
+__global__ void k_real_part(const cplx<double>* spec, double* out, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = spec[i].x;
+}
+
+template <typename T>
+__global__ void k_sqrt_table(const double* eig, T* out, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (T)sqrt(eig[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Field generation, R2C field backend (matrix.hh:374-421,
+// r2cfieldbackend.hh:250-325, :423-441).
+// ---------------------------------------------------------------------------
+struct GenGeom
+{
+  int dim, transposed;
+  int N0, N1, N2, H;
+  int n0, n1, n2;
+  int P0, P1, P2;       // DCT half sizes (eigen layout of the DCT backend)
+  int dct;              // 1: eigen table is the DCT half array (fold k >= P)
+  int64_t noise_stride; // samples per realization
+};
+
+// Reference storage index of half-spectrum entry (kx, k1, k2): the order in
+// which generateField draws the noise.
+__device__ __forceinline__ int64_t storage_index(const GenGeom& g, int kx, int k1, int k2)
+{
+  if (g.dim == 3 && g.transposed)
+    return kx + (int64_t)g.H * (k2 + (int64_t)g.N2 * k1);
+  return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
+}
+
+__device__ __forceinline__ int64_t eig_index(const GenGeom& g, int kx, int k1, int k2)
+{
+  if (g.dct) {
+    const int a = kx >= g.P0 ? g.N0 - kx : kx;
+    const int b = k1 >= g.P1 ? g.N1 - k1 : k1;
+    const int c = k2 >= g.P2 ? g.N2 - k2 : k2;
+    return a + (int64_t)g.P0 * (b + (int64_t)g.P1 * c);
+  }
+  return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
+}
+
+// Spectrum entry: R2CFieldBackend::set, r2cfieldbackend.hh:305-325.
+template <typename T, int NOISE>
+__device__ __forceinline__ cplx<T> spectrum_entry(const GenGeom& g, const T* sq, int kx, int k1,
+                                                  int k2, const double* noise, uint64_t seed,
+                                                  uint64_t real)
+{
+  const int64_t s = storage_index(g, kx, k1, k2);
+  const T lambda = sq[eig_index(g, kx, k1, k2)];
+  T r1, r2;
+  if (NOISE == 0) {
+    r1 = (T)noise[2 * s];
+    r2 = (T)noise[2 * s + 1];
+  } else {
+    philox_pair<T>(seed, real, (uint64_t)s, r1, r2);
+  }
+  const bool allMultiple = ((2 * kx) % g.N0 == 0) && ((2 * k1) % g.N1 == 0) &&
+                           ((2 * k2) % g.N2 == 0);
+  if (allMultiple)
+    return { lambda * r1, T(0) };
+  const T ls = lambda / (T)1.41421356237309504880;
+  return { ls * r1, ls * r2 };
+}
+
+// Pass G: spectrum of line (kx, k_mid) along the last axis, generated in
+// registers, inverse DFT, first n_last outputs -> B1[j][k_mid][kx].
+template <typename T, int MAXE, int NOISE>
+__global__ void k_gen_last(GenGeom g, const T* sq, const double* noise, uint64_t seed,
+                           uint64_t first_real, cplx<T>* B1, Sched s, const cplx<T>* tw, int L)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = s.n;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* buf = twl + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    twl[i] = tw[i];
+  const int b = blockIdx.y;
+  const int64_t nlines = (g.dim == 3) ? (int64_t)g.H * g.N1 : g.H;
+  const int nlast = (g.dim == 3) ? g.n2 : g.n1;
+  const int64_t line0 = (int64_t)blockIdx.x * L;
+  const double* nz = noise ? noise + b * g.noise_stride : nullptr;
+  const uint64_t real = first_real + b;
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx % L, e = idx / L;
+    const int64_t u = line0 + l;
+    cplx<T> z = { 0, 0 };
+    if (u < nlines) {
+      const int kx = (int)(u % g.H), km = (int)(u / g.H);
+      if (g.dim == 3)
+        z = spectrum_entry<T, NOISE>(g, sq, kx, km, e, nz, seed, real);
+      else
+        z = spectrum_entry<T, NOISE>(g, sq, kx, e, 0, nz, seed, real);
+    }
+    buf[e * L + l] = z;
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, false, true>(buf, twl, 1, s, L, L, 1);
+  cplx<T>* out = B1 + (int64_t)b * nlines * nlast;
+  for (int idx = threadIdx.x; idx < L * nlast; idx += blockDim.x) {
+    const int l = idx % L, j = idx / L;
+    const int64_t u = line0 + l;
+    if (u < nlines)
+      out[j * nlines + u] = buf[j * L + l];
+  }
+}
+
+// Pass Y (3D): lines (kx, z) along ky of B1[z][ky][kx]; inverse DFT (or
+// forward with FWD); first n_out outputs -> B2[z][y][kx]. The input holds
+// n_in <= N1 leading points (rest zero).
+template <typename T, int MAXE, bool INV>
+__global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int nz, int in_rows,
+                           int n_in, int out_rows, int n_out, Sched s, const cplx<T>* tw, int L)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = s.n;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* buf = twl + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    twl[i] = tw[i];
+  const int b = blockIdx.y;
+  const int64_t nlines = (int64_t)H * nz;
+  in += (int64_t)b * nlines * in_rows;
+  out += (int64_t)b * nlines * out_rows;
+  const int64_t line0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx % L, e = idx / L;
+    const int64_t u = line0 + l;
+    cplx<T> v = { 0, 0 };
+    if (u < nlines && e < n_in) {
+      const int64_t kx = u % H, z = u / H;
+      v = in[(z * in_rows + e) * H + kx];
+    }
+    buf[e * L + l] = v;
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, false, INV>(buf, twl, 1, s, L, L, 1);
+  for (int idx = threadIdx.x; idx < L * n_out; idx += blockDim.x) {
+    const int l = idx % L, j = idx / L;
+    const int64_t u = line0 + l;
+    if (u < nlines) {
+      const int64_t kx = u % H, z = u / H;
+      out[(z * out_rows + j) * H + kx] = buf[j * L + l];
+    }
+  }
+}
+
+// Pass X: 1-D c2r along x of rows of H complex (FFTW c2r semantics: Re of the
+// DC and Nyquist entries), as an M = N0/2 point complex inverse FFT of
+// Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N0} (X_k - conj X_{M-k});
+// out[2j] = Re z_j, out[2j+1] = Im z_j; first n0 values, value transform.
+// tw is the N0-point table (the M-point FFT uses every second entry).
+template <typename T, typename TO, int MAXE>
+__global__ void k_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H, int n0, Sched s,
+                        const cplx<T>* tw, int L, int transform, T param)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int M = s.n, N0 = 2 * M;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* buf = twl + N0;
+  for (int i = threadIdx.x; i < N0; i += blockDim.x)
+    twl[i] = tw[i];
+  __syncthreads();
+  const int b = blockIdx.y;
+  in += (int64_t)b * rows * H;
+  out += (int64_t)b * rows * n0;
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * M; idx += blockDim.x) {
+    const int l = idx / M, k = idx % M;
+    const int64_t r = row0 + l;
+    cplx<T> z = { 0, 0 };
+    if (r < rows) {
+      cplx<T> xk = in[r * H + k];
+      cplx<T> xm = in[r * H + (M - k)];
+      if (k == 0) {
+        xk.y = 0;
+        xm.y = 0;
+      }
+      const cplx<T> a = { xk.x + xm.x, xk.y - xm.y };  // X_k + conj X_{M-k}
+      const cplx<T> d = { xk.x - xm.x, xk.y + xm.y };  // X_k - conj X_{M-k}
+      const cplx<T> t = cconj(twl[k]);                 // e^{+2 pi i k / N0}
+      const cplx<T> td = cmul(t, d);
+      z = { a.x - td.y, a.y + td.x };
+    }
+    buf[l * M + k] = z;
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, true, true>(buf, twl, 2, s, L, 1, M);
+  for (int idx = threadIdx.x; idx < L * n0; idx += blockDim.x) {
+    const int l = idx / n0, x = idx % n0;
+    const int64_t r = row0 + l;
+    if (r < rows) {
+      const cplx<T> v = buf[l * M + (x >> 1)];
+      out[r * n0 + x] = (TO)value_transform<T>(transform, (x & 1) ? v.y : v.x, param);
+    }
+  }
+}
+
+// Pass FX (matvec input): zero-padded real rows (n0 of N0 values) -> H
+// complex via an M-point FFT of z_j = x_{2j} + i x_{2j+1}:
+// X_k = (Z_k + conj Z_{M-k})/2 - i e^{-2 pi i k/N0} (Z_k - conj Z_{M-k})/2.
+template <typename T, typename TI, int MAXE>
+__global__ void k_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H, int n0, Sched s,
+                        const cplx<T>* tw, int L)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int M = s.n, N0 = 2 * M;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* buf = twl + N0;
+  for (int i = threadIdx.x; i < N0; i += blockDim.x)
+    twl[i] = tw[i];
+  const int b = blockIdx.y;
+  in += (int64_t)b * rows * n0;
+  out += (int64_t)b * rows * H;
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * M; idx += blockDim.x) {
+    const int l = idx / M, j = idx % M;
+    const int64_t r = row0 + l;
+    cplx<T> z = { 0, 0 };
+    if (r < rows) {
+      const int x0 = 2 * j, x1 = 2 * j + 1;
+      z.x = x0 < n0 ? (T)in[r * n0 + x0] : T(0);
+      z.y = x1 < n0 ? (T)in[r * n0 + x1] : T(0);
+    }
+    buf[l * M + j] = z;
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, true, false>(buf, twl, 2, s, L, 1, M);
+  for (int idx = threadIdx.x; idx < L * H; idx += blockDim.x) {
+    const int l = idx / H, k = idx % H;
+    const int64_t r = row0 + l;
+    if (r < rows) {
+      const cplx<T> zk = buf[l * M + (k % M)];
+      const cplx<T> zm = buf[l * M + ((M - k) % M)];
+      const cplx<T> a = { (zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5) }; // E_k
+      const cplx<T> d = { (zk.x - zm.x) * T(0.5), (zk.y + zm.y) * T(0.5) }; // (Z_k - conj Z_{M-k})/2
+      const cplx<T> w = twl[k];                                           // e^{-2 pi i k/N0}
+      const cplx<T> wd = cmul(w, d);
+      // X_k = a - i * wd
+      out[r * H + k] = { a.x + wd.y, a.y - wd.x };
+    }
+  }
+}
+
+// Fused pass along the last axis of the matvec: lines (kx, k_mid) of
+// C[j][k_mid][kx], n_in leading points (rest zero) -> forward DFT -> times
+// f(k)/N_ext -> inverse DFT -> first n_in points back in place.
+template <typename T, int MAXE>
+__global__ void k_fused_mult(cplx<T>* C, GenGeom g, const double* eig, int op, double inv_next,
+                             Sched s, const cplx<T>* tw, int L)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = s.n;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* buf = twl + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    twl[i] = tw[i];
+  const int b = blockIdx.y;
+  const int64_t nlines = (g.dim == 3) ? (int64_t)g.H * g.N1 : g.H;
+  const int nlast = (g.dim == 3) ? g.n2 : g.n1;
+  C += (int64_t)b * nlines * nlast;
+  const int64_t line0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx % L, e = idx / L;
+    const int64_t u = line0 + l;
+    cplx<T> v = { 0, 0 };
+    if (u < nlines && e < nlast)
+      v = C[e * nlines + u];
+    buf[e * L + l] = v;
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, false, false>(buf, twl, 1, s, L, L, 1);
+  for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+    const int l = idx % L, e = idx / L;
+    const int64_t u = line0 + l;
+    if (u < nlines) {
+      const int kx = (int)(u % g.H), km = (int)(u / g.H);
+      const int64_t ei = (g.dim == 3) ? eig_index(g, kx, km, e) : eig_index(g, kx, e, 0);
+      const double m = eig[ei];
+      double f = op == PF_OP_MULTIPLY ? m : (op == PF_OP_ROOT ? sqrt(m) : 1. / m);
+      f *= inv_next;
+      buf[e * L + l] = cscale(buf[e * L + l], (T)f);
+    }
+  }
+  __syncthreads();
+  fft_lds<T, MAXE, false, true>(buf, twl, 1, s, L, L, 1);
+  for (int idx = threadIdx.x; idx < L * nlast; idx += blockDim.x) {
+    const int l = idx % L, j = idx / L;
+    const int64_t u = line0 + l;
+    if (u < nlines)
+      C[j * nlines + u] = buf[j * L + l];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_value_transform(T* v, int64_t n, int kind, T param)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = value_transform<T>(kind, v[i], param);
+}
+
+__global__ void k_philox_normals(uint64_t seed, uint64_t real, uint64_t first, int64_t n,
+                                 double* out)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = philox_normal<double>(seed, real, first + (uint64_t)i);
+}
+
+__global__ void k_philox_raw(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                             uint32_t k1, int64_t nblocks, uint32_t* out)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nblocks;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t c[4] = { c0 + (uint32_t)i, c1, c2, c3 };
+    philox4x32_10(c, k0, k1);
+    out[4 * i + 0] = c[0];
+    out[4 * i + 1] = c[1];
+    out[4 * i + 2] = c[2];
+    out[4 * i + 3] = c[3];
+  }
+}
+
+// dot products: per-block partial sums (FP64)
+__global__ void k_dot_partial(const double* a, const double* b, int64_t n, double* partial)
+{
+  __shared__ double sh[1024 / 64];
+  double s = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i] * b[i];
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_down(s, off);
+  if (threadIdx.x % 64 == 0)
+    sh[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++)
+      t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// y = a*x + y
+__global__ void k_axpy(double a, const double* x, double* y, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+// y = x + beta*y
+__global__ void k_xpby(const double* x, double beta, double* y, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i] + beta * y[i];
+}
+
+// z = x - y
+__global__ void k_sub(const double* x, const double* y, double* z, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    z[i] = x[i] - y[i];
+}
+
+// energy functional partial sums: sum iter_i * (0.5 * Aiter_i - b_i)
+__global__ void k_energy_partial(const double* it, const double* ait, const double* b, int64_t n,
+                                 double* partial)
+{
+  __shared__ double sh[1024 / 64];
+  double s = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += it[i] * (0.5 * ait[i] - b[i]);
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_down(s, off);
+  if (threadIdx.x % 64 == 0)
+    sh[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++)
+      t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+} // namespace pfamd
