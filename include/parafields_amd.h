@@ -225,6 +225,36 @@ int pf_philox_normals(void* plan, uint64_t seed, uint64_t realization, uint64_t 
 int pf_philox_raw(void* plan, const uint32_t ctr[4], const uint32_t key[2], int64_t nblocks,
                   uint32_t* out, int32_t mem);
 
+/* ---- learned spectral preconditioner (no counterpart in the reference) ---
+ * A small MLP maps covariance parameters to per-radial-frequency-bin
+ * log-multipliers L_b; PF_CG_LEARNED_PCG applies
+ * M^{-1} = R F^{-1} diag(exp(L_{bin(k)})) F E (bin(k): radius of the folded
+ * spectral position |k/(N/2)|/sqrt(dim), nbins equal bins) as the
+ * preconditioner of multiplyInverse; plain CG stays the fallback and the FP64
+ * true residual judges convergence. */
+
+/* Training targets from the plan's own FP64 embedding eigenvalues (DCT
+ * backend): L*_b = -log(mean over bin b of max(lambda_k, floor_value)). */
+int pf_spectral_targets(void* plan, int32_t nbins, double floor_value, double* log_targets);
+
+/* FP32 MLP with layer sizes dims[0..ndims-1], exact-GELU hidden layers and a
+ * linear output layer; LeCun-normal initialisation from `seed`. Parameters
+ * are stored layer by layer as W (rows = outputs, row-major) then b. */
+int pf_mlp_create(const int32_t* dims, int32_t ndims, uint64_t seed, void** mlp);
+int pf_mlp_destroy(void* mlp);
+int pf_mlp_nparams(void* mlp, int64_t* n);
+/* set = 1 copies `values` into the network, set = 0 reads them out. */
+int pf_mlp_params(void* mlp, float* values, int64_t n, int32_t set);
+/* y[batch x dims[last]] = MLP(x[batch x dims[0]]) (host pointers). */
+int pf_mlp_forward(void* mlp, const float* x, int32_t batch, float* y);
+/* Gradient of the mean squared error mean((MLP(x) - y)^2) w.r.t. every
+ * parameter (host pointers), and the loss. */
+int pf_mlp_gradients(void* mlp, const float* x, const float* y, int32_t batch, float* grads,
+                     double* loss);
+/* `steps` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8) at rate lr. */
+int pf_mlp_train(void* mlp, const float* x, const float* y, int32_t batch, int32_t steps, float lr,
+                 double* final_loss);
+
 /* Observability (the analogue of the reference's randomField.verbose
  * prints, fieldtraits.hh:180, matrix.hh:301-305): when on, every kernel
  * launch of a call is bracketed by HIP events on the plan's stream and

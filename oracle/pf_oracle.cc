@@ -896,6 +896,22 @@ void generate_dctdst(const pf_desc& d, const Geo& g, const Eigs& e, const double
 // Matrix::multiplyExtended / multiplyRootExtended / multiplyInverseExtended,
 // general version (matrix.hh:1443-1646): zero-pad (r2cfieldbackend.hh:352-368),
 // r2c / N_ext (:210-239), mult (:337-341), c2r, restrict.
+// Learned spectral preconditioner table (exp of the bin log-multipliers),
+// used by matvec op 3: bin of the folded position |k/(N/2)|/sqrt(dim).
+thread_local std::vector<double> g_spec_table;
+
+int radial_bin(const Geo& g, Index k0, Index k1, Index k2, int nbins)
+{
+  Index a = 2 * k0 > g.N[0] ? g.N[0] - k0 : k0;
+  Index b = 2 * k1 > g.N[1] ? g.N[1] - k1 : k1;
+  Index c = 2 * k2 > g.N[2] ? g.N[2] - k2 : k2;
+  const double x = (double)a / (g.N[0] / 2), y = (double)b / std::max<Index>(g.N[1] / 2, 1),
+               z = g.dim == 3 ? (double)c / std::max<Index>(g.N[2] / 2, 1) : 0.;
+  const double r = std::sqrt((x * x + y * y + z * z) / g.dim);
+  int bin = (int)(r * nbins);
+  return bin < nbins ? bin : nbins - 1;
+}
+
 void matvec_r2c(const pf_desc& d, const Geo& g, const Eigs& e, int op, const double* x,
                 double* y)
 {
@@ -915,8 +931,13 @@ void matvec_r2c(const pf_desc& d, const Geo& g, const Eigs& e, int op, const dou
   for (Index k2 = 0; k2 < g.N[2]; k2++)
     for (Index k1 = 0; k1 < g.N[1]; k1++)
       for (Index k0 = 0; k0 < g.H; k0++) {
-        const double m = eig_at(d, g, e, k0, k1, k2);
-        const double f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+        double f;
+        if (op == 3)
+          f = g_spec_table[radial_bin(g, k0, k1, k2, (int)g_spec_table.size())];
+        else {
+          const double m = eig_at(d, g, e, k0, k1, k2);
+          f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+        }
         spec[k0 + g.H * (k1 + g.N[1] * k2)] *= f;
       }
   r2c_backward_extract(g, spec, y);
@@ -1025,8 +1046,9 @@ int inner_cg(const Engine& en, std::vector<double>& iter, const std::vector<doub
   return (int)count;
 }
 
-// Textbook (P)CG on (C/N_ext) x = b with a relative-residual stop, the
-// residual recomputed from the FP64 matvec every iteration. `prec` applies
+// (P)CG with a relative-residual stop: recursive residual, replaced by the
+// true residual b - A x every 50 iterations and whenever the recursion claims
+// convergence (the stop is decided on the true residual). `prec` applies
 // M^{-1}; x starts at 0.
 int pcg(const Engine& en, const std::vector<double>& b, std::vector<double>& x,
         const std::function<void(const std::vector<double>&, std::vector<double>&)>* prec,
@@ -1034,7 +1056,7 @@ int pcg(const Engine& en, const std::vector<double>& b, std::vector<double>& x,
 {
   const size_t n = b.size();
   x.assign(n, 0.);
-  std::vector<double> r = b, z(n), p(n), Ap(n);
+  std::vector<double> r = b, z(n), p(n), Ap(n), Ax(n);
   const double bnorm = std::sqrt(dot(b, b));
   if (bnorm == 0.) {
     *relres = 0.;
@@ -1047,17 +1069,27 @@ int pcg(const Engine& en, const std::vector<double>& b, std::vector<double>& x,
   p = z;
   double rz = dot(r, z);
   int it = 0;
-  double rr = std::sqrt(dot(r, r)) / bnorm;
-  while (rr > rel_tol && it < maxit) {
-    matvec(en, PF_OP_MULTIPLY, p.data(), Ap.data());
-    const double alpha = rz / dot(p, Ap);
-    for (size_t i = 0; i < n; i++)
-      x[i] += alpha * p[i];
-    // true residual r = b - A x
-    std::vector<double> Ax(n);
+  double rr = 1.;
+  auto true_residual = [&]() {
     matvec(en, PF_OP_MULTIPLY, x.data(), Ax.data());
     for (size_t i = 0; i < n; i++)
       r[i] = b[i] - Ax[i];
+    return std::sqrt(dot(r, r)) / bnorm;
+  };
+  while (it < maxit) {
+    matvec(en, PF_OP_MULTIPLY, p.data(), Ap.data());
+    const double alpha = rz / dot(p, Ap);
+    for (size_t i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * Ap[i];
+    }
+    it++;
+    rr = std::sqrt(dot(r, r)) / bnorm;
+    if (rr <= rel_tol || it % 50 == 0) {
+      rr = true_residual();
+      if (rr <= rel_tol)
+        break;
+    }
     if (prec)
       (*prec)(r, z);
     else
@@ -1067,11 +1099,94 @@ int pcg(const Engine& en, const std::vector<double>& b, std::vector<double>& x,
     rz = rzn;
     for (size_t i = 0; i < n; i++)
       p[i] = z[i] + beta * p[i];
-    it++;
-    rr = std::sqrt(dot(r, r)) / bnorm;
   }
+  if (rr > rel_tol || it == 0)
+    rr = true_residual();
   *relres = rr;
   return it;
+}
+
+// FP64 MLP (exact GELU hidden layers, linear output), forward + backward of
+// the mean squared error; parameter layout W (rows = outputs) then b, layer
+// by layer. Reference for the device training kernels.
+double gelu(double x) { return 0.5 * x * (1. + std::erf(x / std::sqrt(2.))); }
+double gelu_grad(double x)
+{
+  return 0.5 * (1. + std::erf(x / std::sqrt(2.))) +
+         x * std::exp(-0.5 * x * x) / std::sqrt(2. * 3.14159265358979323846);
+}
+
+double mlp_forward_backward(const std::vector<int>& dims, const double* prm, const double* X,
+                            const double* Y, int B, double* grads, double* out)
+{
+  const int L = (int)dims.size() - 1;
+  std::vector<Index> woff(L), boff(L);
+  Index off = 0;
+  for (int l = 0; l < L; l++) {
+    woff[l] = off;
+    off += (Index)dims[l] * dims[l + 1];
+    boff[l] = off;
+    off += dims[l + 1];
+  }
+  std::vector<std::vector<double>> Z(L), A(L);
+  for (int l = 0; l < L; l++) {
+    const int a = dims[l], b = dims[l + 1];
+    const double* in = l == 0 ? X : A[l - 1].data();
+    Z[l].resize((size_t)B * b);
+    A[l].resize((size_t)B * b);
+    for (int s = 0; s < B; s++)
+      for (int j = 0; j < b; j++) {
+        double z = prm[boff[l] + j];
+        for (int i = 0; i < a; i++)
+          z += prm[woff[l] + (Index)j * a + i] * in[(Index)s * a + i];
+        Z[l][(Index)s * b + j] = z;
+        A[l][(Index)s * b + j] = l + 1 < L ? gelu(z) : z;
+      }
+  }
+  const int no = dims.back();
+  const double n = (double)B * no;
+  double loss = 0.;
+  std::vector<double> D((size_t)B * no);
+  for (Index i = 0; i < (Index)B * no; i++) {
+    const double e = A[L - 1][i] - Y[i];
+    loss += e * e / n;
+    D[i] = 2. * e / n;
+    if (out)
+      out[i] = A[L - 1][i];
+  }
+  if (grads) {
+    for (int l = L - 1; l >= 0; l--) {
+      const int a = dims[l], b = dims[l + 1];
+      const double* in = l == 0 ? X : A[l - 1].data();
+      if (l + 1 < L)
+        for (Index i = 0; i < (Index)B * b; i++)
+          D[i] *= gelu_grad(Z[l][i]);
+      for (int j = 0; j < b; j++) {
+        double gb = 0.;
+        for (int s = 0; s < B; s++)
+          gb += D[(Index)s * b + j];
+        grads[boff[l] + j] = gb;
+        for (int i = 0; i < a; i++) {
+          double gw = 0.;
+          for (int s = 0; s < B; s++)
+            gw += D[(Index)s * b + j] * in[(Index)s * a + i];
+          grads[woff[l] + (Index)j * a + i] = gw;
+        }
+      }
+      if (l > 0) {
+        std::vector<double> Dn((size_t)B * a, 0.);
+        for (int s = 0; s < B; s++)
+          for (int i = 0; i < a; i++) {
+            double acc = 0.;
+            for (int j = 0; j < b; j++)
+              acc += D[(Index)s * b + j] * prm[woff[l] + (Index)j * a + i];
+            Dn[(Index)s * a + i] = acc;
+          }
+        D.swap(Dn);
+      }
+    }
+  }
+  return loss;
 }
 
 // ---------------------------------------------------------------------------
@@ -1303,9 +1418,9 @@ int pfo_pcg(const pf_desc* d, int mode, int maxit, double rel_tol, const double*
     std::function<void(const std::vector<double>&, std::vector<double>&)> prec =
       [&](const std::vector<double>& r, std::vector<double>& z) {
         z.resize(r.size());
-        matvec(*en, PF_OP_INVERSE_EMBED, r.data(), z.data());
+        matvec(*en, mode == PF_CG_LEARNED_PCG ? 3 : PF_OP_INVERSE_EMBED, r.data(), z.data());
       };
-    int it = pcg(*en, bb, xx, mode == PF_CG_EMBED_PCG ? &prec : nullptr, maxit, rel_tol, relres);
+    int it = pcg(*en, bb, xx, mode == PF_CG_PLAIN ? nullptr : &prec, maxit, rel_tol, relres);
     std::copy(xx.begin(), xx.end(), x);
     if (iterations)
       *iterations = it;
@@ -1366,5 +1481,57 @@ int pfo_cov_eval(const pf_desc* d, const double* x, long long n, double* out)
 }
 
 double pfo_bessel_k(double nu, double x) { return bessel_k(nu, x); }
+// Learned-PCG: preconditioner R F^{-1} diag(exp(L_bin)) F E.
+int pfo_pcg_learned(const pf_desc* d, const double* log_mult, int nbins, int maxit, double rel_tol,
+                    const double* b, double* x, int* iterations, double* relres)
+{
+  g_spec_table.resize(nbins);
+  for (int i = 0; i < nbins; i++)
+    g_spec_table[i] = std::exp(log_mult[i]);
+  return pfo_pcg(d, PF_CG_LEARNED_PCG, maxit, rel_tol, b, x, iterations, relres);
+}
+
+// Radial targets -log(mean_bin max(lambda, floor)) over the DCT-half array.
+int pfo_spectral_targets(const pf_desc* d, int nbins, double floor_value, double* out)
+{
+  PFO_TRY({
+    Engine* en = get_engine(d);
+    if (d->matrix_backend != PF_MATRIX_DCT)
+      throw std::runtime_error("spectral targets need the DCT matrix backend");
+    const Geo& g = en->g;
+    Index P[3];
+    for (int i = 0; i < 3; i++)
+      P[i] = (i < g.dim) ? g.N[i] / 2 + 1 : 1;
+    std::vector<double> sums(nbins, 0.);
+    std::vector<long long> counts(nbins, 0);
+    for (Index c = 0; c < P[2]; c++)
+      for (Index b2 = 0; b2 < P[1]; b2++)
+        for (Index a = 0; a < P[0]; a++) {
+          const int bin = radial_bin(g, a, b2, c, nbins);
+          const double v = en->e.v[a + P[0] * (b2 + P[1] * c)];
+          sums[bin] += v > floor_value ? v : floor_value;
+          counts[bin]++;
+        }
+    double last = 0.;
+    for (int b = 0; b < nbins; b++) {
+      if (counts[b] > 0)
+        last = -std::log(sums[b] / counts[b]);
+      out[b] = counts[b] > 0 ? last : NAN;
+    }
+    for (int b = nbins - 1; b >= 0; b--)
+      if (std::isnan(out[b]))
+        out[b] = (b + 1 < nbins) ? out[b + 1] : last;
+  });
+  return PF_OK;
+}
+
+// MLP loss, gradients (optional) and outputs (optional), FP64.
+double pfo_mlp(const int* dims, int ndims, const double* params, const double* x, const double* y,
+               int batch, double* grads, double* out)
+{
+  std::vector<int> dv(dims, dims + ndims);
+  return mlp_forward_backward(dv, params, x, y, batch, grads, out);
+}
+
 
 } // extern "C"

@@ -22,9 +22,24 @@ HIP_FLAGS = [
 
 
 def build_library(verbose: bool = False) -> str:
+    """Compile the three translation units in parallel and link them."""
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    src = os.path.join(HERE, "csrc", "pf_api.hip")
-    cmd = [HIPCC, *HIP_FLAGS, "-o", LIB + ".tmp", src]
+    objdir = os.path.join(HERE, "lib", "obj")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = ["pf_api.hip", "pf_fast_f32.hip", "pf_fast_f64.hip"]
+    procs, objs = [], []
+    for src in srcs:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmd = [HIPCC, *[f for f in HIP_FLAGS if f != "-shared"], "-c", "-o", obj,
+               os.path.join(HERE, "csrc", src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("hipcc failed")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -14,11 +14,16 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "pf_dctdst.hpp"
+#include "pf_fast.hpp"
+#include "pf_fast_launch.hpp"
 #include "pf_kernels.hpp"
+#include "pf_mlp.hpp"
 
 using namespace pfamd;
 
@@ -140,6 +145,10 @@ size_t lds_budget()
   return budget;
 }
 
+// LDS budget of the specialised kernels; PF_NO_FAST=1 routes every pass
+// through the generic kernels (used by the tests to check both).
+size_t fast_budget() { return getenv("PF_NO_FAST") ? 0 : lds_budget(); }
+
 template <typename T>
 Cfg make_cfg(int n, int64_t nlines, int ntab)
 {
@@ -196,6 +205,8 @@ struct Plan
   DevBuf eig;       // FP64 eigenvalues (logical layout)
   DevBuf sq32, sq64; // sqrt tables
   DevBuf w1, w2, w3, noise, stage_in, stage_out, partial, scratch;
+  DevBuf spec;       // learned radial multipliers exp(L_b) (FP64)
+  int spec_nbins = 0;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tw; // (precision bytes, n) -> table
   bool profiling = false;
   std::vector<std::pair<std::string, double>> last_timings;
@@ -330,6 +341,10 @@ Geo make_geo(const pf_desc& d)
     throw PfError(PF_EINVAL, "unknown field backend");
   if (d.field_backend == PF_FIELD_DCTDST && d.matrix_backend != PF_MATRIX_DCT)
     throw PfError(PF_EINVAL, "DCTDSTFieldBackend requires DCTMatrixBackend");
+  // the reference's DCT/DST extraction walk (dctdstfieldbackend.hh:905-930)
+  // addresses the first n_i entries per axis only for embedding factor 2
+  if (d.field_backend == PF_FIELD_DCTDST && g.factor != 2)
+    throw PfError(PF_EINVAL, "DCTDSTFieldBackend needs embedding.factor 2");
   if (d.covariance < 0 || d.covariance > PF_COV_CUSTOM)
     throw PfError(PF_EINVAL, "covariance structure not known");
   if (d.covariance == PF_COV_CUBIC && d.dim == 3)
@@ -593,16 +608,36 @@ void ensure_eigs(Plan& P)
 // ---------------------------------------------------------------------------
 // Generation (R2C field backend)
 // ---------------------------------------------------------------------------
+LastAxis last_axis(const Geo& g)
+{
+  LastAxis a;
+  a.H = g.H;
+  a.N0h = g.N[0] / 2;
+  a.Nmid = g.dim == 3 ? g.N[1] : 1;
+  a.nlast = g.dim == 3 ? g.n[2] : g.n[1];
+  if (g.dim == 3) {
+    // reference storage index of (kx, k1 = km, k2 = e)
+    a.sA = g.transposed ? 1 : g.N[1];
+    a.sB = g.transposed ? g.N[2] : 1;
+  } else {
+    a.sA = 1;
+    a.sB = 0;
+  }
+  return a;
+}
+
 template <typename T>
 void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
                   int transform, double param, T* out)
 {
   const Geo& g = P.g;
   GenGeom gg = gen_geom(P);
+  LastAxis la = last_axis(g);
+  FastCtx fc{ P.stream, fast_budget() };
   const T* sq = sizeof(T) == 4 ? (const T*)P.sq32.p : (const T*)P.sq64.p;
   const int dim = g.dim;
   const int Nlast = dim == 3 ? g.N[2] : g.N[1];
-  const int nlast = dim == 3 ? g.n[2] : g.n[1];
+  const int nlast = la.nlast;
   const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
   const size_t b1 = (size_t)nlinesG * nlast * sizeof(cplx<T>);
   const size_t b2 = dim == 3 ? (size_t)g.H * g.n[1] * g.n[2] * sizeof(cplx<T>) : 0;
@@ -611,42 +646,177 @@ void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, 
     P.w2.ensure(b2 * batch);
   constexpr int ME = maxe<T>();
   {
-    Sched s = make_sched(Nlast);
-    Cfg c = make_cfg<T>(Nlast, nlinesG, Nlast);
-    dim3 grid(nblocks(nlinesG, c.L), batch);
     Scope sc(&P, "gen_last");
-    if (dnoise)
-      k_gen_last<T, ME, 0><<<grid, c.threads, c.lds, P.stream>>>(
-        gg, sq, dnoise, seed, first, P.w1.as<cplx<T>>(), s, P.table<T>(Nlast), c.L);
-    else
-      k_gen_last<T, ME, 1><<<grid, c.threads, c.lds, P.stream>>>(
-        gg, sq, nullptr, seed, first, P.w1.as<cplx<T>>(), s, P.table<T>(Nlast), c.L);
+    const cplx<T>* tw = P.table<T>(Nlast);
+    if (!fast_gen_last<T>(fc, Nlast, gg, la, sq, dnoise, seed, first, P.w1.as<cplx<T>>(), tw,
+                          batch)) {
+      Sched s = make_sched(Nlast);
+      Cfg c = make_cfg<T>(Nlast, nlinesG, Nlast);
+      dim3 grid(nblocks(nlinesG, c.L), batch);
+      if (dnoise)
+        k_gen_last<T, ME, 0><<<grid, c.threads, c.lds, P.stream>>>(
+          gg, sq, dnoise, seed, first, P.w1.as<cplx<T>>(), s, tw, c.L);
+      else
+        k_gen_last<T, ME, 1><<<grid, c.threads, c.lds, P.stream>>>(
+          gg, sq, nullptr, seed, first, P.w1.as<cplx<T>>(), s, tw, c.L);
+    }
     check_launch();
   }
   const cplx<T>* xin = P.w1.as<cplx<T>>();
   if (dim == 3) {
-    const int64_t nl = (int64_t)g.H * g.n[2];
-    Sched s = make_sched(g.N[1]);
-    Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
-    dim3 grid(nblocks(nl, c.L), batch);
     Scope sc(&P, "gen_mid");
-    k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, P.stream>>>(
-      P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
-      P.table<T>(g.N[1]), c.L);
+    const cplx<T>* tw = P.table<T>(g.N[1]);
+    if (!fast_mid<T>(fc, g.N[1], true, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.N[0] / 2,
+                     g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], tw, batch)) {
+      const int64_t nl = (int64_t)g.H * g.n[2];
+      Sched s = make_sched(g.N[1]);
+      Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+      dim3 grid(nblocks(nl, c.L), batch);
+      k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, P.stream>>>(
+        P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
+        tw, c.L);
+    }
     check_launch();
     xin = P.w2.as<cplx<T>>();
   }
   {
     const int M = g.N[0] / 2;
     const int64_t rows = (int64_t)g.n[1] * g.n[2];
-    Sched s = make_sched(M);
-    Cfg c = make_cfg<T>(M, rows, g.N[0]);
-    dim3 grid(nblocks(rows, c.L), batch);
     Scope sc(&P, "gen_c2r_x");
-    k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, P.stream>>>(xin, out, rows, g.H, g.n[0], s,
-                                                           P.table<T>(g.N[0]), c.L, transform,
-                                                           (T)param);
+    const cplx<T>* tw = P.table<T>(g.N[0]);
+    if (!fast_c2r<T>(fc, M, xin, out, rows, g.H, g.n[0], tw, transform, (T)param, batch)) {
+      Sched s = make_sched(M);
+      Cfg c = make_cfg<T>(M, rows, g.N[0]);
+      dim3 grid(nblocks(rows, c.L), batch);
+      k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, P.stream>>>(xin, out, rows, g.H, g.n[0], s, tw,
+                                                             c.L, transform, (T)param);
+    }
     check_launch();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generation, DCT/DST field backend (matrix.hh:342-372)
+// ---------------------------------------------------------------------------
+DctGeom dct_geom(const Plan& P)
+{
+  const Geo& g = P.g;
+  DctGeom d;
+  d.dim = g.dim;
+  d.P0 = g.P[0];
+  d.P1 = g.P[1];
+  d.P2 = g.P[2];
+  d.N0 = g.N[0];
+  d.N1 = g.N[1];
+  d.N2 = g.N[2];
+  d.n0 = g.n[0];
+  d.n1 = g.n[1];
+  d.n2 = g.n[2];
+  d.Ptot = (int64_t)g.P[0] * g.P[1] * g.P[2];
+  // storage order of the reference arrays: transposeIfNeeded swaps the two
+  // last dims (dctdstfieldbackend.hh:868-882)
+  if (g.dim == 3) {
+    d.sx = 1;
+    d.s1 = g.transposed ? (int64_t)g.P[0] * g.P[2] : g.P[0];
+    d.s2 = g.transposed ? g.P[0] : (int64_t)g.P[0] * g.P[1];
+  } else {
+    d.sx = g.transposed ? g.P[1] : 1;
+    d.s1 = g.transposed ? 1 : g.P[0];
+    d.s2 = 0;
+  }
+  d.noise_stride = g.noise_per_field;
+  return d;
+}
+
+// largest power of two <= cap dividing n
+int pow2_divisor(int n, int cap)
+{
+  int L = 1;
+  while (L * 2 <= cap && n % (L * 2) == 0)
+    L *= 2;
+  return L;
+}
+
+size_t work_cap()
+{
+  if (const char* e = getenv("PF_MAX_WORK_BYTES"))
+    return (size_t)atoll(e);
+  return (size_t)96 << 30;
+}
+
+template <typename T>
+void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
+                     int transform, double param, T* out)
+{
+  const Geo& g = P.g;
+  DctGeom dg = dct_geom(P);
+  const T* sq = sizeof(T) == 4 ? (const T*)P.sq32.p : (const T*)P.sq64.p;
+  const int dim = g.dim;
+  const int Pm = dim == 3 ? g.P[1] : 1;
+  const int nlast = dim == 3 ? g.n[2] : g.n[1];
+  const int Nlast = dim == 3 ? g.N[2] : g.N[1];
+  const int ncomb = 1 << (dim - 1);
+  const size_t gsz = (size_t)ncomb * Pm * g.P[0] * nlast * sizeof(T);
+  const size_t hsz = dim == 3 ? (size_t)2 * g.n[1] * g.n[2] * g.P[0] * sizeof(T) : 0;
+  int sub = (int)std::max<size_t>(1, work_cap() / (gsz + hsz));
+  sub = std::min(sub, batch);
+  P.w1.ensure(gsz * sub);
+  if (dim == 3)
+    P.w2.ensure(hsz * sub);
+  const size_t cs = sizeof(cplx<T>);
+  const int maxL = sizeof(T) == 4 ? 16 : 8;
+  for (int b0 = 0; b0 < batch; b0 += sub) {
+    const int nb = std::min(sub, batch - b0);
+    const double* nz = dnoise ? dnoise + (int64_t)b0 * g.noise_per_field : nullptr;
+    {
+      const int M = Nlast / 2;
+      int L = pow2_divisor(g.P[0] - 1, maxL);
+      while (L > 1 && ((size_t)(L + 1) * M + L) * cs > lds_budget())
+        L /= 2;
+      const size_t lds = ((size_t)(L + 1) * M + L) * cs;
+      const int nt = std::min(1024, std::max(64, ((L * M / maxe<T>() + 63) / 64) * 64));
+      dim3 grid((g.P[0] - 1) / L + 1, Pm, nb);
+      Scope sc(&P, "dct_gen_last");
+      if (nz)
+        kd_gen_last<T, 0, 0, 0, 0><<<grid, nt, lds, P.stream>>>(
+          dg, sq, nz, seed, first + b0, P.w1.as<T>(), P.table<T>(Nlast), make_sched(M), M, L);
+      else
+        kd_gen_last<T, 0, 0, 0, 1><<<grid, nt, lds, P.stream>>>(
+          dg, sq, nullptr, seed, first + b0, P.w1.as<T>(), P.table<T>(Nlast), make_sched(M), M, L);
+      check_launch();
+    }
+    const T* hin = P.w1.as<T>();
+    if (dim == 3) {
+      const int M = g.N[1] / 2;
+      int L = pow2_divisor(g.P[0] - 1, maxL);
+      while (L > 1 && ((size_t)(L + 1) * M + L) * cs > lds_budget())
+        L /= 2;
+      const size_t lds = ((size_t)(L + 1) * M + L) * cs;
+      const int nt = std::min(1024, std::max(64, ((L * M / maxe<T>() + 63) / 64) * 64));
+      dim3 grid((g.P[0] - 1) / L + 1, g.n[2], nb);
+      Scope sc(&P, "dct_mid");
+      kd_mid<T, 0, 0, 0><<<grid, nt, lds, P.stream>>>(dg, P.w1.as<T>(), P.w2.as<T>(),
+                                                      P.table<T>(g.N[1]), make_sched(M), M, L);
+      check_launch();
+      hin = P.w2.as<T>();
+    }
+    {
+      const int M = g.N[0] / 2;
+      const int64_t rows = (int64_t)g.n[1] * g.n[2];
+      int L = maxL;
+      while (L > 1 && ((size_t)(L + 1) * M + L) * cs > lds_budget())
+        L /= 2;
+      while (L > 1 && L / 2 >= rows)
+        L /= 2;
+      const size_t lds = ((size_t)(L + 1) * M + L) * cs;
+      const int nt = std::min(1024, std::max(64, ((L * M / maxe<T>() + 63) / 64) * 64));
+      dim3 grid((unsigned)((rows + L - 1) / L), nb);
+      Scope sc(&P, "dct_x");
+      kd_x<T, 0, 0, 0><<<grid, nt, lds, P.stream>>>(dg, hin, out + (int64_t)b0 * g.domain, rows,
+                                                    P.table<T>(g.N[0]), make_sched(M), M, L,
+                                                    transform, (T)param);
+      check_launch();
+    }
   }
 }
 
@@ -658,6 +828,8 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
 {
   const Geo& g = P.g;
   GenGeom gg = gen_geom(P);
+  LastAxis la = last_axis(g);
+  FastCtx fc{ P.stream, fast_budget() };
   const int dim = g.dim;
   constexpr int ME = maxe<T>();
   const int M = g.N[0] / 2;
@@ -668,52 +840,71 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
   if (dim == 3)
     P.w2.ensure(c2 * batch);
   {
-    Sched s = make_sched(M);
-    Cfg c = make_cfg<T>(M, rows, g.N[0]);
     Scope sc(&P, "mv_r2c_x");
-    k_r2c_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
-      x, P.w1.as<cplx<T>>(), rows, g.H, g.n[0], s, P.table<T>(g.N[0]), c.L);
+    const cplx<T>* tw = P.table<T>(g.N[0]);
+    if (!fast_r2c<T>(fc, M, x, P.w1.as<cplx<T>>(), rows, g.H, g.n[0], tw, batch)) {
+      Sched s = make_sched(M);
+      Cfg c = make_cfg<T>(M, rows, g.N[0]);
+      k_r2c_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
+        x, P.w1.as<cplx<T>>(), rows, g.H, g.n[0], s, tw, c.L);
+    }
     check_launch();
   }
   cplx<T>* C = P.w1.as<cplx<T>>();
   if (dim == 3) {
-    const int64_t nl = (int64_t)g.H * g.n[2];
-    Sched s = make_sched(g.N[1]);
-    Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
     Scope sc(&P, "mv_fwd_mid");
-    k_pass_mid<T, ME, false><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
-      P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], s,
-      P.table<T>(g.N[1]), c.L);
+    const cplx<T>* tw = P.table<T>(g.N[1]);
+    if (!fast_mid<T>(fc, g.N[1], false, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.N[0] / 2,
+                     g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], tw, batch)) {
+      const int64_t nl = (int64_t)g.H * g.n[2];
+      Sched s = make_sched(g.N[1]);
+      Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+      k_pass_mid<T, ME, false><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
+        P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], s,
+        tw, c.L);
+    }
     check_launch();
     C = P.w2.as<cplx<T>>();
   }
   {
     const int Nlast = dim == 3 ? g.N[2] : g.N[1];
     const int64_t nlines = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
-    Sched s = make_sched(Nlast);
-    Cfg c = make_cfg<T>(Nlast, nlines, Nlast);
     Scope sc(&P, "mv_fused_mult");
-    k_fused_mult<T, ME><<<dim3(nblocks(nlines, c.L), batch), c.threads, c.lds, P.stream>>>(
-      C, gg, P.eig.as<double>(), op, 1.0 / (double)g.Next, s, P.table<T>(Nlast), c.L);
+    const cplx<T>* tw = P.table<T>(Nlast);
+    if (!fast_fused<T>(fc, Nlast, C, gg, la, P.eig.as<double>(), op, 1.0 / (double)g.Next, tw,
+                       batch, P.spec.as<double>(), P.spec_nbins)) {
+      Sched s = make_sched(Nlast);
+      Cfg c = make_cfg<T>(Nlast, nlines, Nlast);
+      k_fused_mult<T, ME><<<dim3(nblocks(nlines, c.L), batch), c.threads, c.lds, P.stream>>>(
+        C, gg, P.eig.as<double>(), op, 1.0 / (double)g.Next, s, tw, c.L, P.spec.as<double>(),
+        P.spec_nbins);
+    }
     check_launch();
   }
   if (dim == 3) {
-    const int64_t nl = (int64_t)g.H * g.n[2];
-    Sched s = make_sched(g.N[1]);
-    Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
     Scope sc(&P, "mv_inv_mid");
-    k_pass_mid<T, ME, true><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
-      P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
-      P.table<T>(g.N[1]), c.L);
+    const cplx<T>* tw = P.table<T>(g.N[1]);
+    if (!fast_mid<T>(fc, g.N[1], true, P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.N[0] / 2,
+                     g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], tw, batch)) {
+      const int64_t nl = (int64_t)g.H * g.n[2];
+      Sched s = make_sched(g.N[1]);
+      Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+      k_pass_mid<T, ME, true><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
+        P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
+        tw, c.L);
+    }
     check_launch();
   }
   {
-    Sched s = make_sched(M);
-    Cfg c = make_cfg<T>(M, rows, g.N[0]);
     Scope sc(&P, "mv_c2r_x");
-    k_c2r_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
-      P.w1.as<cplx<T>>(), y, rows, g.H, g.n[0], s, P.table<T>(g.N[0]), c.L, PF_TRANSFORM_NONE,
-      T(0));
+    const cplx<T>* tw = P.table<T>(g.N[0]);
+    if (!fast_c2r<T>(fc, M, P.w1.as<cplx<T>>(), y, rows, g.H, g.n[0], tw, PF_TRANSFORM_NONE, T(0),
+                     batch)) {
+      Sched s = make_sched(M);
+      Cfg c = make_cfg<T>(M, rows, g.N[0]);
+      k_c2r_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
+        P.w1.as<cplx<T>>(), y, rows, g.H, g.n[0], s, tw, c.L, PF_TRANSFORM_NONE, T(0));
+    }
     check_launch();
   }
 }
@@ -765,6 +956,8 @@ double energy(Plan& P, const double* it, const double* ait, const double* b, int
 
 void matvec64(Plan& P, int op, const double* x, double* y)
 {
+  if (op == PF_OP_SPECTRAL && P.spec_nbins <= 0)
+    throw PfError(PF_EINVAL, "no learned multipliers loaded");
   if (P.d.field_backend != PF_FIELD_R2C)
     throw PfError(PF_EINVAL, "matvec is implemented for the R2C field backend");
   matvec_r2c<double>(P, op, x, y, 1);
@@ -818,8 +1011,11 @@ int inner_cg(Plan& P, double* iter, const double* solution, int64_t n, unsigned 
   return (int)count;
 }
 
-// (P)CG with a relative-residual stop; the residual is the FP64 true residual
-// b - A x recomputed from a matvec of x every iteration.
+// (P)CG with a relative-residual stop. The residual is updated recursively
+// and replaced by the FP64 true residual b - A x every 50 iterations and
+// whenever the recursion claims convergence; the stop is decided on the true
+// residual. Preconditioner: none, the embedding inverse, or the learned
+// radial spectral multiplier (PF_OP_SPECTRAL).
 int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, double rel_tol,
         double* relres)
 {
@@ -840,6 +1036,8 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
   auto prec = [&](const double* rr, double* zz) {
     if (mode == PF_CG_EMBED_PCG)
       matvec64(P, PF_OP_INVERSE_EMBED, rr, zz);
+    else if (mode == PF_CG_LEARNED_PCG)
+      matvec64(P, PF_OP_SPECTRAL, rr, zz);
     else
       HIPCHECK(hipMemcpyAsync(zz, rr, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
   };
@@ -847,26 +1045,126 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
   HIPCHECK(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
   double rz = dot(P, r, z, n);
   int it = 0;
-  double rr = std::sqrt(dot(P, r, r, n)) / bnorm;
-  while (rr > rel_tol && it < maxit) {
+  double rr = 1.;
+  while (it < maxit) {
     matvec64(P, PF_OP_MULTIPLY, p, Ap);
     const double alpha = rz / dot(P, p, Ap, n);
     k_axpy<<<gb, 256, 0, P.stream>>>(alpha, p, x, n);
-    matvec64(P, PF_OP_MULTIPLY, x, Ax);
-    k_sub<<<gb, 256, 0, P.stream>>>(b, Ax, r, n);
+    k_axpy<<<gb, 256, 0, P.stream>>>(-alpha, Ap, r, n);
     check_launch();
+    it++;
+    rr = std::sqrt(dot(P, r, r, n)) / bnorm;
+    if (rr <= rel_tol || it % 50 == 0) {
+      matvec64(P, PF_OP_MULTIPLY, x, Ax);
+      k_sub<<<gb, 256, 0, P.stream>>>(b, Ax, r, n);
+      check_launch();
+      rr = std::sqrt(dot(P, r, r, n)) / bnorm;
+      if (rr <= rel_tol)
+        break;
+    }
     prec(r, z);
     const double rzn = dot(P, r, z, n);
     const double beta = rzn / rz;
     rz = rzn;
     k_xpby<<<gb, 256, 0, P.stream>>>(z, beta, p, n);
     check_launch();
-    it++;
+  }
+  if (rr > rel_tol || it == 0) {
+    matvec64(P, PF_OP_MULTIPLY, x, Ax);
+    k_sub<<<gb, 256, 0, P.stream>>>(b, Ax, r, n);
+    check_launch();
     rr = std::sqrt(dot(P, r, r, n)) / bnorm;
   }
   *relres = rr;
   return it;
 }
+
+// ---------------------------------------------------------------------------
+// MLP of the learned spectral preconditioner
+// ---------------------------------------------------------------------------
+struct Mlp
+{
+  std::vector<int> dims;
+  std::vector<int64_t> woff, boff;
+  int64_t nparams = 0;
+  DevBuf params, grads, m, v, x, y, d0, d1, partial;
+  std::vector<DevBuf> Z, A;
+  int t = 0;
+  hipStream_t stream = nullptr;
+  int cap = 0;
+  ~Mlp()
+  {
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+  int nl() const { return (int)dims.size() - 1; }
+  void ensure_batch(int B)
+  {
+    if (B <= cap)
+      return;
+    Z.resize(nl());
+    A.resize(nl());
+    int wmax = 0;
+    for (int l = 0; l < nl(); l++) {
+      Z[l].ensure((size_t)B * dims[l + 1] * sizeof(float));
+      A[l].ensure((size_t)B * dims[l + 1] * sizeof(float));
+    }
+    for (int d : dims)
+      wmax = std::max(wmax, d);
+    x.ensure((size_t)B * dims[0] * sizeof(float));
+    y.ensure((size_t)B * dims.back() * sizeof(float));
+    d0.ensure((size_t)B * wmax * sizeof(float));
+    d1.ensure((size_t)B * wmax * sizeof(float));
+    partial.ensure(1024 * sizeof(double));
+    cap = B;
+  }
+  void forward(int B)
+  {
+    const float* in = x.as<float>();
+    for (int l = 0; l < nl(); l++) {
+      const int a = dims[l], b = dims[l + 1];
+      const int64_t tot = (int64_t)B * b;
+      k_mlp_linear_fwd<<<(unsigned)((tot + 255) / 256), 256, 0, stream>>>(
+        in, params.as<float>() + woff[l], params.as<float>() + boff[l], Z[l].as<float>(),
+        A[l].as<float>(), B, a, b, l + 1 < nl() ? 1 : 0);
+      in = A[l].as<float>();
+    }
+    check_launch();
+  }
+  double backward(int B)
+  {
+    const int L = nl();
+    const int64_t nout = (int64_t)B * dims.back();
+    const unsigned nb = std::min<unsigned>(1024, grid1d(nout));
+    float* D = d0.as<float>();
+    float* Dn = d1.as<float>();
+    k_mlp_mse<<<nb, 256, 0, stream>>>(A[L - 1].as<float>(), y.as<float>(), D, nout,
+                                      partial.as<double>());
+    for (int l = L - 1; l >= 0; l--) {
+      const int a = dims[l], b = dims[l + 1];
+      if (l + 1 < L)
+        k_mlp_gelu_back<<<grid1d((int64_t)B * b), 256, 0, stream>>>(D, Z[l].as<float>(),
+                                                                    (int64_t)B * b);
+      const float* in = l == 0 ? x.as<float>() : A[l - 1].as<float>();
+      k_mlp_grad_w<<<(unsigned)((a * b + b + 255) / 256), 256, 0, stream>>>(
+        D, in, grads.as<float>() + woff[l], grads.as<float>() + boff[l], B, a, b);
+      if (l > 0) {
+        k_mlp_back_in<<<(unsigned)(((int64_t)B * a + 255) / 256), 256, 0, stream>>>(
+          D, params.as<float>() + woff[l], Dn, B, a, b);
+        std::swap(D, Dn);
+      }
+    }
+    check_launch();
+    std::vector<double> h(nb);
+    HIPCHECK(hipMemcpyAsync(h.data(), partial.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    double loss = 0.;
+    for (double q : h)
+      loss += q;
+    return loss;
+  }
+};
 
 #define PF_GUARD(...)                                                                              \
   try {                                                                                            \
@@ -1029,8 +1327,6 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
       throw PfError(PF_EINVAL, "batch must be >= 1");
     if (transform < 0 || transform > PF_TRANSFORM_BOXCOX)
       throw PfError(PF_EINVAL, "transform type not known");
-    if (P->d.field_backend != PF_FIELD_R2C)
-      throw PfError(PF_EINVAL, "field backend not supported by pf_generate yet");
     ensure_eigs(*P);
     const Geo& g = P->g;
     const double* dn = nullptr;
@@ -1047,12 +1343,22 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
       P->stage_out.ensure(outb);
       dout = P->stage_out.p;
     }
-    if (P->d.precision == PF_FP32)
-      generate_r2c<float>(*P, dn, seed, first_realization, batch, transform, transform_param,
-                          (float*)dout);
-    else
-      generate_r2c<double>(*P, dn, seed, first_realization, batch, transform, transform_param,
-                           (double*)dout);
+    const bool dct = P->d.field_backend == PF_FIELD_DCTDST;
+    if (P->d.precision == PF_FP32) {
+      if (dct)
+        generate_dctdst<float>(*P, dn, seed, first_realization, batch, transform,
+                               transform_param, (float*)dout);
+      else
+        generate_r2c<float>(*P, dn, seed, first_realization, batch, transform, transform_param,
+                            (float*)dout);
+    } else {
+      if (dct)
+        generate_dctdst<double>(*P, dn, seed, first_realization, batch, transform,
+                                transform_param, (double*)dout);
+      else
+        generate_r2c<double>(*P, dn, seed, first_realization, batch, transform, transform_param,
+                             (double*)dout);
+    }
     if (out_mem == PF_MEM_HOST)
       HIPCHECK(hipMemcpyAsync(out, dout, outb, hipMemcpyDeviceToHost, P->stream));
     P->sync();
@@ -1107,8 +1413,16 @@ int pf_cg(void* plan, const double* b, double* x, int32_t mem, const pf_cg_opts*
     const int maxit = (opts && opts->max_iterations > 0) ? opts->max_iterations
                                                          : P->d.cg_iterations;
     const double tol = opts ? opts->rel_tol : 1e-8;
-    if (mode == PF_CG_LEARNED_PCG)
-      throw PfError(PF_EINVAL, "learned preconditioner not available in this build");
+    if (mode == PF_CG_LEARNED_PCG) {
+      if (!opts->bin_log_multipliers || opts->nbins < 1)
+        throw PfError(PF_EINVAL, "learned PCG needs bin_log_multipliers");
+      std::vector<double> t(opts->nbins);
+      for (int i = 0; i < opts->nbins; i++)
+        t[i] = std::exp((double)opts->bin_log_multipliers[i]);
+      P->spec.ensure(t.size() * sizeof(double));
+      HIPCHECK(hipMemcpy(P->spec.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+      P->spec_nbins = opts->nbins;
+    }
     // the CG runs in FP64 whatever the plan's field precision
     if (P->d.precision != PF_FP64)
       throw PfError(PF_EINVAL, "pf_cg needs a PF_FP64 plan (FP64 matvecs and dot products)");
@@ -1229,6 +1543,183 @@ int pf_philox_raw(void* plan, const uint32_t ctr[4], const uint32_t key[2], int6
     if (mem == PF_MEM_HOST)
       HIPCHECK(hipMemcpyAsync(out, d, nb, hipMemcpyDeviceToHost, P->stream));
     P->sync();
+    return PF_OK;
+  })
+}
+
+
+// ---- learned spectral preconditioner --------------------------------------
+
+int pf_spectral_targets(void* plan, int32_t nbins, double floor_value, double* log_targets)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (P->d.matrix_backend != PF_MATRIX_DCT)
+      throw PfError(PF_EINVAL, "spectral targets need the DCT matrix backend");
+    if (nbins < 1)
+      throw PfError(PF_EINVAL, "nbins must be positive");
+    if (!P->eig_valid)
+      compute_eigs(*P, nullptr);
+    const Geo& g = P->g;
+    DevBuf sums, counts;
+    sums.ensure(nbins * sizeof(double));
+    counts.ensure(nbins * sizeof(unsigned long long));
+    HIPCHECK(hipMemsetAsync(sums.p, 0, nbins * sizeof(double), P->stream));
+    HIPCHECK(hipMemsetAsync(counts.p, 0, nbins * sizeof(unsigned long long), P->stream));
+    const int64_t total = (int64_t)g.P[0] * g.P[1] * g.P[2];
+    k_radial_sums<<<grid1d(total), 256, 0, P->stream>>>(P->eig.as<double>(), g.P[0], g.P[1],
+                                                        g.P[2], g.dim, nbins, floor_value,
+                                                        sums.as<double>(),
+                                                        counts.as<unsigned long long>());
+    check_launch();
+    std::vector<double> hs(nbins);
+    std::vector<unsigned long long> hc(nbins);
+    HIPCHECK(hipMemcpyAsync(hs.data(), sums.p, nbins * sizeof(double), hipMemcpyDeviceToHost,
+                            P->stream));
+    HIPCHECK(hipMemcpyAsync(hc.data(), counts.p, nbins * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, P->stream));
+    P->sync();
+    double last = 0.;
+    for (int b = 0; b < nbins; b++) {
+      // empty bins (possible near r = 0) take the nearest filled neighbour
+      if (hc[b] > 0)
+        last = -std::log(hs[b] / (double)hc[b]);
+      log_targets[b] = hc[b] > 0 ? last : NAN;
+    }
+    for (int b = nbins - 1; b >= 0; b--)
+      if (std::isnan(log_targets[b]))
+        log_targets[b] = (b + 1 < nbins) ? log_targets[b + 1] : last;
+    return PF_OK;
+  })
+}
+
+int pf_mlp_create(const int32_t* dims, int32_t ndims, uint64_t seed, void** mlp)
+{
+  PF_GUARD({
+    if (ndims < 2)
+      throw PfError(PF_EINVAL, "an MLP needs at least input and output sizes");
+    auto M = std::make_unique<Mlp>();
+    M->dims.assign(dims, dims + ndims);
+    int64_t off = 0;
+    for (int l = 0; l + 1 < ndims; l++) {
+      M->woff.push_back(off);
+      off += (int64_t)dims[l] * dims[l + 1];
+      M->boff.push_back(off);
+      off += dims[l + 1];
+    }
+    M->nparams = off;
+    HIPCHECK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    M->params.ensure(off * sizeof(float));
+    M->grads.ensure(off * sizeof(float));
+    M->m.ensure(off * sizeof(float));
+    M->v.ensure(off * sizeof(float));
+    // LeCun-normal weights, zero biases
+    std::vector<float> h(off, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l + 1 < ndims; l++) {
+      std::normal_distribution<double> nd(0., 1. / std::sqrt((double)dims[l]));
+      for (int64_t i = 0; i < (int64_t)dims[l] * dims[l + 1]; i++)
+        h[M->woff[l] + i] = (float)nd(rng);
+    }
+    HIPCHECK(hipMemcpy(M->params.p, h.data(), off * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(M->m.p, 0, off * sizeof(float)));
+    HIPCHECK(hipMemset(M->v.p, 0, off * sizeof(float)));
+    *mlp = M.release();
+    return PF_OK;
+  })
+}
+
+int pf_mlp_destroy(void* mlp)
+{
+  PF_GUARD({
+    delete static_cast<Mlp*>(mlp);
+    return PF_OK;
+  })
+}
+
+int pf_mlp_nparams(void* mlp, int64_t* n)
+{
+  PF_GUARD({
+    *n = static_cast<Mlp*>(mlp)->nparams;
+    return PF_OK;
+  })
+}
+
+int pf_mlp_params(void* mlp, float* values, int64_t n, int32_t set)
+{
+  PF_GUARD({
+    Mlp* M = static_cast<Mlp*>(mlp);
+    if (n != M->nparams)
+      throw PfError(PF_EINVAL, "parameter count mismatch");
+    if (set)
+      HIPCHECK(hipMemcpy(M->params.p, values, n * sizeof(float), hipMemcpyHostToDevice));
+    else
+      HIPCHECK(hipMemcpy(values, M->params.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return PF_OK;
+  })
+}
+
+int pf_mlp_forward(void* mlp, const float* x, int32_t batch, float* y)
+{
+  PF_GUARD({
+    Mlp* M = static_cast<Mlp*>(mlp);
+    M->ensure_batch(batch);
+    HIPCHECK(hipMemcpyAsync(M->x.p, x, (size_t)batch * M->dims[0] * sizeof(float),
+                            hipMemcpyHostToDevice, M->stream));
+    M->forward(batch);
+    HIPCHECK(hipMemcpyAsync(y, M->A[M->nl() - 1].p, (size_t)batch * M->dims.back() * sizeof(float),
+                            hipMemcpyDeviceToHost, M->stream));
+    HIPCHECK(hipStreamSynchronize(M->stream));
+    return PF_OK;
+  })
+}
+
+int pf_mlp_gradients(void* mlp, const float* x, const float* y, int32_t batch, float* grads,
+                     double* loss)
+{
+  PF_GUARD({
+    Mlp* M = static_cast<Mlp*>(mlp);
+    M->ensure_batch(batch);
+    HIPCHECK(hipMemcpyAsync(M->x.p, x, (size_t)batch * M->dims[0] * sizeof(float),
+                            hipMemcpyHostToDevice, M->stream));
+    HIPCHECK(hipMemcpyAsync(M->y.p, y, (size_t)batch * M->dims.back() * sizeof(float),
+                            hipMemcpyHostToDevice, M->stream));
+    M->forward(batch);
+    const double l = M->backward(batch);
+    HIPCHECK(hipMemcpy(grads, M->grads.p, M->nparams * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss)
+      *loss = l;
+    return PF_OK;
+  })
+}
+
+// Full-batch Adam on the mean-squared error for `steps` steps.
+int pf_mlp_train(void* mlp, const float* x, const float* y, int32_t batch, int32_t steps, float lr,
+                 double* final_loss)
+{
+  PF_GUARD({
+    Mlp* M = static_cast<Mlp*>(mlp);
+    M->ensure_batch(batch);
+    HIPCHECK(hipMemcpyAsync(M->x.p, x, (size_t)batch * M->dims[0] * sizeof(float),
+                            hipMemcpyHostToDevice, M->stream));
+    HIPCHECK(hipMemcpyAsync(M->y.p, y, (size_t)batch * M->dims.back() * sizeof(float),
+                            hipMemcpyHostToDevice, M->stream));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    double l = 0.;
+    for (int s = 0; s < steps; s++) {
+      M->forward(batch);
+      l = M->backward(batch);
+      M->t++;
+      const float c1 = 1.f - std::pow(b1, (float)M->t), c2 = 1.f - std::pow(b2, (float)M->t);
+      k_mlp_adam<<<grid1d(M->nparams), 256, 0, M->stream>>>(
+        M->params.as<float>(), M->grads.as<float>(), M->m.as<float>(), M->v.as<float>(),
+        M->nparams, lr, b1, b2, eps, c1, c2);
+      check_launch();
+    }
+    HIPCHECK(hipStreamSynchronize(M->stream));
+    if (final_loss)
+      *final_loss = l;
     return PF_OK;
   })
 }

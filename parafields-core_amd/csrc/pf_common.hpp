@@ -1,0 +1,65 @@
+// pf_common.hpp -- spectrum geometry shared by the generic and the
+// specialised kernels: how a half-spectrum position maps to the reference's
+// noise order and to the eigenvalue table.
+#pragma once
+
+#include "pf_device.hpp"
+
+namespace pfamd {
+
+struct GenGeom
+{
+  int dim, transposed;
+  int N0, N1, N2, H;
+  int n0, n1, n2;
+  int P0, P1, P2;       // DCT half sizes (eigen layout of the DCT backend)
+  int dct;              // 1: eigen table is the DCT half array (fold k >= P)
+  int64_t noise_stride; // samples per realization
+};
+
+// Reference storage index of half-spectrum entry (kx, k1, k2): the order in
+// which generateField draws the noise.
+__device__ __forceinline__ int64_t storage_index(const GenGeom& g, int kx, int k1, int k2)
+{
+  if (g.dim == 3 && g.transposed)
+    return kx + (int64_t)g.H * (k2 + (int64_t)g.N2 * k1);
+  return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
+}
+
+__device__ __forceinline__ int64_t eig_index(const GenGeom& g, int kx, int k1, int k2)
+{
+  if (g.dct) {
+    const int a = kx >= g.P0 ? g.N0 - kx : kx;
+    const int b = k1 >= g.P1 ? g.N1 - k1 : k1;
+    const int c = k2 >= g.P2 ? g.N2 - k2 : k2;
+    return a + (int64_t)g.P0 * (b + (int64_t)g.P1 * c);
+  }
+  return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
+}
+
+// Operator index of the learned spectral preconditioner in the fused matvec
+// pass (after PF_OP_MULTIPLY / PF_OP_ROOT / PF_OP_INVERSE_EMBED).
+constexpr int PF_OP_SPECTRAL = 3;
+
+// Radial bin of a (folded) spectral position: r = |k / (N/2)| / sqrt(dim).
+__device__ __forceinline__ int radial_bin(int a, int b, int c, int h0, int h1, int h2, int dim,
+                                          int nbins)
+{
+  const double x = (double)a / h0, y = (double)b / h1, z = dim == 3 ? (double)c / h2 : 0.;
+  const double r = sqrt((x * x + y * y + z * z) / dim);
+  int bin = (int)(r * nbins);
+  return bin < nbins ? bin : nbins - 1;
+}
+
+// Multiplier of a spectral position by the radial table (folded k).
+__device__ __forceinline__ double spectral_factor(const GenGeom& g, int kx, int k1, int k2,
+                                                  const double* table, int nbins)
+{
+  const int a = 2 * kx > g.N0 ? g.N0 - kx : kx;
+  const int b = 2 * k1 > g.N1 ? g.N1 - k1 : k1;
+  const int c = 2 * k2 > g.N2 ? g.N2 - k2 : k2;
+  return table[radial_bin(a, b, c, g.N0 / 2, g.N1 > 1 ? g.N1 / 2 : 1, g.N2 > 1 ? g.N2 / 2 : 1,
+                          g.dim, nbins)];
+}
+
+} // namespace pfamd

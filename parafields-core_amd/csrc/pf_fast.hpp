@@ -1,0 +1,462 @@
+// pf_fast.hpp -- compile-time specialised line-FFT kernels of the hot path.
+//
+// Line length N, lines per tile L and threads NT are template parameters, so
+// the radix schedule, the stage loop and every index computation fold to
+// constants and shifts. Twiddles come from a small global table (N entries,
+// or N0 = 2M for the half-length real transforms) that stays cache resident,
+// so the whole LDS budget holds the tile.
+//
+// Tiles of the strided passes are L consecutive kx of one (k_mid) row, so
+// global loads and stores move L*sizeof(complex) contiguous bytes per
+// element; the Nyquist column kx = N0/2 forms its own tiles of L
+// consecutive rows, which keeps H = N0/2+1 (odd) out of every division.
+#pragma once
+
+#include "pf_common.hpp"
+#include "pf_fft.hpp"
+
+namespace pfamd {
+
+// --- constexpr radix schedule (same rule as the host's make_sched) --------
+constexpr int ct_p2(int n)
+{
+  int p = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    p++;
+  }
+  return p;
+}
+
+constexpr int ct_radix(int n, int st)
+{
+  int p2 = ct_p2(n), m = n;
+  while (m % 2 == 0)
+    m /= 2;
+  int n8 = p2 / 3, rem = p2 % 3;
+  int t0 = 0, t1 = 0, nt2 = 0;
+  if (rem == 1 && n8 > 0) {
+    n8--;
+    t0 = 4;
+    t1 = 4;
+    nt2 = 2;
+  } else if (rem == 1) {
+    t0 = 2;
+    nt2 = 1;
+  } else if (rem == 2) {
+    t0 = 4;
+    nt2 = 1;
+  }
+  if (st < n8)
+    return 8;
+  st -= n8;
+  if (st < nt2)
+    return st == 0 ? t0 : t1;
+  st -= nt2;
+  while (m % 3 == 0) {
+    if (st == 0)
+      return 3;
+    st--;
+    m /= 3;
+  }
+  while (m % 5 == 0) {
+    if (st == 0)
+      return 5;
+    st--;
+    m /= 5;
+  }
+  while (m % 7 == 0) {
+    if (st == 0)
+      return 7;
+    st--;
+    m /= 7;
+  }
+  return 0;
+}
+
+constexpr int ct_nst(int n)
+{
+  int s = 0;
+  while (ct_radix(n, s) != 0)
+    s++;
+  return s;
+}
+
+// One Stockham stage; tile layout buf[e*SE + l], SE = L + PAD; lane mapping
+// l fastest. Ends with a barrier.
+template <typename T, int N, int L, int NT, int PAD, int R, int NS, bool INV, int TWM>
+__device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw)
+{
+  constexpr int SE = L + PAD;
+  constexpr int NR = N / R;
+  constexpr int TOTAL = L * NR;
+  constexpr int IT = (TOTAL + NT - 1) / NT;
+  constexpr int TSTEP = (N / (NS * R)) * TWM;
+  cplx<T> v[IT][R];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int w = threadIdx.x + it * NT;
+    if (TOTAL % NT == 0 || w < TOTAL) {
+      const int l = w % L, j = w / L;
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        v[it][r] = buf[(j + r * NR) * SE + l];
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          cplx<T> t = tw[r * k * TSTEP];
+          if (INV)
+            t.y = -t.y;
+          v[it][r] = cmul(v[it][r], t);
+        }
+      }
+      Butterfly<T, R, INV>::run(v[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int w = threadIdx.x + it * NT;
+    if (TOTAL % NT == 0 || w < TOTAL) {
+      const int l = w % L, j = w / L;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        buf[(base + r * NS) * SE + l] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int L, int NT, int PAD, bool INV, int TWM, int ST = 0, int NS = 1>
+__device__ __forceinline__ void tfft(cplx<T>* buf, const cplx<T>* tw)
+{
+  if constexpr (ST < ct_nst(N)) {
+    constexpr int R = ct_radix(N, ST);
+    tstage<T, N, L, NT, PAD, R, NS, INV, TWM>(buf, tw);
+    tfft<T, N, L, NT, PAD, INV, TWM, ST + 1, NS * R>(buf, tw);
+  }
+}
+
+// Geometry of the half spectrum as the generation / matvec last-axis passes
+// see it: line (kx, km) with elements e along the last axis.
+struct LastAxis
+{
+  int H, N0h;        // H = N0/2 + 1, N0h = N0/2
+  int Nmid;          // rows of lines (3D: N1, 2D: 1)
+  int nlast;         // pruned output length
+  int64_t sA, sB;    // storage index base = H * (e*sA + km*sB)
+};
+
+__device__ __forceinline__ bool self_conj(int k, int N) { return k == 0 || 2 * k == N; }
+
+template <typename T>
+__device__ __forceinline__ cplx<T> make_entry(T lambda, T r1, T r2, bool selfconj)
+{
+  if (selfconj)
+    return { lambda * r1, T(0) };
+  const T ls = lambda / (T)1.41421356237309504880;
+  return { ls * r1, ls * r2 };
+}
+
+// logical (k0, k1, k2) of line (kx, km) element e
+__device__ __forceinline__ void line_k(const GenGeom& g, int kx, int km, int e, int& k1, int& k2)
+{
+  if (g.dim == 3) {
+    k1 = km;
+    k2 = e;
+  } else {
+    k1 = e;
+    k2 = 0;
+  }
+}
+
+// Pass G: spectrum generated in registers along the last axis (length N),
+// inverse DFT, first nlast outputs -> B1[j][km][kx].
+template <typename T, int N, int L, int NT, int NOISE>
+__global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T* sq,
+                                                  const double* noise, uint64_t seed,
+                                                  uint64_t first_real, cplx<T>* B1,
+                                                  const cplx<T>* tw)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+  const int ntile = a.N0h / L;
+  const bool nyq = (int)blockIdx.x >= ntile;
+  int kx0, km0;
+  if (!nyq) {
+    kx0 = blockIdx.x * L;
+    km0 = blockIdx.y;
+  } else {
+    kx0 = a.N0h;
+    km0 = blockIdx.y * L;
+    if (km0 >= a.Nmid)
+      return;
+  }
+  const int b = blockIdx.z;
+  const uint64_t real = first_real + b;
+  const double* nz = noise ? noise + (int64_t)b * g.noise_stride : nullptr;
+  if (NOISE == 1 && !nyq) {
+    // Philox pairs: entries s and s+1 (s even) share one block; with the row
+    // base parity odd the pairs start one entry to the left of the tile.
+    constexpr int SLOTS = L / 2 + 1;
+    for (int slot = threadIdx.x; slot < SLOTS * N; slot += NT) {
+      const int p = slot % SLOTS, e = slot / SLOTS;
+      int k1, k2;
+      line_k(g, kx0, km0, e, k1, k2);
+      const int64_t base = (int64_t)a.H * ((int64_t)e * a.sA + (int64_t)km0 * a.sB) + kx0;
+      const int odd = (int)(base & 1);
+      const int la = 2 * p - odd, lb = la + 1;
+      if (la >= L || lb < 0)
+        continue;
+      const uint64_t blk = (uint64_t)(base + la) >> 1;
+      uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
+                        (uint32_t)(real >> 32) };
+      philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+      const bool sc12 = self_conj(k1, g.N1) && self_conj(k2, g.N2);
+      if (la >= 0) {
+        T r1, r2;
+        BoxMuller<T>::pair(c[0], c[1], r1, r2);
+        const int kx = kx0 + la;
+        const T lam = sq[eig_index(g, kx, k1, k2)];
+        buf[e * L + la] = make_entry<T>(lam, r1, r2, sc12 && self_conj(kx, g.N0));
+      }
+      if (lb < L) {
+        T r1, r2;
+        BoxMuller<T>::pair(c[2], c[3], r1, r2);
+        const int kx = kx0 + lb;
+        const T lam = sq[eig_index(g, kx, k1, k2)];
+        buf[e * L + lb] = make_entry<T>(lam, r1, r2, sc12 && self_conj(kx, g.N0));
+      }
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < L * N; idx += NT) {
+      const int l = idx % L, e = idx / L;
+      const int kx = nyq ? kx0 : kx0 + l;
+      const int km = nyq ? km0 + l : km0;
+      cplx<T> z = { 0, 0 };
+      if (km < a.Nmid) {
+        int k1, k2;
+        line_k(g, kx, km, e, k1, k2);
+        const int64_t s = (int64_t)a.H * ((int64_t)e * a.sA + (int64_t)km * a.sB) + kx;
+        T r1, r2;
+        if (NOISE == 0) {
+          r1 = (T)nz[2 * s];
+          r2 = (T)nz[2 * s + 1];
+        } else {
+          philox_pair<T>(seed, real, (uint64_t)s, r1, r2);
+        }
+        const T lam = sq[eig_index(g, kx, k1, k2)];
+        z = make_entry<T>(lam, r1, r2,
+                          self_conj(kx, g.N0) && self_conj(k1, g.N1) && self_conj(k2, g.N2));
+      }
+      buf[e * L + l] = z;
+    }
+  }
+  __syncthreads();
+  tfft<T, N, L, NT, 0, true, 1>(buf, tw);
+  const int64_t plane = (int64_t)a.H * a.Nmid;
+  cplx<T>* out = B1 + (int64_t)b * plane * a.nlast;
+  for (int idx = threadIdx.x; idx < L * a.nlast; idx += NT) {
+    const int l = idx % L, j = idx / L;
+    const int kx = nyq ? kx0 : kx0 + l;
+    const int km = nyq ? km0 + l : km0;
+    if (km < a.Nmid)
+      out[j * plane + (int64_t)km * a.H + kx] = buf[j * L + l];
+  }
+}
+
+// Pass along the middle axis (3D): lines (kx, z) of in[z][e][kx] (in_rows
+// rows per z, n_in leading points, rest zero), DFT, first n_out points ->
+// out[z][j][kx] (out_rows rows per z).
+template <typename T, int N, int L, int NT, bool INV>
+__global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, int H, int N0h,
+                                             int nz, int in_rows, int n_in, int out_rows,
+                                             int n_out, const cplx<T>* tw)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+  const int ntile = N0h / L;
+  const bool nyq = (int)blockIdx.x >= ntile;
+  int kx0, z0;
+  if (!nyq) {
+    kx0 = blockIdx.x * L;
+    z0 = blockIdx.y;
+  } else {
+    kx0 = N0h;
+    z0 = blockIdx.y * L;
+    if (z0 >= nz)
+      return;
+  }
+  const int b = blockIdx.z;
+  in += (int64_t)b * H * nz * in_rows;
+  out += (int64_t)b * H * nz * out_rows;
+  for (int idx = threadIdx.x; idx < L * N; idx += NT) {
+    const int l = idx % L, e = idx / L;
+    const int kx = nyq ? kx0 : kx0 + l;
+    const int z = nyq ? z0 + l : z0;
+    cplx<T> v = { 0, 0 };
+    if (e < n_in && z < nz)
+      v = in[((int64_t)z * in_rows + e) * H + kx];
+    buf[e * L + l] = v;
+  }
+  __syncthreads();
+  tfft<T, N, L, NT, 0, INV, 1>(buf, tw);
+  for (int idx = threadIdx.x; idx < L * n_out; idx += NT) {
+    const int l = idx % L, j = idx / L;
+    const int kx = nyq ? kx0 : kx0 + l;
+    const int z = nyq ? z0 + l : z0;
+    if (z < nz)
+      out[((int64_t)z * out_rows + j) * H + kx] = buf[j * L + l];
+  }
+}
+
+// c2r along x (see k_c2r_x for the algebra), M = N0/2 point inverse FFT on a
+// padded tile of L rows; first n0 outputs with the value transform.
+template <typename T, typename TO, int M, int L, int NT>
+__global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H,
+                                               int n0, const cplx<T>* tw, int transform, T param)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int N0 = 2 * M, SE = L + 1;
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+  const int b = blockIdx.y;
+  in += (int64_t)b * rows * H;
+  out += (int64_t)b * rows * n0;
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * M; idx += NT) {
+    const int l = idx / M, k = idx % M;
+    const int64_t r = row0 + l;
+    cplx<T> z = { 0, 0 };
+    if (r < rows) {
+      cplx<T> xk = in[r * H + k];
+      cplx<T> xm = in[r * H + (M - k)];
+      if (k == 0) {
+        xk.y = 0;
+        xm.y = 0;
+      }
+      const cplx<T> s = { xk.x + xm.x, xk.y - xm.y };
+      const cplx<T> d = { xk.x - xm.x, xk.y + xm.y };
+      const cplx<T> td = cmul(cconj(tw[k]), d);
+      z = { s.x - td.y, s.y + td.x };
+    }
+    buf[k * SE + l] = z;
+  }
+  __syncthreads();
+  tfft<T, M, L, NT, 1, true, 2>(buf, tw);
+  for (int idx = threadIdx.x; idx < L * n0; idx += NT) {
+    const int l = idx / n0, x = idx % n0;
+    const int64_t r = row0 + l;
+    if (r < rows) {
+      const cplx<T> v = buf[(x >> 1) * SE + l];
+      out[r * n0 + x] = (TO)value_transform<T>(transform, (x & 1) ? v.y : v.x, param);
+    }
+  }
+}
+
+// r2c along x of zero-padded real rows (n0 of N0 values), see k_r2c_x.
+template <typename T, typename TI, int M, int L, int NT>
+__global__ void __launch_bounds__(NT) kf_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H,
+                                               int n0, const cplx<T>* tw)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int N0 = 2 * M, SE = L + 1;
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+  const int b = blockIdx.y;
+  in += (int64_t)b * rows * n0;
+  out += (int64_t)b * rows * H;
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * M; idx += NT) {
+    const int l = idx / M, j = idx % M;
+    const int64_t r = row0 + l;
+    cplx<T> z = { 0, 0 };
+    if (r < rows) {
+      const int x0 = 2 * j, x1 = 2 * j + 1;
+      z.x = x0 < n0 ? (T)in[r * n0 + x0] : T(0);
+      z.y = x1 < n0 ? (T)in[r * n0 + x1] : T(0);
+    }
+    buf[j * SE + l] = z;
+  }
+  __syncthreads();
+  tfft<T, M, L, NT, 1, false, 2>(buf, tw);
+  for (int idx = threadIdx.x; idx < L * H; idx += NT) {
+    const int l = idx / H, k = idx % H;
+    const int64_t r = row0 + l;
+    if (r < rows) {
+      const cplx<T> zk = buf[(k % M) * SE + l];
+      const cplx<T> zm = buf[((M - k) % M) * SE + l];
+      const cplx<T> s = { (zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5) };
+      const cplx<T> d = { (zk.x - zm.x) * T(0.5), (zk.y + zm.y) * T(0.5) };
+      const cplx<T> wd = cmul(tw[k], d);
+      out[r * H + k] = { s.x + wd.y, s.y - wd.x };
+    }
+  }
+}
+
+// Fused last-axis pass of the matvec: forward DFT of n_last leading points,
+// times f(k)/N_ext, inverse DFT, first n_last points back in place.
+template <typename T, int N, int L, int NT>
+__global__ void __launch_bounds__(NT) kf_fused_mult(cplx<T>* C, GenGeom g, LastAxis a,
+                                                    const double* eig, int op, double inv_next,
+                                                    const cplx<T>* tw, const double* spec,
+                                                    int nbins)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+  const int ntile = a.N0h / L;
+  const bool nyq = (int)blockIdx.x >= ntile;
+  int kx0, km0;
+  if (!nyq) {
+    kx0 = blockIdx.x * L;
+    km0 = blockIdx.y;
+  } else {
+    kx0 = a.N0h;
+    km0 = blockIdx.y * L;
+    if (km0 >= a.Nmid)
+      return;
+  }
+  const int64_t plane = (int64_t)a.H * a.Nmid;
+  C += (int64_t)blockIdx.z * plane * a.nlast;
+  for (int idx = threadIdx.x; idx < L * N; idx += NT) {
+    const int l = idx % L, e = idx / L;
+    const int kx = nyq ? kx0 : kx0 + l;
+    const int km = nyq ? km0 + l : km0;
+    cplx<T> v = { 0, 0 };
+    if (e < a.nlast && km < a.Nmid)
+      v = C[e * plane + (int64_t)km * a.H + kx];
+    buf[e * L + l] = v;
+  }
+  __syncthreads();
+  tfft<T, N, L, NT, 0, false, 1>(buf, tw);
+  for (int idx = threadIdx.x; idx < L * N; idx += NT) {
+    const int l = idx % L, e = idx / L;
+    const int kx = nyq ? kx0 : kx0 + l;
+    const int km = nyq ? km0 + l : km0;
+    if (km < a.Nmid) {
+      int k1, k2;
+      line_k(g, kx, km, e, k1, k2);
+      double f;
+      if (op == PF_OP_SPECTRAL)
+        f = spectral_factor(g, kx, k1, k2, spec, nbins);
+      else {
+        const double m = eig[eig_index(g, kx, k1, k2)];
+        f = op == PF_OP_MULTIPLY ? m : (op == PF_OP_ROOT ? sqrt(m) : 1. / m);
+      }
+      f *= inv_next;
+      buf[e * L + l] = cscale(buf[e * L + l], (T)f);
+    }
+  }
+  __syncthreads();
+  tfft<T, N, L, NT, 0, true, 1>(buf, tw);
+  for (int idx = threadIdx.x; idx < L * a.nlast; idx += NT) {
+    const int l = idx % L, j = idx / L;
+    const int kx = nyq ? kx0 : kx0 + l;
+    const int km = nyq ? km0 + l : km0;
+    if (km < a.Nmid)
+      C[j * plane + (int64_t)km * a.H + kx] = buf[j * L + l];
+  }
+}
+
+} // namespace pfamd

@@ -1,0 +1,5 @@
+// FP64 instantiations of the specialised line-FFT kernels.
+#include "pf_fast_impl.hpp"
+namespace pfamd {
+PF_INSTANTIATE(double)
+}

@@ -1,0 +1,201 @@
+// pf_fast_impl.hpp -- size dispatch of the specialised kernels (included by
+// one translation unit per precision).
+#pragma once
+
+#include "pf_fast.hpp"
+#include "pf_fast_launch.hpp"
+
+namespace pfamd {
+
+// Lines per tile: the largest power of two with L*N complex values in 64 KB
+// (FP32: 16 lines x 512; FP64: 8 x 512), one thread per MAXE values.
+template <typename T>
+constexpr int fast_L(int N)
+{
+  int L = sizeof(T) == 4 ? 16 : 8;
+  const int lim = sizeof(T) == 4 ? 8192 : 4096;
+  while (L > 1 && L * N > lim)
+    L /= 2;
+  return L;
+}
+
+template <typename T>
+constexpr int fast_NT(int N, int L)
+{
+  const int me = sizeof(T) == 4 ? 32 : 16;
+  int nt = (L * N + me - 1) / me;
+  nt = ((nt + 63) / 64) * 64;
+  return nt < 64 ? 64 : nt;
+}
+
+#define PF_FAST_SIZES(X)                                                                           \
+  X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048)
+
+template <typename T, int N>
+bool gen_last_n(const FastCtx& c, const GenGeom& g, const LastAxis& a, const T* sq,
+                const double* noise, uint64_t seed, uint64_t first, cplx<T>* B1,
+                const cplx<T>* tw, int batch)
+{
+  constexpr int L = fast_L<T>(N), NT = fast_NT<T>(N, L);
+  const size_t lds = (size_t)L * N * sizeof(cplx<T>);
+  if (a.N0h % L != 0 || lds > c.lds_budget)
+    return false;
+  dim3 grid(a.N0h / L + 1, a.Nmid, batch);
+  if (noise)
+    kf_gen_last<T, N, L, NT, 0><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first, B1, tw);
+  else
+    kf_gen_last<T, N, L, NT, 1><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first, B1, tw);
+  return true;
+}
+
+template <typename T>
+bool fast_gen_last(const FastCtx& c, int N, const GenGeom& g, const LastAxis& a, const T* sq,
+                   const double* noise, uint64_t seed, uint64_t first, cplx<T>* B1,
+                   const cplx<T>* tw, int batch)
+{
+  switch (N) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return gen_last_n<T, n>(c, g, a, sq, noise, seed, first, B1, tw, batch);
+    PF_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int N>
+bool mid_n(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, int N0h, int nz,
+           int in_rows, int n_in, int out_rows, int n_out, const cplx<T>* tw, int batch)
+{
+  constexpr int L = fast_L<T>(N), NT = fast_NT<T>(N, L);
+  const size_t lds = (size_t)L * N * sizeof(cplx<T>);
+  if (N0h % L != 0 || lds > c.lds_budget)
+    return false;
+  dim3 grid(N0h / L + 1, nz, batch);
+  if (inv)
+    kf_mid<T, N, L, NT, true><<<grid, NT, lds, c.stream>>>(in, out, H, N0h, nz, in_rows, n_in,
+                                                           out_rows, n_out, tw);
+  else
+    kf_mid<T, N, L, NT, false><<<grid, NT, lds, c.stream>>>(in, out, H, N0h, nz, in_rows, n_in,
+                                                            out_rows, n_out, tw);
+  return true;
+}
+
+template <typename T>
+bool fast_mid(const FastCtx& c, int N, bool inv, const cplx<T>* in, cplx<T>* out, int H, int N0h,
+              int nz, int in_rows, int n_in, int out_rows, int n_out, const cplx<T>* tw,
+              int batch)
+{
+  switch (N) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return mid_n<T, n>(c, inv, in, out, H, N0h, nz, in_rows, n_in, out_rows, n_out, tw, batch);
+    PF_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int M>
+bool c2r_n(const FastCtx& c, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
+           const cplx<T>* tw, int transform, T param, int batch)
+{
+  constexpr int L = fast_L<T>(M), NT = fast_NT<T>(M, L);
+  const size_t lds = (size_t)(L + 1) * M * sizeof(cplx<T>);
+  if (lds > c.lds_budget)
+    return false;
+  dim3 grid((unsigned)((rows + L - 1) / L), batch);
+  kf_c2r_x<T, T, M, L, NT><<<grid, NT, lds, c.stream>>>(in, out, rows, H, n0, tw, transform, param);
+  return true;
+}
+
+template <typename T>
+bool fast_c2r(const FastCtx& c, int M, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
+              const cplx<T>* tw, int transform, T param, int batch)
+{
+  switch (M) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return c2r_n<T, n>(c, in, out, rows, H, n0, tw, transform, param, batch);
+    PF_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int M>
+bool r2c_n(const FastCtx& c, const T* in, cplx<T>* out, int64_t rows, int H, int n0,
+           const cplx<T>* tw, int batch)
+{
+  constexpr int L = fast_L<T>(M), NT = fast_NT<T>(M, L);
+  const size_t lds = (size_t)(L + 1) * M * sizeof(cplx<T>);
+  if (lds > c.lds_budget)
+    return false;
+  dim3 grid((unsigned)((rows + L - 1) / L), batch);
+  kf_r2c_x<T, T, M, L, NT><<<grid, NT, lds, c.stream>>>(in, out, rows, H, n0, tw);
+  return true;
+}
+
+template <typename T>
+bool fast_r2c(const FastCtx& c, int M, const T* in, cplx<T>* out, int64_t rows, int H, int n0,
+              const cplx<T>* tw, int batch)
+{
+  switch (M) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return r2c_n<T, n>(c, in, out, rows, H, n0, tw, batch);
+    PF_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int N>
+bool fused_n(const FastCtx& c, cplx<T>* C, const GenGeom& g, const LastAxis& a, const double* eig,
+             int op, double inv_next, const cplx<T>* tw, int batch, const double* spec, int nbins)
+{
+  constexpr int L = fast_L<T>(N), NT = fast_NT<T>(N, L);
+  const size_t lds = (size_t)L * N * sizeof(cplx<T>);
+  if (a.N0h % L != 0 || lds > c.lds_budget)
+    return false;
+  dim3 grid(a.N0h / L + 1, a.Nmid, batch);
+  kf_fused_mult<T, N, L, NT><<<grid, NT, lds, c.stream>>>(C, g, a, eig, op, inv_next, tw, spec,
+                                                          nbins);
+  return true;
+}
+
+template <typename T>
+bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const LastAxis& a,
+                const double* eig, int op, double inv_next, const cplx<T>* tw, int batch,
+                const double* spec, int nbins)
+{
+  switch (N) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return fused_n<T, n>(c, C, g, a, eig, op, inv_next, tw, batch, spec, nbins);
+    PF_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+#define PF_INSTANTIATE(T)                                                                          \
+  template bool fast_gen_last<T>(const FastCtx&, int, const GenGeom&, const LastAxis&, const T*,   \
+                                 const double*, uint64_t, uint64_t, cplx<T>*, const cplx<T>*,      \
+                                 int);                                                             \
+  template bool fast_mid<T>(const FastCtx&, int, bool, const cplx<T>*, cplx<T>*, int, int, int,    \
+                            int, int, int, int, const cplx<T>*, int);                              \
+  template bool fast_c2r<T>(const FastCtx&, int, const cplx<T>*, T*, int64_t, int, int,            \
+                            const cplx<T>*, int, T, int);                                          \
+  template bool fast_r2c<T>(const FastCtx&, int, const T*, cplx<T>*, int64_t, int, int,            \
+                            const cplx<T>*, int);                                                  \
+  template bool fast_fused<T>(const FastCtx&, int, cplx<T>*, const GenGeom&, const LastAxis&,      \
+                              const double*, int, double, const cplx<T>*, int, const double*,      \
+                              int);
+
+} // namespace pfamd

@@ -1,0 +1,38 @@
+// pf_fast_launch.hpp -- launchers of the compile-time specialised kernels.
+// Each returns false when the line length has no specialisation or the tile
+// constraints do not hold; the caller then uses the generic kernels.
+#pragma once
+
+#include "pf_common.hpp"
+#include "pf_fft.hpp"
+
+namespace pfamd {
+
+struct LastAxis;
+
+struct FastCtx
+{
+  hipStream_t stream;
+  size_t lds_budget;
+};
+
+template <typename T>
+bool fast_gen_last(const FastCtx& c, int N, const GenGeom& g, const LastAxis& a, const T* sq,
+                   const double* noise, uint64_t seed, uint64_t first, cplx<T>* B1,
+                   const cplx<T>* tw, int batch);
+template <typename T>
+bool fast_mid(const FastCtx& c, int N, bool inv, const cplx<T>* in, cplx<T>* out, int H, int N0h,
+              int nz, int in_rows, int n_in, int out_rows, int n_out, const cplx<T>* tw,
+              int batch);
+template <typename T>
+bool fast_c2r(const FastCtx& c, int M, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
+              const cplx<T>* tw, int transform, T param, int batch);
+template <typename T>
+bool fast_r2c(const FastCtx& c, int M, const T* in, cplx<T>* out, int64_t rows, int H, int n0,
+              const cplx<T>* tw, int batch);
+template <typename T>
+bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const LastAxis& a,
+                const double* eig, int op, double inv_next, const cplx<T>* tw, int batch,
+                const double* spec, int nbins);
+
+} // namespace pfamd

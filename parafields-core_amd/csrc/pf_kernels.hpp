@@ -9,6 +9,7 @@
 //   field         T [n2][n1][n0]
 #pragma once
 
+#include "pf_common.hpp"
 #include "pf_fft.hpp"
 
 namespace pfamd {
@@ -281,36 +282,6 @@ __global__ void k_sqrt_table(const double* eig, T* out, int64_t n)
 // Field generation, R2C field backend (matrix.hh:374-421,
 // r2cfieldbackend.hh:250-325, :423-441).
 // ---------------------------------------------------------------------------
-struct GenGeom
-{
-  int dim, transposed;
-  int N0, N1, N2, H;
-  int n0, n1, n2;
-  int P0, P1, P2;       // DCT half sizes (eigen layout of the DCT backend)
-  int dct;              // 1: eigen table is the DCT half array (fold k >= P)
-  int64_t noise_stride; // samples per realization
-};
-
-// Reference storage index of half-spectrum entry (kx, k1, k2): the order in
-// which generateField draws the noise.
-__device__ __forceinline__ int64_t storage_index(const GenGeom& g, int kx, int k1, int k2)
-{
-  if (g.dim == 3 && g.transposed)
-    return kx + (int64_t)g.H * (k2 + (int64_t)g.N2 * k1);
-  return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
-}
-
-__device__ __forceinline__ int64_t eig_index(const GenGeom& g, int kx, int k1, int k2)
-{
-  if (g.dct) {
-    const int a = kx >= g.P0 ? g.N0 - kx : kx;
-    const int b = k1 >= g.P1 ? g.N1 - k1 : k1;
-    const int c = k2 >= g.P2 ? g.N2 - k2 : k2;
-    return a + (int64_t)g.P0 * (b + (int64_t)g.P1 * c);
-  }
-  return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
-}
-
 // Spectrum entry: R2CFieldBackend::set, r2cfieldbackend.hh:305-325.
 template <typename T, int NOISE>
 __device__ __forceinline__ cplx<T> spectrum_entry(const GenGeom& g, const T* sq, int kx, int k1,
@@ -518,7 +489,7 @@ __global__ void k_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H, int n0,
 // f(k)/N_ext -> inverse DFT -> first n_in points back in place.
 template <typename T, int MAXE>
 __global__ void k_fused_mult(cplx<T>* C, GenGeom g, const double* eig, int op, double inv_next,
-                             Sched s, const cplx<T>* tw, int L)
+                             Sched s, const cplx<T>* tw, int L, const double* spec, int nbins)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = s.n;
@@ -546,9 +517,14 @@ __global__ void k_fused_mult(cplx<T>* C, GenGeom g, const double* eig, int op, d
     const int64_t u = line0 + l;
     if (u < nlines) {
       const int kx = (int)(u % g.H), km = (int)(u / g.H);
-      const int64_t ei = (g.dim == 3) ? eig_index(g, kx, km, e) : eig_index(g, kx, e, 0);
-      const double m = eig[ei];
-      double f = op == PF_OP_MULTIPLY ? m : (op == PF_OP_ROOT ? sqrt(m) : 1. / m);
+      const int k1 = (g.dim == 3) ? km : e, k2 = (g.dim == 3) ? e : 0;
+      double f;
+      if (op == PF_OP_SPECTRAL)
+        f = spectral_factor(g, kx, k1, k2, spec, nbins);
+      else {
+        const double m = eig[eig_index(g, kx, k1, k2)];
+        f = op == PF_OP_MULTIPLY ? m : (op == PF_OP_ROOT ? sqrt(m) : 1. / m);
+      }
       f *= inv_next;
       buf[e * L + l] = cscale(buf[e * L + l], (T)f);
     }

@@ -181,6 +181,19 @@ def load(path: str = LIB_PATH):
         "pf_philox_normals": (i32, [vp, u64, u64, u64, i64, vp, i32]),
         "pf_philox_raw": (i32, [vp, vp, vp, i64, vp, i32]),
     }
+    sig.update({
+        "pf_spectral_targets": (i32, [vp, i32, dp, vp]),
+        "pf_mlp_create": (i32, [vp, i32, u64, ctypes.POINTER(vp)]),
+        "pf_mlp_destroy": (i32, [vp]),
+        "pf_mlp_nparams": (i32, [vp, ctypes.POINTER(i64)]),
+        "pf_mlp_params": (i32, [vp, vp, i64, i32]),
+        "pf_mlp_forward": (i32, [vp, vp, i32, vp]),
+        "pf_mlp_gradients": (i32, [vp, vp, vp, i32, vp, ctypes.POINTER(dp)]),
+        "pf_mlp_train": (i32, [vp, vp, vp, i32, i32, ctypes.c_float, ctypes.POINTER(dp)]),
+        "pf_set_profiling": (i32, [vp, i32]),
+        "pf_last_timings": (i32, [vp, ctypes.c_char_p, i32, ctypes.POINTER(dp), i32,
+                                  ctypes.POINTER(i32)]),
+    })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -193,7 +206,10 @@ def exported_symbols():
     """Names every caller may bind (must all be exported by the library)."""
     return ["pf_plan_create", "pf_plan_destroy", "pf_plan_sizes", "pf_last_error",
             "pf_device_sync", "pf_eigs", "pf_eigs_custom", "pf_eigs_copy", "pf_generate",
-            "pf_matvec", "pf_cg", "pf_value_transform", "pf_philox_normals", "pf_philox_raw"]
+            "pf_matvec", "pf_cg", "pf_value_transform", "pf_philox_normals", "pf_philox_raw",
+            "pf_spectral_targets", "pf_mlp_create", "pf_mlp_destroy", "pf_mlp_nparams",
+            "pf_mlp_params", "pf_mlp_forward", "pf_mlp_gradients", "pf_mlp_train",
+            "pf_set_profiling", "pf_last_timings"]
 
 
 def check(status: int):
@@ -325,3 +341,78 @@ class Plan:
 
     def sync(self):
         check(self.lib.pf_device_sync(self.h))
+
+    def spectral_targets(self, nbins: int, floor_value: float) -> np.ndarray:
+        out = np.empty(nbins, dtype=np.float64)
+        check(self.lib.pf_spectral_targets(self.h, nbins, floor_value, out.ctypes.data))
+        return out
+
+    def set_profiling(self, on: bool):
+        check(self.lib.pf_set_profiling(self.h, int(on)))
+
+    def last_timings(self):
+        names = ctypes.create_string_buffer(8192)
+        ms = (ctypes.c_double * 256)()
+        n = ctypes.c_int32()
+        check(self.lib.pf_last_timings(self.h, names, 8192, ms, 256, ctypes.byref(n)))
+        labels = names.value.decode().split(";")[: n.value]
+        return list(zip(labels, [ms[i] for i in range(min(n.value, 256))]))
+
+
+class Mlp:
+    """The learned preconditioner's network: FP32, exact-GELU hidden layers,
+    trained on the device by the library's own kernels."""
+
+    def __init__(self, dims, seed: int = 0):
+        self.lib = load()
+        self.dims = list(dims)
+        d = np.ascontiguousarray(self.dims, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(self.lib.pf_mlp_create(d.ctypes.data, len(self.dims), seed, ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int64()
+        check(self.lib.pf_mlp_nparams(self.h, ctypes.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if self.h:
+            self.lib.pf_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        check(self.lib.pf_mlp_params(self.h, out.ctypes.data, self.nparams, 0))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(self.lib.pf_mlp_params(self.h, p.ctypes.data, self.nparams, 1))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((x.shape[0], self.dims[-1]), dtype=np.float32)
+        check(self.lib.pf_mlp_forward(self.h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
+    def gradients(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = ctypes.c_double()
+        check(self.lib.pf_mlp_gradients(self.h, x.ctypes.data, y.ctypes.data, x.shape[0],
+                                        g.ctypes.data, ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, y, steps: int, lr: float) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_double()
+        check(self.lib.pf_mlp_train(self.h, x.ctypes.data, y.ctypes.data, x.shape[0], steps, lr,
+                                    ctypes.byref(loss)))
+        return loss.value

@@ -46,6 +46,10 @@ def load():
         "pfo_philox_raw": (i32, [vp, vp, ctypes.c_longlong, vp]),
         "pfo_cov_eval": (i32, [D, vp, ctypes.c_longlong, vp]),
         "pfo_bessel_k": (dp, [dp, dp]),
+        "pfo_pcg_learned": (i32, [D, vp, i32, i32, dp, vp, vp, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_double)]),
+        "pfo_spectral_targets": (i32, [D, i32, dp, vp]),
+        "pfo_mlp": (dp, [vp, i32, vp, vp, vp, i32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -157,3 +161,34 @@ def cov_eval(desc, pts):
 
 def bessel_k(nu, x):
     return load().pfo_bessel_k(nu, x)
+
+
+def pcg_learned(desc, b, log_mult, maxit=1000, rel_tol=1e-8):
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    lm = np.ascontiguousarray(log_mult, dtype=np.float64)
+    out = np.empty_like(bb)
+    it = ctypes.c_int()
+    rr = ctypes.c_double()
+    _check(load().pfo_pcg_learned(ctypes.byref(desc), lm.ctypes.data, lm.size, maxit, rel_tol,
+                                  bb.ctypes.data, out.ctypes.data, ctypes.byref(it),
+                                  ctypes.byref(rr)))
+    return out, it.value, rr.value
+
+
+def spectral_targets(desc, nbins, floor_value):
+    out = np.empty(nbins)
+    _check(load().pfo_spectral_targets(ctypes.byref(desc), nbins, floor_value, out.ctypes.data))
+    return out
+
+
+def mlp(dims, params, x, y, grads=True):
+    """(loss, grads or None, outputs) of the FP64 reference MLP."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    yy = np.ascontiguousarray(y, dtype=np.float64)
+    g = np.empty(p.size) if grads else None
+    out = np.empty((xx.shape[0], dims[-1]))
+    loss = load().pfo_mlp(d.ctypes.data, d.size, p.ctypes.data, xx.ctypes.data, yy.ctypes.data,
+                          xx.shape[0], None if g is None else g.ctypes.data, out.ctypes.data)
+    return loss, g, out

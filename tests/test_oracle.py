@@ -381,3 +381,65 @@ def test_value_transforms():
     lam = 0.3
     np.testing.assert_allclose(orc.value_transform(v, "boxCox", lam),
                                np.power(v * lam + 1, -lam), rtol=1e-14)
+
+
+# ------------------------------------------ learned preconditioner (oracle) --
+
+def test_mlp_backward_matches_central_differences():
+    dims = [6, 16, 16, 16, 8]
+    rng = np.random.default_rng(3)
+    n = sum(dims[i] * dims[i + 1] + dims[i + 1] for i in range(len(dims) - 1))
+    p = rng.normal(0, 0.4, n)
+    x = rng.normal(size=(7, 6))
+    y = rng.normal(size=(7, 8))
+    loss, g, _ = orc.mlp(dims, p, x, y)
+    h = 1e-6
+    for i in rng.choice(n, 60, replace=False):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (orc.mlp(dims, pp, x, y, grads=False)[0] - orc.mlp(dims, pm, x, y, grads=False)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+    # forward = explicit layer composition with exact GELU
+    a = x
+    off = 0
+    for l in range(len(dims) - 1):
+        W = p[off:off + dims[l] * dims[l + 1]].reshape(dims[l + 1], dims[l])
+        off += dims[l] * dims[l + 1]
+        b = p[off:off + dims[l + 1]]
+        off += dims[l + 1]
+        z = a @ W.T + b
+        a = 0.5 * z * (1 + special.erf(z / np.sqrt(2))) if l + 2 < len(dims) else z
+    assert abs(loss - np.mean((a - y) ** 2)) < 1e-12
+
+
+def test_spectral_targets_vs_numpy():
+    cells = [8, 6, 4]
+    d = desc(3, cells, covariance="exponential", corr_length=0.2, nugget=1e-3)
+    s = orc.sizes(d)
+    floor = 1e-3 / s.extended_size
+    got = orc.spectral_targets(d, 12, floor)
+    v, _, _ = orc.eigs(d)
+    P = [c + 1 for c in cells]
+    lam = v.reshape(P[1], P[2], P[0]).transpose(1, 0, 2)  # natural [kz][ky][kx]
+    kz, ky, kx = np.meshgrid(*[np.arange(q) for q in P[::-1]], indexing="ij")
+    r = np.sqrt(((kx / cells[0]) ** 2 + (ky / cells[1]) ** 2 + (kz / cells[2]) ** 2) / 3)
+    b = np.minimum((r * 12).astype(int), 11)
+    want = np.array([-np.log(np.maximum(lam[b == i], floor).mean()) for i in range(12)])
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+def test_learned_pcg_with_method_targets_solves_dense_system():
+    cells = [6, 5, 4]
+    d = desc(3, cells, covariance="exponential", corr_length=0.15, nugget=1e-3)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0] * 3, "exponential", "none", [0.15])
+    C = (C + 1e-3 * np.eye(C.shape[0])) / s.extended_size
+    b = np.random.default_rng(9).standard_normal(C.shape[0])
+    t = orc.spectral_targets(d, 16, 1e-3 / s.extended_size)
+    x, it, rr = orc.pcg_learned(d, b, t, maxit=500, rel_tol=1e-10)
+    assert rr <= 1e-10
+    want = np.linalg.solve(C, b)
+    assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-7
+    _, it_plain, _ = orc.pcg(d, b, mode="plain", maxit=500, rel_tol=1e-10)
+    assert it < it_plain
