@@ -75,7 +75,7 @@ def set_profiling(plan, on):
     pfb.check(lib.pf_set_profiling(plan.h, int(on)))
 
 
-def run_gpu(steps, warmup, batch):
+def run_gpu(steps, warmup, batch, prof_steps=2):
     import torch
     torch.cuda.set_device(0)
     d = pfb.make_desc(3, CELLS, [1.0, 1.0, 1.0], precision="fp32", **KW)
@@ -84,22 +84,26 @@ def run_gpu(steps, warmup, batch):
     st = plan.eigs()
     eig_s = time.time() - t0
     out = torch.empty((batch, plan.domain), dtype=torch.float32, device="cuda")
-    set_profiling(plan, True)
     for w in range(warmup):
         plan.generate(batch, seed=SEED, first=w * batch, transform="logNormal", out=out)
-    per_kernel = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
         plan.generate(batch, seed=SEED, first=(warmup + s) * batch, transform="logNormal",
                       out=out)
-        for name, ms in timings(plan):
-            per_kernel.setdefault(name, []).append(ms)
         print(f"bench: step {s + 1}/{steps} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # per-kernel device times: extra steps with HIP events around every launch
+    # (the plan then runs the passes back to back on one stream)
+    per_kernel = {}
+    set_profiling(plan, True)
+    for s in range(prof_steps):
+        plan.generate(batch, seed=SEED, first=(warmup + steps + s) * batch,
+                      transform="logNormal", out=out)
+        for name, ms in timings(plan):
+            per_kernel.setdefault(name, []).append(ms)
     set_profiling(plan, False)
-    # validate the last batch's first field against the plain statistics
     sample = out[0].float().log().cpu().numpy()
     stats = {"mean_log": float(sample.mean()), "var_log": float(sample.var())}
     plan.close()
@@ -153,6 +157,114 @@ def run_cpu_baseline():
                       f"{el:.1f} s per field, eigenvalue setup {eig_s:.1f} s excluded"}
 
 
+def run_uneven(nfields=1000, seed=3, reps=3):
+    """Config 3: nfields 2D fields, sides log-uniform on {64..1024} (2^k and
+    3*2^k), Gaussian covariance with full geometric anisotropy, R2C matrix
+    with per-field FP64 eigenvalues, FP32 realizations, whole batch captured
+    in one hipGraph and replayed."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sides = [64, 96, 128, 192, 256, 384, 512, 768, 1024]
+    descs, alg = [], 0
+    for _ in range(nfields):
+        n0, n1 = (int(rng.choice(sides)) for _ in range(2))
+        th, r, L = rng.uniform(0, np.pi), rng.uniform(1, 8), rng.uniform(0.02, 0.2)
+        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+        A = R @ np.diag([L, L / r])
+        descs.append(pfb.make_desc(2, [n0, n1], [1.0, 1.0], covariance="gaussian",
+                                   anisotropy="geometric", corr_length=list(A.ravel()),
+                                   approximate=True, r2c_check_all=True, precision="fp32"))
+        H, N1 = n0 + 1, 2 * n1
+        alg += 48 * H * N1 + 16 * H * n1 + 4 * n0 * n1
+    t0 = time.perf_counter()
+    b = pfb.Batch(descs)
+    out = torch.empty(int(b.offsets[-1]), dtype=torch.float32, device="cuda")
+    b.generate(seed=11, out=out, use_graph=True)   # eager preparation + capture + first launch
+    setup = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.generate(seed=11, out=out, use_graph=True)
+    el = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    b.generate(seed=11, out=out, use_graph=False)
+    eager = time.perf_counter() - t0
+    b.close()
+    del out
+    torch.cuda.empty_cache()
+    return {"fields": nfields, "fields_per_s": round(nfields / el, 1),
+            "ms_per_batch": round(el * 1e3, 2), "eager_ms_per_batch": round(eager * 1e3, 2),
+            "alg_GBps": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / HBM_PEAK, 4),
+            "setup_s": round(setup, 2),
+            "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch"}
+
+
+def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):
+    """Config 4: 128^3 exponential + nugget 1e-3; learned MLP trained on ntrain
+    method-generated parameter sets; plain CG vs learned PCG to 1e-8 (FP64
+    true residual), RHS N(0,1) seed 11."""
+    sys.path.insert(0, os.path.join(ROOT, "parafields-core_amd"))
+    import learned_precond as lp
+    t0 = time.perf_counter()
+    cases = lp.sample_cases(ntrain, seed=4, cells=cells)
+    x, y = lp.build_training_set(cases)
+    data_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    model, losses = lp.train(x, y, steps=3000, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    b = np.random.default_rng(11).standard_normal(cells ** 3)
+    res = []
+    for l in held:
+        case = lp.CovCase(variance=1.0, corr_length=[l], cells=cells)
+        _, sp = lp.solve(case, b, "plain", max_iterations=max_it)
+        _, sl = lp.solve(case, b, "learned_pcg", model, max_iterations=max_it)
+        res.append({"corr_length": l,
+                    "plain": {k: round(v, 6) if isinstance(v, float) else v for k, v in sp.items()},
+                    "learned_pcg": {k: round(v, 6) if isinstance(v, float) else v
+                                    for k, v in sl.items()},
+                    "iteration_ratio": round(sp["iterations"] / max(1, sl["iterations"]), 2),
+                    "time_ratio": round(sp["seconds"] / max(1e-9, sl["seconds"]), 2)})
+    return {"cells": cells, "train_sets": ntrain, "data_s": round(data_s, 1),
+            "train_s": round(train_s, 1), "final_train_mse": round(float(losses[-1]), 5),
+            "cases": res}
+
+
+def run_dctdst(batch=8, steps=1):
+    """Config 5: 1024^3 cells, DCT/DST field backend (2048^3 even-symmetric
+    embedding), Matern nu=1.5 with a random rotation and lengths in
+    [0.01, 0.05] (geometric anisotropy evaluated by the DCT matrix), FP64
+    eigenvalues in HBM, 8 FP32 realizations per batch, sign transform at 0.5."""
+    import torch
+    rng = np.random.default_rng(5)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    A = q @ np.diag(rng.uniform(0.01, 0.05, 3))
+    d = pfb.make_desc(3, [1024] * 3, [1.0] * 3, covariance="matern32", anisotropy="geometric",
+                      corr_length=list(A.ravel()), matrix="dct", field="dctdst",
+                      approximate=True, precision="fp32")
+    plan = pfb.Plan(d)
+    t0 = time.perf_counter()
+    st = plan.eigs()
+    eig_s = time.perf_counter() - t0
+    out = torch.empty((batch, plan.domain), dtype=torch.float32, device="cuda")
+    plan.generate(1, seed=13, transform="sign", param=0.5, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        plan.generate(batch, seed=13, first=s * batch, transform="sign", param=0.5, out=out)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    P = 1025 ** 3
+    alg = 4 * 12 * P + 4 * 1024 ** 3 + 8 * P / batch
+    plan.close()
+    del out
+    torch.cuda.empty_cache()
+    return {"batch": batch, "fields_per_s": round(batch / el, 3), "s_per_batch": round(el, 3),
+            "alg_GB_per_field": round(alg / 1e9, 2),
+            "alg_GBps": round(alg * batch / el / 1e9, 1),
+            "frac_of_hbm": round(alg * batch / el / HBM_PEAK, 4), "eig_setup_s": round(eig_s, 2),
+            "eig_negative": st["negative"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,11 +273,26 @@ def main():
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a 1 GiB device-to-device copy of known size (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
+    ap.add_argument("--extra", action="store_true",
+                    help="also run the uneven-batch, CG and DCT/DST legs")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine holds the whole extended grid on one "
               "device and has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("bench: copy probe of 1073741824 bytes x3 done", file=sys.stderr, flush=True)
+        del src, dst
+        torch.cuda.empty_cache()
     elapsed, per_kernel, st, eig_s, stats = run_gpu(a.steps, a.warmup, a.batch)
     fields = a.steps * a.batch
     fps = fields / elapsed
@@ -178,6 +305,15 @@ def main():
                    "alg_GBps": round(kb[k] * a.batch / (avg[k] * 1e-3) / 1e9, 1)}
                for k in avg if k in kb}
     fp64_fps = None if a.no_fp64 else run_gpu_fp64_single(3)
+    extra = {}
+    if a.extra:
+        for name, fn in (("uneven_batch_2d", run_uneven), ("cg_vs_learned_pcg", run_cg),
+                         ("dctdst_1024", run_dctdst)):
+            try:
+                print(f"bench: {name} leg", file=sys.stderr, flush=True)
+                extra[name] = fn()
+            except Exception as e:  # a failing leg is reported, not hidden
+                extra[name] = {"error": repr(e)}
     cpu = None if a.no_cpu else run_cpu_baseline()
     line = {
         "metric": METRIC,
@@ -219,6 +355,7 @@ def main():
         "eig_stats": st,
         "field_log_stats": stats,
         "cpu_baseline": cpu,
+        "other_configs": extra,
     }
     print(json.dumps(line), flush=True)
 

@@ -110,6 +110,10 @@ typedef struct pf_desc {
   int32_t precision;           /* enum pf_precision */
   int32_t cg_iterations;       /* randomField.cgIterations (default 100) */
   double nugget;               /* added to C(0) (custom-iso hook of the reference); 0 = none */
+  int32_t r2c_check_all;       /* R2C matrix: 0 = the reference's check/clamp over the interleaved
+                                  buffer (matrix.hh:270-291 reading r2cmatrixbackend.hh:378-400, which
+                                  covers Re and Im of the first half of the entries only); 1 = check
+                                  and clamp the real part of every entry (not reference behaviour) */
 } pf_desc;
 
 /* eigenvalue classification, matrix.hh:266-305 */
@@ -254,6 +258,24 @@ int pf_mlp_gradients(void* mlp, const float* x, const float* y, int32_t batch, f
 /* `steps` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8) at rate lr. */
 int pf_mlp_train(void* mlp, const float* x, const float* y, int32_t batch, int32_t steps, float lr,
                  double* final_loss);
+
+/* ---- uneven batches (no counterpart in the reference) --------------------
+ * n independent fields, each with its own descriptor (own grid and
+ * covariance; all of one precision), one realization each: per-field FP64
+ * eigenvalues followed by the generation passes, enqueued on one stream.
+ * With use_graph the whole sequence is captured once into a hipGraph (keyed
+ * by seed, first_realization, transform, param and the output pointer) and
+ * replayed as one launch. Field i uses Philox realization first_realization
+ * + i and is written at out + offsets[i] (pf_batch_offsets, n+1 entries).
+ * stats (n entries, may be NULL) receives each field's eigenvalue
+ * classification; PF_ENEG is returned if any field has negative eigenvalues
+ * without desc.approximate (its field is still written). */
+int pf_batch_create(const pf_desc* descs, int32_t n, void** batch);
+int pf_batch_destroy(void* batch);
+int pf_batch_offsets(void* batch, int64_t* offsets);
+int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, int32_t transform,
+                      double transform_param, void* out, int32_t out_mem, int32_t use_graph,
+                      pf_eig_stats* stats);
 
 /* Observability (the analogue of the reference's randomField.verbose
  * prints, fieldtraits.hh:180, matrix.hh:301-305): when on, every kernel

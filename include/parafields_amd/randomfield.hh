@@ -130,6 +130,7 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   d.precision = std::is_same<RF, float>::value ? PF_FP32 : PF_FP64;
   d.cg_iterations = c.get<int>("randomField.cgIterations", 100);
   d.nugget = 0.;
+  d.r2c_check_all = 0;
   return d;
 }
 

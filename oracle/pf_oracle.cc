@@ -691,13 +691,18 @@ Eigs eigs_r2c(const pf_desc& d, const Geo& g, const double* custom)
   };
   const Index count = H * rows;
   Eigs e;
-  classify(
-    count, d.threshold,
-    [&](Index i) -> double& {
-      double* p = reinterpret_cast<double*>(&spec[storage_to_logical(i / 2)]);
-      return p[i % 2];
-    },
-    e.st);
+  if (d.r2c_check_all)
+    classify(
+      count, d.threshold,
+      [&](Index i) -> double& { return reinterpret_cast<double*>(&spec[i])[0]; }, e.st);
+  else
+    classify(
+      count, d.threshold,
+      [&](Index i) -> double& {
+        double* p = reinterpret_cast<double*>(&spec[storage_to_logical(i / 2)]);
+        return p[i % 2];
+      },
+      e.st);
   e.v.resize(count);
   for (Index c = 0; c < count; c++)
     e.v[c] = spec[c].real();

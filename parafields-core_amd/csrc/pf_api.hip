@@ -8,6 +8,7 @@
 // its device work has completed.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -198,6 +199,11 @@ struct Plan
   Geo g;
   int device = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
+  bool capturing = false;
+  hipStream_t stream2 = nullptr;     // second stream of the pipelined generation
+  hipEvent_t ev_join = nullptr;
+  std::array<hipEvent_t, 8> ev_chunk{};
   std::mutex mu;
   bool eig_valid = false;
   int eig_status = PF_OK;
@@ -218,7 +224,14 @@ struct Plan
       (void)hipEventDestroy(std::get<1>(p));
       (void)hipEventDestroy(std::get<2>(p));
     }
-    if (stream)
+    if (stream2) {
+      (void)hipStreamSynchronize(stream2);
+      (void)hipStreamDestroy(stream2);
+      (void)hipEventDestroy(ev_join);
+      for (auto e : ev_chunk)
+        (void)hipEventDestroy(e);
+    }
+    if (stream && own_stream)
       (void)hipStreamDestroy(stream);
   }
 
@@ -479,7 +492,9 @@ pf_eig_stats reduce_partials(Plan& P, int nb)
   return s;
 }
 
-void compute_eigs(Plan& P, const double* custom_host)
+// dev_stats != nullptr: capture-safe mode for batches (no host syncs, no
+// frees); the classification lands in *dev_stats and the caller finalises.
+void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats = nullptr)
 {
   const Geo& g = P.g;
   const pf_desc& d = P.d;
@@ -524,7 +539,10 @@ void compute_eigs(Plan& P, const double* custom_host)
                                                     P.partial.as<ClassifyPartial>());
       check_launch();
     }
-    P.st = reduce_partials(P, nb);
+    if (dev_stats)
+      k_reduce_partials<<<1, 64, 0, P.stream>>>(P.partial.as<ClassifyPartial>(), nb, dev_stats);
+    else
+      P.st = reduce_partials(P, nb);
     P.st.entries = total;
   } else {
     const int64_t H = g.H, N0 = g.N[0], rows = (int64_t)g.N[1] * g.N[2];
@@ -573,13 +591,17 @@ void compute_eigs(Plan& P, const double* custom_host)
     const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
     k_classify_r2c<<<nb, 256, 0, P.stream>>>(spec, count, g.H, g.N[1], g.N[2],
                                              (g.dim == 3 && g.transposed) ? 1 : 0, d.threshold,
-                                             P.partial.as<ClassifyPartial>());
+                                             P.partial.as<ClassifyPartial>(), d.r2c_check_all);
     check_launch();
-    P.st = reduce_partials(P, nb);
+    if (dev_stats)
+      k_reduce_partials<<<1, 64, 0, P.stream>>>(P.partial.as<ClassifyPartial>(), nb, dev_stats);
+    else
+      P.st = reduce_partials(P, nb);
     P.st.entries = count;
     k_real_part<<<grid1d(count), 256, 0, P.stream>>>(spec, P.eig.as<double>(), count);
     check_launch();
-    P.scratch.release();
+    if (!dev_stats)
+      P.scratch.release();
   }
   // sqrt tables for generation
   const int64_t ne = g.eig_entries;
@@ -592,6 +614,8 @@ void compute_eigs(Plan& P, const double* custom_host)
       <<<grid1d(ne), 256, 0, P.stream>>>(P.eig.as<double>(), P.sq64.as<double>(), ne);
   }
   check_launch();
+  if (dev_stats)
+    return;
   P.sync();
   P.eig_valid = true;
   P.eig_status = (P.st.negative > 0 && !d.approximate) ? PF_ENEG : PF_OK;
@@ -626,73 +650,123 @@ LastAxis last_axis(const Geo& g)
   return a;
 }
 
+// One chunk of realizations [b0, b0 + nb): pass G on stream sa, then the
+// middle and x passes on stream sb (sb waits on an event after pass G).
 template <typename T>
-void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
-                  int transform, double param, T* out)
+void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
+                        const double* dnoise, uint64_t seed, uint64_t first, int b0, int nb,
+                        int transform, double param, T* out)
 {
   const Geo& g = P.g;
   GenGeom gg = gen_geom(P);
   LastAxis la = last_axis(g);
-  FastCtx fc{ P.stream, fast_budget() };
   const T* sq = sizeof(T) == 4 ? (const T*)P.sq32.p : (const T*)P.sq64.p;
   const int dim = g.dim;
   const int Nlast = dim == 3 ? g.N[2] : g.N[1];
   const int nlast = la.nlast;
   const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
-  const size_t b1 = (size_t)nlinesG * nlast * sizeof(cplx<T>);
-  const size_t b2 = dim == 3 ? (size_t)g.H * g.n[1] * g.n[2] * sizeof(cplx<T>) : 0;
-  P.w1.ensure(b1 * batch);
-  if (dim == 3)
-    P.w2.ensure(b2 * batch);
+  const int64_t s1 = nlinesG * nlast;                                   // B1 per realization
+  const int64_t s2 = dim == 3 ? (int64_t)g.H * g.n[1] * g.n[2] : 0;    // B2 per realization
+  cplx<T>* B1 = P.w1.as<cplx<T>>() + (int64_t)b0 * s1;
+  cplx<T>* B2 = dim == 3 ? P.w2.as<cplx<T>>() + (int64_t)b0 * s2 : nullptr;
+  const double* nz = dnoise ? dnoise + (int64_t)b0 * g.noise_per_field : nullptr;
+  out += (int64_t)b0 * g.domain;
   constexpr int ME = maxe<T>();
   {
+    FastCtx fc{ sa, fast_budget() };
     Scope sc(&P, "gen_last");
     const cplx<T>* tw = P.table<T>(Nlast);
-    if (!fast_gen_last<T>(fc, Nlast, gg, la, sq, dnoise, seed, first, P.w1.as<cplx<T>>(), tw,
-                          batch)) {
+    if (!fast_gen_last<T>(fc, Nlast, gg, la, sq, nz, seed, first + b0, B1, tw, nb)) {
       Sched s = make_sched(Nlast);
       Cfg c = make_cfg<T>(Nlast, nlinesG, Nlast);
-      dim3 grid(nblocks(nlinesG, c.L), batch);
-      if (dnoise)
-        k_gen_last<T, ME, 0><<<grid, c.threads, c.lds, P.stream>>>(
-          gg, sq, dnoise, seed, first, P.w1.as<cplx<T>>(), s, tw, c.L);
+      dim3 grid(nblocks(nlinesG, c.L), nb);
+      if (nz)
+        k_gen_last<T, ME, 0><<<grid, c.threads, c.lds, sa>>>(gg, sq, nz, seed, first + b0, B1, s,
+                                                            tw, c.L);
       else
-        k_gen_last<T, ME, 1><<<grid, c.threads, c.lds, P.stream>>>(
-          gg, sq, nullptr, seed, first, P.w1.as<cplx<T>>(), s, tw, c.L);
+        k_gen_last<T, ME, 1><<<grid, c.threads, c.lds, sa>>>(gg, sq, nullptr, seed, first + b0,
+                                                            B1, s, tw, c.L);
     }
     check_launch();
   }
-  const cplx<T>* xin = P.w1.as<cplx<T>>();
+  if (sb != sa) {
+    HIPCHECK(hipEventRecord(ev, sa));
+    HIPCHECK(hipStreamWaitEvent(sb, ev, 0));
+  }
+  FastCtx fb{ sb, fast_budget() };
+  const cplx<T>* xin = B1;
   if (dim == 3) {
     Scope sc(&P, "gen_mid");
     const cplx<T>* tw = P.table<T>(g.N[1]);
-    if (!fast_mid<T>(fc, g.N[1], true, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.N[0] / 2,
-                     g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], tw, batch)) {
+    if (!fast_mid<T>(fb, g.N[1], true, B1, B2, g.H, g.N[0] / 2, g.n[2], g.N[1], g.N[1], g.n[1],
+                     g.n[1], tw, nb)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
       Sched s = make_sched(g.N[1]);
       Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
-      dim3 grid(nblocks(nl, c.L), batch);
-      k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, P.stream>>>(
-        P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
-        tw, c.L);
+      dim3 grid(nblocks(nl, c.L), nb);
+      k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, sb>>>(
+        B1, B2, g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s, tw, c.L);
     }
     check_launch();
-    xin = P.w2.as<cplx<T>>();
+    xin = B2;
   }
   {
     const int M = g.N[0] / 2;
     const int64_t rows = (int64_t)g.n[1] * g.n[2];
     Scope sc(&P, "gen_c2r_x");
     const cplx<T>* tw = P.table<T>(g.N[0]);
-    if (!fast_c2r<T>(fc, M, xin, out, rows, g.H, g.n[0], tw, transform, (T)param, batch)) {
+    if (!fast_c2r<T>(fb, M, xin, out, rows, g.H, g.n[0], tw, transform, (T)param, nb)) {
       Sched s = make_sched(M);
       Cfg c = make_cfg<T>(M, rows, g.N[0]);
-      dim3 grid(nblocks(rows, c.L), batch);
-      k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, P.stream>>>(xin, out, rows, g.H, g.n[0], s, tw,
-                                                             c.L, transform, (T)param);
+      dim3 grid(nblocks(rows, c.L), nb);
+      k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, sb>>>(xin, out, rows, g.H, g.n[0], s, tw, c.L,
+                                                       transform, (T)param);
     }
     check_launch();
   }
+}
+
+// Generation of `batch` realizations. Large batches are cut into chunks and
+// pipelined over two streams: the VALU-bound pass G of chunk k+1 runs while
+// the HBM-bound middle and x passes of chunk k run (PF_OVERLAP=0 disables;
+// profiling serialises so the per-launch event timings stay exact).
+template <typename T>
+void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
+                  int transform, double param, T* out)
+{
+  const Geo& g = P.g;
+  const int dim = g.dim;
+  const int nlast = dim == 3 ? g.n[2] : g.n[1];
+  const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
+  P.w1.ensure((size_t)nlinesG * nlast * sizeof(cplx<T>) * batch);
+  if (dim == 3)
+    P.w2.ensure((size_t)g.H * g.n[1] * g.n[2] * sizeof(cplx<T>) * batch);
+  const char* ov = getenv("PF_OVERLAP");
+  const bool overlap = !P.profiling && batch >= 4 && !(ov && ov[0] == '0') && !P.capturing;
+  if (!overlap) {
+    generate_r2c_chunk<T>(P, P.stream, P.stream, nullptr, dnoise, seed, first, 0, batch,
+                          transform, param, out);
+    return;
+  }
+  if (!P.stream2) {
+    HIPCHECK(hipStreamCreateWithFlags(&P.stream2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
+    for (auto& e : P.ev_chunk)
+      HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  // stream2 starts after everything already queued on the plan stream
+  HIPCHECK(hipEventRecord(P.ev_join, P.stream));
+  HIPCHECK(hipStreamWaitEvent(P.stream2, P.ev_join, 0));
+  const int nchunks = std::min(batch, 8);
+  const int per = (batch + nchunks - 1) / nchunks;
+  int k = 0;
+  for (int b0 = 0; b0 < batch; b0 += per, k++) {
+    const int nb = std::min(per, batch - b0);
+    generate_r2c_chunk<T>(P, P.stream, P.stream2, P.ev_chunk[k % P.ev_chunk.size()], dnoise,
+                          seed, first, b0, nb, transform, param, out);
+  }
+  HIPCHECK(hipEventRecord(P.ev_join, P.stream2));
+  HIPCHECK(hipStreamWaitEvent(P.stream, P.ev_join, 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -1163,6 +1237,58 @@ struct Mlp
     for (double q : h)
       loss += q;
     return loss;
+  }
+};
+
+
+// ---------------------------------------------------------------------------
+// Uneven batches: one stream, optional hipGraph capture of the whole batch
+// ---------------------------------------------------------------------------
+struct Batch
+{
+  std::vector<std::unique_ptr<Plan>> plans;
+  std::vector<int64_t> offsets;
+  hipStream_t stream = nullptr;
+  DevBuf stats, stage;
+  bool prepared = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t key_seed = 0, key_first = 0;
+  int key_transform = -1;
+  double key_param = 0.;
+  void* key_out = nullptr;
+  ~Batch()
+  {
+    if (exec)
+      (void)hipGraphExecDestroy(exec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    plans.clear();
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+  // Every field: FP64 eigenvalues (classification reduced on the device),
+  // then one realization. No host synchronisation, so it can be captured.
+  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out)
+  {
+    ClassifyPartial* st = stats.as<ClassifyPartial>();
+    for (size_t i = 0; i < plans.size(); i++) {
+      Plan& P = *plans[i];
+      compute_eigs(P, nullptr, st + i);
+      if (P.d.precision == PF_FP32) {
+        float* o = (float*)out + offsets[i];
+        if (P.d.field_backend == PF_FIELD_DCTDST)
+          generate_dctdst<float>(P, nullptr, seed, first + i, 1, transform, param, o);
+        else
+          generate_r2c<float>(P, nullptr, seed, first + i, 1, transform, param, o);
+      } else {
+        double* o = (double*)out + offsets[i];
+        if (P.d.field_backend == PF_FIELD_DCTDST)
+          generate_dctdst<double>(P, nullptr, seed, first + i, 1, transform, param, o);
+        else
+          generate_r2c<double>(P, nullptr, seed, first + i, 1, transform, param, o);
+      }
+    }
   }
 };
 
@@ -1721,6 +1847,132 @@ int pf_mlp_train(void* mlp, const float* x, const float* y, int32_t batch, int32
     if (final_loss)
       *final_loss = l;
     return PF_OK;
+  })
+}
+
+
+int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
+{
+  PF_GUARD({
+    if (!descs || n < 1 || !batch)
+      throw PfError(PF_EINVAL, "a batch needs at least one descriptor");
+    auto B = std::make_unique<Batch>();
+    HIPCHECK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    int64_t off = 0;
+    for (int i = 0; i < n; i++) {
+      if (descs[i].precision != descs[0].precision)
+        throw PfError(PF_EINVAL, "all fields of a batch must use one precision");
+      auto P = std::make_unique<Plan>();
+      P->d = descs[i];
+      P->g = make_geo(descs[i]);
+      HIPCHECK(hipGetDevice(&P->device));
+      P->stream = B->stream;
+      P->own_stream = false;
+      P->capturing = true;
+      B->offsets.push_back(off);
+      off += P->g.domain;
+      B->plans.push_back(std::move(P));
+    }
+    B->offsets.push_back(off);
+    B->stats.ensure((size_t)n * sizeof(ClassifyPartial));
+    *batch = B.release();
+    return PF_OK;
+  })
+}
+
+int pf_batch_destroy(void* batch)
+{
+  PF_GUARD({
+    Batch* B = static_cast<Batch*>(batch);
+    if (B) {
+      (void)hipStreamSynchronize(B->stream);
+      delete B;
+    }
+    return PF_OK;
+  })
+}
+
+int pf_batch_offsets(void* batch, int64_t* offsets)
+{
+  PF_GUARD({
+    Batch* B = static_cast<Batch*>(batch);
+    for (size_t i = 0; i < B->offsets.size(); i++)
+      offsets[i] = B->offsets[i];
+    return PF_OK;
+  })
+}
+
+int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, int32_t transform,
+                      double transform_param, void* out, int32_t out_mem, int32_t use_graph,
+                      pf_eig_stats* stats)
+{
+  PF_GUARD({
+    Batch* B = static_cast<Batch*>(batch);
+    if (transform < 0 || transform > PF_TRANSFORM_BOXCOX)
+      throw PfError(PF_EINVAL, "transform type not known");
+    const size_t es = B->plans[0]->d.precision == PF_FP32 ? 4 : 8;
+    const size_t outb = (size_t)B->offsets.back() * es;
+    void* dout = out;
+    if (out_mem == PF_MEM_HOST) {
+      B->stage.ensure(outb);
+      dout = B->stage.p;
+    }
+    if (!B->prepared) {
+      // eager pass: allocates every buffer and twiddle table the capture needs
+      B->enqueue(seed, first_realization, transform, transform_param, dout);
+      HIPCHECK(hipStreamSynchronize(B->stream));
+      B->prepared = true;
+    } else if (!use_graph) {
+      B->enqueue(seed, first_realization, transform, transform_param, dout);
+    }
+    if (use_graph) {
+      const bool same = B->exec && B->key_seed == seed && B->key_first == first_realization &&
+                        B->key_transform == transform && B->key_param == transform_param &&
+                        B->key_out == dout;
+      if (!same) {
+        if (B->exec)
+          (void)hipGraphExecDestroy(B->exec);
+        if (B->graph)
+          (void)hipGraphDestroy(B->graph);
+        B->exec = nullptr;
+        B->graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));
+        B->enqueue(seed, first_realization, transform, transform_param, dout);
+        HIPCHECK(hipStreamEndCapture(B->stream, &B->graph));
+        HIPCHECK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        B->key_seed = seed;
+        B->key_first = first_realization;
+        B->key_transform = transform;
+        B->key_param = transform_param;
+        B->key_out = dout;
+      }
+      HIPCHECK(hipGraphLaunch(B->exec, B->stream));
+    }
+    if (out_mem == PF_MEM_HOST)
+      HIPCHECK(hipMemcpyAsync(out, dout, outb, hipMemcpyDeviceToHost, B->stream));
+    HIPCHECK(hipStreamSynchronize(B->stream));
+    const size_t n = B->plans.size();
+    std::vector<ClassifyPartial> h(n);
+    HIPCHECK(hipMemcpy(h.data(), B->stats.p, n * sizeof(ClassifyPartial), hipMemcpyDeviceToHost));
+    int status = PF_OK;
+    for (size_t i = 0; i < n; i++) {
+      Plan& P = *B->plans[i];
+      pf_eig_stats st{};
+      st.small = h[i].small;
+      st.zero = h[i].zero;
+      st.negative = h[i].negative;
+      st.smallest = h[i].smallest;
+      st.entries = P.d.matrix_backend == PF_MATRIX_DCT
+                     ? (int64_t)P.g.P[0] * P.g.P[1] * P.g.P[2]
+                     : P.g.half;
+      if (stats)
+        stats[i] = st;
+      if (st.negative > 0 && !P.d.approximate)
+        status = PF_ENEG;
+    }
+    if (status != PF_OK)
+      g_err = "negative eigenvalues in covariance matrix";
+    return status;
   })
 }
 

@@ -135,6 +135,44 @@ __global__ void kd_gen_last(DctGeom g, const T* sq, const double* noise, uint64_
   T* out = G + (int64_t)b * ncomb * plane * nlast;
   for (int c = 0; c < ncomb; c++) {
     const int t0 = c, t1 = c + oddlast;
+    if (NOISE == 1 && !edge && g.sx == 1) {
+      // consecutive kx are consecutive samples: one Philox block (4 normals)
+      // feeds up to 4 entries of the tile; NB blocks cover the L samples of a
+      // (type, m) row whatever its alignment
+      const int NB = L / 4 + 1;
+      for (int idx = threadIdx.x; idx < 2 * NB * (M + 1); idx += NT) {
+        const int j = idx % NB, rest = idx / NB;
+        const int half = rest & 1, m = rest >> 1;
+        const int t = half ? t1 : t0;
+        const int k1 = g.dim == 3 ? km0 : m, k2 = g.dim == 3 ? m : 0;
+        const int64_t q0 = t * g.Ptot + kx0 + k1 * g.s1 + k2 * g.s2;
+        const uint64_t blk = (uint64_t)(q0 >> 2) + j;
+        const int64_t qb = (int64_t)(blk << 2);
+        if (qb >= q0 + L)
+          continue;
+        uint32_t cc[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
+                           (uint32_t)(real >> 32) };
+        philox4x32_10(cc, (uint32_t)seed, (uint32_t)(seed >> 32));
+        T nrm[4];
+        BoxMuller<T>::pair(cc[0], cc[1], nrm[0], nrm[1]);
+        BoxMuller<T>::pair(cc[2], cc[3], nrm[2], nrm[3]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int l = (int)(qb + i - q0);
+          if (l < 0 || l >= L)
+            continue;
+          const int kx = kx0 + l;
+          const T sl = sq[kx + (int64_t)g.P0 * (k1 + (int64_t)g.P1 * k2)];
+          const T v = dct_factor<T>(g, t, kx, k1, k2, sl) * nrm[i];
+          cplx<T>* dst = (m < M) ? &buf[m * SE + l] : &xM[l];
+          if (half)
+            dst->y = -v;
+          else
+            dst->x = v;
+        }
+      }
+      __syncthreads();
+    } else
     for (int idx = threadIdx.x; idx < L * (M + 1); idx += NT) {
       const int l = idx % L, m = idx / L;
       const int kx = edge ? kx0 : kx0 + l;

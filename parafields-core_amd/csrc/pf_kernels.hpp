@@ -244,9 +244,18 @@ __global__ void k_scale_complex(cplx<double>* a, int64_t n, double inv_next)
 // read through matrix.hh:270-291): real i is component i%2 of storage entry
 // i/2; 3D transposed storage is (kx, kz, ky).
 __global__ void k_classify_r2c(cplx<double>* spec, int64_t count, int H, int N1, int N2,
-                               int transposed3d, double thr, ClassifyPartial* partial)
+                               int transposed3d, double thr, ClassifyPartial* partial,
+                               int check_all)
 {
   ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  if (check_all) {
+    // opt-in: real part of every entry
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+      classify_one(spec[i].x, thr, c);
+    block_reduce_write(c, partial);
+    return;
+  }
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t s = i >> 1;
@@ -261,6 +270,21 @@ __global__ void k_classify_r2c(cplx<double>* spec, int64_t count, int H, int N1,
     *p = v;
   }
   block_reduce_write(c, partial);
+}
+
+// Sum `n` block partials into one record on the device (capture-safe).
+__global__ void k_reduce_partials(const ClassifyPartial* partial, int n, ClassifyPartial* out)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  ClassifyPartial r = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int i = 0; i < n; i++) {
+    r.small += partial[i].small;
+    r.zero += partial[i].zero;
+    r.negative += partial[i].negative;
+    r.smallest = fmin(r.smallest, partial[i].smallest);
+  }
+  *out = r;
 }
 
 __global__ void k_real_part(const cplx<double>* spec, double* out, int64_t n)

@@ -66,6 +66,7 @@ class PfDesc(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("cg_iterations", ctypes.c_int32),
         ("nugget", ctypes.c_double),
+        ("r2c_check_all", ctypes.c_int32),
     ]
 
 
@@ -120,7 +121,8 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
               field: str = "r2c", transposed: int = -1, approximate: bool = False,
               threshold: float = 1e-14, precision: str = "fp64", cg_iterations: int = 100,
               nugget: float = 0.0, matern_nu: float = 1.0, exp_gamma: float = 1.0,
-              cauchy_alpha: float = 1.0, cauchy_beta: float = 1.0) -> PfDesc:
+              cauchy_alpha: float = 1.0, cauchy_beta: float = 1.0,
+              r2c_check_all: bool = False) -> PfDesc:
     """Descriptor with the reference's defaults. `matrix=None` picks what
     RandomField would (randomfield.hh:153-164): DCT (DefaultIsoMatrix) for
     none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy."""
@@ -149,6 +151,7 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
     d.precision = PRECISION[precision]
     d.cg_iterations = cg_iterations
     d.nugget = nugget
+    d.r2c_check_all = int(r2c_check_all)
     return d
 
 
@@ -191,6 +194,10 @@ def load(path: str = LIB_PATH):
         "pf_mlp_gradients": (i32, [vp, vp, vp, i32, vp, ctypes.POINTER(dp)]),
         "pf_mlp_train": (i32, [vp, vp, vp, i32, i32, ctypes.c_float, ctypes.POINTER(dp)]),
         "pf_set_profiling": (i32, [vp, i32]),
+        "pf_batch_create": (i32, [ctypes.POINTER(PfDesc), i32, ctypes.POINTER(vp)]),
+        "pf_batch_destroy": (i32, [vp]),
+        "pf_batch_offsets": (i32, [vp, vp]),
+        "pf_batch_generate": (i32, [vp, u64, u64, i32, dp, vp, i32, i32, vp]),
         "pf_last_timings": (i32, [vp, ctypes.c_char_p, i32, ctypes.POINTER(dp), i32,
                                   ctypes.POINTER(i32)]),
     })
@@ -209,7 +216,8 @@ def exported_symbols():
             "pf_matvec", "pf_cg", "pf_value_transform", "pf_philox_normals", "pf_philox_raw",
             "pf_spectral_targets", "pf_mlp_create", "pf_mlp_destroy", "pf_mlp_nparams",
             "pf_mlp_params", "pf_mlp_forward", "pf_mlp_gradients", "pf_mlp_train",
-            "pf_set_profiling", "pf_last_timings"]
+            "pf_set_profiling", "pf_last_timings", "pf_batch_create", "pf_batch_destroy",
+            "pf_batch_offsets", "pf_batch_generate"]
 
 
 def check(status: int):
@@ -416,3 +424,45 @@ class Mlp:
         check(self.lib.pf_mlp_train(self.h, x.ctypes.data, y.ctypes.data, x.shape[0], steps, lr,
                                     ctypes.byref(loss)))
         return loss.value
+
+
+class Batch:
+    """Uneven batch: many fields with their own descriptors, one launch."""
+
+    def __init__(self, descs):
+        self.lib = load()
+        n = len(descs)
+        arr = (PfDesc * n)(*descs)
+        h = ctypes.c_void_p()
+        check(self.lib.pf_batch_create(arr, n, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+        off = np.empty(n + 1, dtype=np.int64)
+        check(self.lib.pf_batch_offsets(self.h, off.ctypes.data))
+        self.offsets = off
+        self.dtype = np.float32 if descs[0].precision == 1 else np.float64
+
+    def close(self):
+        if self.h:
+            self.lib.pf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def generate(self, seed=0, first=0, transform="none", param=0.0, out=None, use_graph=True,
+                 allow_negative=False):
+        if out is None:
+            out = np.empty(int(self.offsets[-1]), dtype=self.dtype)
+        st = (PfEigStats * self.n)()
+        status = self.lib.pf_batch_generate(self.h, seed, first, TRANSFORMS[transform], param,
+                                            _ptr(out), _mem(out), int(use_graph), st)
+        if not (allow_negative and status == PF_ENEG):
+            check(status)
+        return out, [s.as_dict() for s in st]
+
+    def field(self, out, i):
+        return out[int(self.offsets[i]):int(self.offsets[i + 1])]

@@ -39,6 +39,6 @@ def test_binding_names_are_exported():
 
 def test_desc_struct_layout_matches_header():
     # offsets of the C struct as compiled by the C compiler
-    assert ctypes.sizeof(pfb.PfDesc) == 216
+    assert ctypes.sizeof(pfb.PfDesc) == 224
     assert ctypes.sizeof(pfb.PfEigStats) == 40
     assert ctypes.sizeof(pfb.PfSizes) == 64

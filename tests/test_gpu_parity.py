@@ -459,3 +459,46 @@ def test_trained_network_gains_over_plain_cg():
         gains.append(sp["iterations"] / sl["iterations"])
     print("learned-PCG iteration gains over plain CG:", gains, flush=True)
     assert min(gains) >= 3.0
+
+
+def _uneven_descs(n, seed, precision="fp32"):
+    """Config-3 style fields: 2D, sides from {64..1024 as 2^k and 3*2^k} (small
+    here), Gaussian covariance with full geometric anisotropy, R2C matrix with
+    the full real-part check (the reference's interleaved check turns
+    Gaussian R2C fields into NaN)."""
+    rng = np.random.default_rng(seed)
+    sides = [16, 24, 32, 48, 64, 96, 128]
+    out = []
+    for _ in range(n):
+        n0, n1 = (int(rng.choice(sides)) for _ in range(2))
+        th, r, L = rng.uniform(0, np.pi), rng.uniform(1, 8), rng.uniform(0.02, 0.2)
+        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+        A = R @ np.diag([L, L / r])
+        out.append(desc(2, [n0, n1], covariance="gaussian", anisotropy="geometric",
+                        corr_length=list(A.ravel()), approximate=True, r2c_check_all=True,
+                        precision=precision))
+    return out
+
+
+def test_uneven_batch_graph_matches_single_plans():
+    descs = _uneven_descs(14, seed=3)
+    descs.append(desc(3, [16, 16, 24], covariance="matern32", anisotropy="axiparallel",
+                      corr_length=[0.1, 0.1, 0.05], approximate=True, precision="fp32"))
+    descs.append(desc(3, [8, 8, 8], covariance="exponential", corr_length=0.2,
+                      field="dctdst", matrix="dct", precision="fp32"))
+    b = pfb.Batch(descs)
+    out_g, st = b.generate(seed=21, first=100, use_graph=True)
+    out_g2, _ = b.generate(seed=21, first=100, use_graph=True)
+    out_e, _ = b.generate(seed=21, first=100, use_graph=False)
+    assert np.array_equal(out_g, out_g2) and np.array_equal(out_g, out_e)
+    for i, d in enumerate(descs):
+        p = pfb.Plan(d)
+        single = p.generate(1, seed=21, first=100 + i)[0]
+        assert np.array_equal(b.field(out_g, i), single), i
+        assert np.all(np.isfinite(single))
+    for i in (0, 5, 14, 15):
+        d64 = descs[i]
+        d64.precision = 0
+        want = orc.generate(d64, orc.noise_philox(21, 100 + i, 0,
+                                                  int(orc.sizes(d64).noise_per_field)))
+        assert np.abs(b.field(out_g, i) - want).max() / want.std() < FP32_TOL
