@@ -7,7 +7,8 @@ grid, DCT matrix (FP64 eigenvalues) + R2C field, FP32 realizations from the
 device Philox4x32-10 stream (seed 7), log-normal value transform, output
 resident in HBM. value = fields per second over the timed steps.
 
-Also reported: per-kernel device times (HIP events on the plan's stream), the
+Also reported: per-kernel device times (HIP events around every launch of the
+timed steps, on the stream the launch runs on), the
 roofline of the dominant kernel against 8 TB/s, the FP64 per-realization GPU
 path on the same grid, and the FP64 CPU oracle on a bounded sample (one
 realization) as the CPU baseline.
@@ -69,13 +70,31 @@ def timings(plan):
     return list(zip(labels, [ms[i] for i in range(min(n.value, 64))]))
 
 
+def measured_traffic(kernel, fields_per_launch):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes of this same workload, stored per field (each counter
+    scaled by what it reads for a 1 GiB device copy) in
+    profiles/pmc_traffic_config2.json; None when that record is absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_config2.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get("per_field_bytes", {}).get(kernel)
+    if rec is None:
+        return None
+    return int((rec["fetch"] + rec["write"]) * fields_per_launch)
+
+
 def set_profiling(plan, on):
     lib = pfb.load()
     lib.pf_set_profiling.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     pfb.check(lib.pf_set_profiling(plan.h, int(on)))
 
 
-def run_gpu(steps, warmup, batch, prof_steps=2):
+def run_gpu(steps, warmup, batch):
+    """The timed region: `steps` calls of 64 realizations each. Every launch
+    is bracketed by HIP events on the stream it runs on (the generation pass
+    and the y/x passes run on two streams, chunk by chunk), so the per-kernel
+    durations come from the timed steps themselves."""
     import torch
     torch.cuda.set_device(0)
     d = pfb.make_desc(3, CELLS, [1.0, 1.0, 1.0], precision="fp32", **KW)
@@ -84,25 +103,20 @@ def run_gpu(steps, warmup, batch, prof_steps=2):
     st = plan.eigs()
     eig_s = time.time() - t0
     out = torch.empty((batch, plan.domain), dtype=torch.float32, device="cuda")
+    set_profiling(plan, True)
     for w in range(warmup):
         plan.generate(batch, seed=SEED, first=w * batch, transform="logNormal", out=out)
     torch.cuda.synchronize()
+    per_kernel = {}
     t0 = time.perf_counter()
     for s in range(steps):
         plan.generate(batch, seed=SEED, first=(warmup + s) * batch, transform="logNormal",
                       out=out)
+        for name, ms in timings(plan):
+            per_kernel.setdefault(name, []).append(ms)
         print(f"bench: step {s + 1}/{steps} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    # per-kernel device times: extra steps with HIP events around every launch
-    # (the plan then runs the passes back to back on one stream)
-    per_kernel = {}
-    set_profiling(plan, True)
-    for s in range(prof_steps):
-        plan.generate(batch, seed=SEED, first=(warmup + steps + s) * batch,
-                      transform="logNormal", out=out)
-        for name, ms in timings(plan):
-            per_kernel.setdefault(name, []).append(ms)
     set_profiling(plan, False)
     sample = out[0].float().log().cpu().numpy()
     stats = {"mean_log": float(sample.mean()), "var_log": float(sample.var())}
@@ -149,7 +163,7 @@ def run_cpu_baseline():
     t0 = time.perf_counter()
     nz = orc.noise_philox(SEED, 0, 0, n)
     f = orc.generate(d, nz)
-    orc.value_transform(f[:16], "logNormal")
+    orc.value_transform(f, "logNormal")
     el = time.perf_counter() - t0
     return {"value": 1.0 / el, "unit": "fields/s", "cores": cores, "kind": "port",
             "sample": f"1 realization of the same 256^3/512^3 workload on the FP64 CPU oracle "
@@ -185,6 +199,13 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     t0 = time.perf_counter()
     for _ in range(reps):
         b.generate(seed=11, out=out, use_graph=True)
+    serial = (time.perf_counter() - t0) / reps
+    best, tuned_ms = b.autotune(seed=11, out=out)      # one replay per branch count
+    b.generate(seed=11, out=out, use_graph=True)       # capture for the chosen count
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.generate(seed=11, out=out, use_graph=True)
     el = (time.perf_counter() - t0) / reps
     t0 = time.perf_counter()
     b.generate(seed=11, out=out, use_graph=False)
@@ -194,6 +215,7 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     torch.cuda.empty_cache()
     return {"fields": nfields, "fields_per_s": round(nfields / el, 1),
             "ms_per_batch": round(el * 1e3, 2), "eager_ms_per_batch": round(eager * 1e3, 2),
+            "serial_graph_ms_per_batch": round(serial * 1e3, 2), "branches": best,
             "alg_GBps": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / HBM_PEAK, 4),
             "setup_s": round(setup, 2),
             "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch"}
@@ -297,13 +319,18 @@ def main():
     fields = a.steps * a.batch
     fps = fields / elapsed
     total_b, kb = alg_bytes(CELLS, a.batch)
+    # per launch: one launch of each kernel processes batch / launches-per-step fields
     avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
-    dom = max(avg, key=avg.get)
-    dom_bytes = kb[dom] * a.batch
-    achieved = dom_bytes / (avg[dom] * 1e-3)
-    kernels = {k: {"avg_ms": round(avg[k], 4),
-                   "alg_GBps": round(kb[k] * a.batch / (avg[k] * 1e-3) / 1e9, 1)}
+    per_step = {k: len(v) / a.steps for k, v in per_kernel.items()}
+    fields_per_launch = {k: a.batch / per_step[k] for k in per_kernel}
+    total_ms = {k: avg[k] * per_step[k] for k in avg}
+    dom = max(total_ms, key=total_ms.get)
+    achieved = kb[dom] * fields_per_launch[dom] / (avg[dom] * 1e-3)
+    kernels = {k: {"avg_ms_per_launch": round(avg[k], 4), "launches_per_step": per_step[k],
+                   "fields_per_launch": fields_per_launch[k],
+                   "alg_GBps": round(kb[k] * fields_per_launch[k] / (avg[k] * 1e-3) / 1e9, 1)}
                for k in avg if k in kb}
+    traffic = measured_traffic(dom, fields_per_launch[dom])
     fp64_fps = None if a.no_fp64 else run_gpu_fp64_single(3)
     extra = {}
     if a.extra:
@@ -343,7 +370,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
         "pipeline_alg_GBps": round(total_b * fps / 1e9, 1),
         "pipeline_frac_of_hbm": round(total_b * fps / HBM_PEAK, 4),

@@ -277,6 +277,15 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
                       double transform_param, void* out, int32_t out_mem, int32_t use_graph,
                       pf_eig_stats* stats);
 
+/* Batch packing: the fields are bin-packed by algorithmic bytes onto
+ * `branches` concurrent graph branches (streams forked from and joined to the
+ * batch stream). pf_batch_autotune times one replay for 1, 2, 4, 8 and 16
+ * branches on the real batch and keeps the fastest (best, best_ms out). */
+int pf_batch_set_branches(void* batch, int32_t branches);
+int pf_batch_autotune(void* batch, uint64_t seed, uint64_t first_realization, int32_t transform,
+                      double transform_param, void* out, int32_t out_mem, int32_t* best,
+                      double* best_ms);
+
 /* Observability (the analogue of the reference's randomField.verbose
  * prints, fieldtraits.hh:180, matrix.hh:301-305): when on, every kernel
  * launch of a call is bracketed by HIP events on the plan's stream and

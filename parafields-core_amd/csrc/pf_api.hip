@@ -8,6 +8,7 @@
 // its device work has completed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -184,6 +185,7 @@ struct Geo
 {
   int dim, factor, transposed;
   int n[3], N[3], H, P[3];
+  int Hp;  // row pitch of the complex work arrays: H rounded up to 16 entries (128-byte rows)
   double h[3], ext[3];
   int64_t Next, domain, eig_entries, noise_per_field, half;
 };
@@ -256,21 +258,21 @@ struct Plan
     return ptr;
   }
 
-  // optional per-launch timing with events on the plan's stream
-  void mark_begin(const char* name, hipEvent_t& a, hipEvent_t& b)
+  // optional per-launch timing: events on the stream the launch goes to
+  void mark_begin(const char* name, hipEvent_t& a, hipEvent_t& b, hipStream_t s)
   {
     if (!profiling)
       return;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
-    HIPCHECK(hipEventRecord(a, stream));
+    HIPCHECK(hipEventRecord(a, s));
     pending.emplace_back(name, a, b);
   }
-  void mark_end(hipEvent_t b)
+  void mark_end(hipEvent_t b, hipStream_t s)
   {
     if (!profiling)
       return;
-    HIPCHECK(hipEventRecord(b, stream));
+    HIPCHECK(hipEventRecord(b, s));
   }
   void collect_timings()
   {
@@ -294,13 +296,15 @@ struct Plan
 struct Scope
 {
   Plan* p;
+  hipStream_t s;
   hipEvent_t a{}, b{};
-  Scope(Plan* p_, const char* name)
+  Scope(Plan* p_, const char* name, hipStream_t s_ = nullptr)
     : p(p_)
+    , s(s_ ? s_ : p_->stream)
   {
-    p->mark_begin(name, a, b);
+    p->mark_begin(name, a, b, s);
   }
-  ~Scope() noexcept(false) { p->mark_end(b); }
+  ~Scope() noexcept(false) { p->mark_end(b, s); }
 };
 
 void check_launch()
@@ -340,6 +344,7 @@ Geo make_geo(const pf_desc& d)
   if (g.N[0] % 2 != 0)
     throw PfError(PF_EINVAL, "the extended x size must be even");
   g.H = g.N[0] / 2 + 1;
+  g.Hp = (g.H + 15) / 16 * 16;
   if (d.transposed >= 0)
     g.transposed = d.transposed != 0;
   else if (d.dim == 2)
@@ -459,6 +464,7 @@ GenGeom gen_geom(const Plan& P)
   gg.P2 = g.P[2];
   gg.dct = P.d.matrix_backend == PF_MATRIX_DCT;
   gg.noise_stride = g.noise_per_field;
+  gg.Hp = g.Hp;
   return gg;
 }
 
@@ -636,6 +642,7 @@ LastAxis last_axis(const Geo& g)
 {
   LastAxis a;
   a.H = g.H;
+  a.Hp = g.Hp;
   a.N0h = g.N[0] / 2;
   a.Nmid = g.dim == 3 ? g.N[1] : 1;
   a.nlast = g.dim == 3 ? g.n[2] : g.n[1];
@@ -665,8 +672,8 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
   const int Nlast = dim == 3 ? g.N[2] : g.N[1];
   const int nlast = la.nlast;
   const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
-  const int64_t s1 = nlinesG * nlast;                                   // B1 per realization
-  const int64_t s2 = dim == 3 ? (int64_t)g.H * g.n[1] * g.n[2] : 0;    // B2 per realization
+  const int64_t s1 = (int64_t)g.Hp * (dim == 3 ? g.N[1] : 1) * nlast;  // B1 per realization
+  const int64_t s2 = dim == 3 ? (int64_t)g.Hp * g.n[1] * g.n[2] : 0;   // B2 per realization
   cplx<T>* B1 = P.w1.as<cplx<T>>() + (int64_t)b0 * s1;
   cplx<T>* B2 = dim == 3 ? P.w2.as<cplx<T>>() + (int64_t)b0 * s2 : nullptr;
   const double* nz = dnoise ? dnoise + (int64_t)b0 * g.noise_per_field : nullptr;
@@ -674,7 +681,7 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
   constexpr int ME = maxe<T>();
   {
     FastCtx fc{ sa, fast_budget() };
-    Scope sc(&P, "gen_last");
+    Scope sc(&P, "gen_last", sa);
     const cplx<T>* tw = P.table<T>(Nlast);
     if (!fast_gen_last<T>(fc, Nlast, gg, la, sq, nz, seed, first + b0, B1, tw, nb)) {
       Sched s = make_sched(Nlast);
@@ -696,16 +703,16 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
   FastCtx fb{ sb, fast_budget() };
   const cplx<T>* xin = B1;
   if (dim == 3) {
-    Scope sc(&P, "gen_mid");
+    Scope sc(&P, "gen_mid", sb);
     const cplx<T>* tw = P.table<T>(g.N[1]);
-    if (!fast_mid<T>(fb, g.N[1], true, B1, B2, g.H, g.N[0] / 2, g.n[2], g.N[1], g.N[1], g.n[1],
+    if (!fast_mid<T>(fb, g.N[1], true, B1, B2, g.Hp, g.N[0] / 2, g.n[2], g.N[1], g.N[1], g.n[1],
                      g.n[1], tw, nb)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
       Sched s = make_sched(g.N[1]);
       Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
       dim3 grid(nblocks(nl, c.L), nb);
       k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, sb>>>(
-        B1, B2, g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s, tw, c.L);
+        B1, B2, g.H, g.Hp, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s, tw, c.L);
     }
     check_launch();
     xin = B2;
@@ -713,13 +720,13 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
   {
     const int M = g.N[0] / 2;
     const int64_t rows = (int64_t)g.n[1] * g.n[2];
-    Scope sc(&P, "gen_c2r_x");
+    Scope sc(&P, "gen_c2r_x", sb);
     const cplx<T>* tw = P.table<T>(g.N[0]);
-    if (!fast_c2r<T>(fb, M, xin, out, rows, g.H, g.n[0], tw, transform, (T)param, nb)) {
+    if (!fast_c2r<T>(fb, M, xin, out, rows, g.Hp, g.n[0], tw, transform, (T)param, nb)) {
       Sched s = make_sched(M);
       Cfg c = make_cfg<T>(M, rows, g.N[0]);
       dim3 grid(nblocks(rows, c.L), nb);
-      k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, sb>>>(xin, out, rows, g.H, g.n[0], s, tw, c.L,
+      k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, sb>>>(xin, out, rows, g.Hp, g.n[0], s, tw, c.L,
                                                        transform, (T)param);
     }
     check_launch();
@@ -728,8 +735,9 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
 
 // Generation of `batch` realizations. Large batches are cut into chunks and
 // pipelined over two streams: the VALU-bound pass G of chunk k+1 runs while
-// the HBM-bound middle and x passes of chunk k run (PF_OVERLAP=0 disables;
-// profiling serialises so the per-launch event timings stay exact).
+// the HBM-bound middle and x passes of chunk k run (PF_OVERLAP=0 disables).
+// With profiling on, each launch is bracketed by events on its own stream,
+// so its time includes any sharing of the device with the other stream.
 template <typename T>
 void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
                   int transform, double param, T* out)
@@ -738,11 +746,11 @@ void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, 
   const int dim = g.dim;
   const int nlast = dim == 3 ? g.n[2] : g.n[1];
   const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
-  P.w1.ensure((size_t)nlinesG * nlast * sizeof(cplx<T>) * batch);
+  P.w1.ensure((size_t)g.Hp * (dim == 3 ? g.N[1] : 1) * nlast * sizeof(cplx<T>) * batch);
   if (dim == 3)
-    P.w2.ensure((size_t)g.H * g.n[1] * g.n[2] * sizeof(cplx<T>) * batch);
+    P.w2.ensure((size_t)g.Hp * g.n[1] * g.n[2] * sizeof(cplx<T>) * batch);
   const char* ov = getenv("PF_OVERLAP");
-  const bool overlap = !P.profiling && batch >= 4 && !(ov && ov[0] == '0') && !P.capturing;
+  const bool overlap = batch >= 4 && !(ov && ov[0] == '0') && !P.capturing;
   if (!overlap) {
     generate_r2c_chunk<T>(P, P.stream, P.stream, nullptr, dnoise, seed, first, 0, batch,
                           transform, param, out);
@@ -799,6 +807,7 @@ DctGeom dct_geom(const Plan& P)
     d.s2 = 0;
   }
   d.noise_stride = g.noise_per_field;
+  d.P0p = (g.P[0] + 31) / 32 * 32;
   return d;
 }
 
@@ -830,8 +839,8 @@ void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t firs
   const int nlast = dim == 3 ? g.n[2] : g.n[1];
   const int Nlast = dim == 3 ? g.N[2] : g.N[1];
   const int ncomb = 1 << (dim - 1);
-  const size_t gsz = (size_t)ncomb * Pm * g.P[0] * nlast * sizeof(T);
-  const size_t hsz = dim == 3 ? (size_t)2 * g.n[1] * g.n[2] * g.P[0] * sizeof(T) : 0;
+  const size_t gsz = (size_t)ncomb * Pm * dg.P0p * nlast * sizeof(T);
+  const size_t hsz = dim == 3 ? (size_t)2 * g.n[1] * g.n[2] * dg.P0p * sizeof(T) : 0;
   int sub = (int)std::max<size_t>(1, work_cap() / (gsz + hsz));
   sub = std::min(sub, batch);
   P.w1.ensure(gsz * sub);
@@ -908,19 +917,19 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
   constexpr int ME = maxe<T>();
   const int M = g.N[0] / 2;
   const int64_t rows = (int64_t)g.n[1] * g.n[2];
-  const size_t c1 = (size_t)rows * g.H * sizeof(cplx<T>);
-  const size_t c2 = dim == 3 ? (size_t)g.H * g.N[1] * g.n[2] * sizeof(cplx<T>) : 0;
+  const size_t c1 = (size_t)rows * g.Hp * sizeof(cplx<T>);
+  const size_t c2 = dim == 3 ? (size_t)g.Hp * g.N[1] * g.n[2] * sizeof(cplx<T>) : 0;
   P.w1.ensure(c1 * batch);
   if (dim == 3)
     P.w2.ensure(c2 * batch);
   {
     Scope sc(&P, "mv_r2c_x");
     const cplx<T>* tw = P.table<T>(g.N[0]);
-    if (!fast_r2c<T>(fc, M, x, P.w1.as<cplx<T>>(), rows, g.H, g.n[0], tw, batch)) {
+    if (!fast_r2c<T>(fc, M, x, P.w1.as<cplx<T>>(), rows, g.Hp, g.n[0], tw, batch)) {
       Sched s = make_sched(M);
       Cfg c = make_cfg<T>(M, rows, g.N[0]);
       k_r2c_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
-        x, P.w1.as<cplx<T>>(), rows, g.H, g.n[0], s, tw, c.L);
+        x, P.w1.as<cplx<T>>(), rows, g.Hp, g.n[0], s, tw, c.L);
     }
     check_launch();
   }
@@ -928,14 +937,14 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
   if (dim == 3) {
     Scope sc(&P, "mv_fwd_mid");
     const cplx<T>* tw = P.table<T>(g.N[1]);
-    if (!fast_mid<T>(fc, g.N[1], false, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.N[0] / 2,
+    if (!fast_mid<T>(fc, g.N[1], false, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.Hp, g.N[0] / 2,
                      g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], tw, batch)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
       Sched s = make_sched(g.N[1]);
       Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
       k_pass_mid<T, ME, false><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
-        P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], s,
-        tw, c.L);
+        P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.Hp, g.n[2], g.n[1], g.n[1], g.N[1], g.N[1],
+        s, tw, c.L);
     }
     check_launch();
     C = P.w2.as<cplx<T>>();
@@ -958,26 +967,26 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
   if (dim == 3) {
     Scope sc(&P, "mv_inv_mid");
     const cplx<T>* tw = P.table<T>(g.N[1]);
-    if (!fast_mid<T>(fc, g.N[1], true, P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.N[0] / 2,
+    if (!fast_mid<T>(fc, g.N[1], true, P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.Hp, g.N[0] / 2,
                      g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], tw, batch)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
       Sched s = make_sched(g.N[1]);
       Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
       k_pass_mid<T, ME, true><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
-        P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s,
-        tw, c.L);
+        P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.Hp, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1],
+        s, tw, c.L);
     }
     check_launch();
   }
   {
     Scope sc(&P, "mv_c2r_x");
     const cplx<T>* tw = P.table<T>(g.N[0]);
-    if (!fast_c2r<T>(fc, M, P.w1.as<cplx<T>>(), y, rows, g.H, g.n[0], tw, PF_TRANSFORM_NONE, T(0),
+    if (!fast_c2r<T>(fc, M, P.w1.as<cplx<T>>(), y, rows, g.Hp, g.n[0], tw, PF_TRANSFORM_NONE, T(0),
                      batch)) {
       Sched s = make_sched(M);
       Cfg c = make_cfg<T>(M, rows, g.N[0]);
       k_c2r_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
-        P.w1.as<cplx<T>>(), y, rows, g.H, g.n[0], s, tw, c.L, PF_TRANSFORM_NONE, T(0));
+        P.w1.as<cplx<T>>(), y, rows, g.Hp, g.n[0], s, tw, c.L, PF_TRANSFORM_NONE, T(0));
     }
     check_launch();
   }
@@ -1248,13 +1257,18 @@ struct Batch
 {
   std::vector<std::unique_ptr<Plan>> plans;
   std::vector<int64_t> offsets;
-  hipStream_t stream = nullptr;
+  std::vector<double> cost;           // algorithmic bytes per field (bin packing weight)
+  hipStream_t stream = nullptr;       // origin stream
+  std::vector<hipStream_t> branch;    // extra streams of the forked graph branches
+  std::vector<hipEvent_t> ev;         // fork/join events (one per branch)
+  hipEvent_t ev_fork = nullptr;
+  int branches = 1;
   DevBuf stats, stage;
   bool prepared = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t key_seed = 0, key_first = 0;
-  int key_transform = -1;
+  int key_transform = -1, key_branches = 0;
   double key_param = 0.;
   void* key_out = nullptr;
   ~Batch()
@@ -1264,30 +1278,87 @@ struct Batch
     if (graph)
       (void)hipGraphDestroy(graph);
     plans.clear();
+    for (auto s : branch)
+      (void)hipStreamDestroy(s);
+    for (auto e : ev)
+      (void)hipEventDestroy(e);
+    if (ev_fork)
+      (void)hipEventDestroy(ev_fork);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
+  void ensure_branches(int k)
+  {
+    while ((int)branch.size() < k - 1) {
+      hipStream_t s;
+      HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      branch.push_back(s);
+      hipEvent_t e;
+      HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ev.push_back(e);
+    }
+    if (!ev_fork)
+      HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  }
+  void enqueue_field(size_t i, uint64_t seed, uint64_t first, int transform, double param,
+                     void* out)
+  {
+    Plan& P = *plans[i];
+    compute_eigs(P, nullptr, stats.as<ClassifyPartial>() + i);
+    if (P.d.precision == PF_FP32) {
+      float* o = (float*)out + offsets[i];
+      if (P.d.field_backend == PF_FIELD_DCTDST)
+        generate_dctdst<float>(P, nullptr, seed, first + i, 1, transform, param, o);
+      else
+        generate_r2c<float>(P, nullptr, seed, first + i, 1, transform, param, o);
+    } else {
+      double* o = (double*)out + offsets[i];
+      if (P.d.field_backend == PF_FIELD_DCTDST)
+        generate_dctdst<double>(P, nullptr, seed, first + i, 1, transform, param, o);
+      else
+        generate_r2c<double>(P, nullptr, seed, first + i, 1, transform, param, o);
+    }
+  }
   // Every field: FP64 eigenvalues (classification reduced on the device),
   // then one realization. No host synchronisation, so it can be captured.
-  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out)
+  // With k > 1 branches the fields are bin-packed by algorithmic bytes
+  // (largest first onto the least loaded branch) and each branch runs on its
+  // own stream forked from and joined to the origin stream by events.
+  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out, int k)
   {
-    ClassifyPartial* st = stats.as<ClassifyPartial>();
-    for (size_t i = 0; i < plans.size(); i++) {
-      Plan& P = *plans[i];
-      compute_eigs(P, nullptr, st + i);
-      if (P.d.precision == PF_FP32) {
-        float* o = (float*)out + offsets[i];
-        if (P.d.field_backend == PF_FIELD_DCTDST)
-          generate_dctdst<float>(P, nullptr, seed, first + i, 1, transform, param, o);
-        else
-          generate_r2c<float>(P, nullptr, seed, first + i, 1, transform, param, o);
-      } else {
-        double* o = (double*)out + offsets[i];
-        if (P.d.field_backend == PF_FIELD_DCTDST)
-          generate_dctdst<double>(P, nullptr, seed, first + i, 1, transform, param, o);
-        else
-          generate_r2c<double>(P, nullptr, seed, first + i, 1, transform, param, o);
+    const size_t n = plans.size();
+    if (k <= 1) {
+      for (size_t i = 0; i < n; i++) {
+        plans[i]->stream = stream;
+        enqueue_field(i, seed, first, transform, param, out);
       }
+      return;
+    }
+    ensure_branches(k);
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++)
+      order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<double> load(k, 0.);
+    std::vector<std::vector<size_t>> bins(k);
+    for (size_t i : order) {
+      const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[j] += cost[i];
+      bins[j].push_back(i);
+    }
+    HIPCHECK(hipEventRecord(ev_fork, stream));
+    for (int j = 1; j < k; j++)
+      HIPCHECK(hipStreamWaitEvent(branch[j - 1], ev_fork, 0));
+    for (int j = 0; j < k; j++) {
+      hipStream_t s = j == 0 ? stream : branch[j - 1];
+      for (size_t i : bins[j]) {
+        plans[i]->stream = s;
+        enqueue_field(i, seed, first, transform, param, out);
+      }
+    }
+    for (int j = 1; j < k; j++) {
+      HIPCHECK(hipEventRecord(ev[j - 1], branch[j - 1]));
+      HIPCHECK(hipStreamWaitEvent(stream, ev[j - 1], 0));
     }
   }
 };
@@ -1871,6 +1942,9 @@ int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
       P->capturing = true;
       B->offsets.push_back(off);
       off += P->g.domain;
+      // algorithmic bytes of one field: FP64 eigen passes + spectrum + output
+      const double es = P->d.precision == PF_FP32 ? 4. : 8.;
+      B->cost.push_back(48. * P->g.half + 4. * es * P->g.half + es * P->g.domain);
       B->plans.push_back(std::move(P));
     }
     B->offsets.push_back(off);
@@ -1917,18 +1991,19 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
       B->stage.ensure(outb);
       dout = B->stage.p;
     }
+    const int k = std::max(1, B->branches);
     if (!B->prepared) {
       // eager pass: allocates every buffer and twiddle table the capture needs
-      B->enqueue(seed, first_realization, transform, transform_param, dout);
+      B->enqueue(seed, first_realization, transform, transform_param, dout, 1);
       HIPCHECK(hipStreamSynchronize(B->stream));
       B->prepared = true;
     } else if (!use_graph) {
-      B->enqueue(seed, first_realization, transform, transform_param, dout);
+      B->enqueue(seed, first_realization, transform, transform_param, dout, 1);
     }
     if (use_graph) {
       const bool same = B->exec && B->key_seed == seed && B->key_first == first_realization &&
                         B->key_transform == transform && B->key_param == transform_param &&
-                        B->key_out == dout;
+                        B->key_out == dout && B->key_branches == k;
       if (!same) {
         if (B->exec)
           (void)hipGraphExecDestroy(B->exec);
@@ -1936,8 +2011,9 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
           (void)hipGraphDestroy(B->graph);
         B->exec = nullptr;
         B->graph = nullptr;
+        B->ensure_branches(k);
         HIPCHECK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));
-        B->enqueue(seed, first_realization, transform, transform_param, dout);
+        B->enqueue(seed, first_realization, transform, transform_param, dout, k);
         HIPCHECK(hipStreamEndCapture(B->stream, &B->graph));
         HIPCHECK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
         B->key_seed = seed;
@@ -1945,6 +2021,7 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
         B->key_transform = transform;
         B->key_param = transform_param;
         B->key_out = dout;
+        B->key_branches = k;
       }
       HIPCHECK(hipGraphLaunch(B->exec, B->stream));
     }
@@ -1973,6 +2050,58 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
     if (status != PF_OK)
       g_err = "negative eigenvalues in covariance matrix";
     return status;
+  })
+}
+
+int pf_batch_set_branches(void* batch, int32_t branches)
+{
+  PF_GUARD({
+    Batch* B = static_cast<Batch*>(batch);
+    if (branches < 1 || branches > 64)
+      throw PfError(PF_EINVAL, "branches must be in [1, 64]");
+    B->branches = branches;
+    return PF_OK;
+  })
+}
+
+// Autotuner of the batch packing: times one graph replay per candidate
+// branch count (1, 2, 4, 8, 16) on the real batch and keeps the fastest.
+int pf_batch_autotune(void* batch, uint64_t seed, uint64_t first_realization, int32_t transform,
+                      double transform_param, void* out, int32_t out_mem, int32_t* best,
+                      double* best_ms)
+{
+  PF_GUARD({
+    Batch* B = static_cast<Batch*>(batch);
+    int bestk = 1;
+    double bestt = 1e300;
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    for (int k : { 1, 2, 4, 8, 16 }) {
+      B->branches = k;
+      int st = pf_batch_generate(batch, seed, first_realization, transform, transform_param, out,
+                                 out_mem, 1, nullptr); // builds the graph for k
+      if (st != PF_OK && st != PF_ENEG)
+        return st;
+      HIPCHECK(hipEventRecord(a, B->stream));
+      HIPCHECK(hipGraphLaunch(B->exec, B->stream));
+      HIPCHECK(hipEventRecord(b, B->stream));
+      HIPCHECK(hipEventSynchronize(b));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, a, b));
+      if (ms < bestt) {
+        bestt = ms;
+        bestk = k;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    B->branches = bestk;
+    if (best)
+      *best = bestk;
+    if (best_ms)
+      *best_ms = bestt;
+    return PF_OK;
   })
 }
 

@@ -15,6 +15,8 @@ struct GenGeom
   int P0, P1, P2;       // DCT half sizes (eigen layout of the DCT backend)
   int dct;              // 1: eigen table is the DCT half array (fold k >= P)
   int64_t noise_stride; // samples per realization
+  int Hp;               // row pitch (complex entries) of the spectrum work arrays, H rounded up
+                        // to 128-byte rows
 };
 
 // Reference storage index of half-spectrum entry (kx, k1, k2): the order in

@@ -28,6 +28,7 @@ struct DctGeom
   int64_t Ptot;       // P0*P1*P2
   int64_t sx, s1, s2; // reference storage index of (kx, k1, k2): kx*sx + k1*s1 + k2*s2
   int64_t noise_stride;
+  int P0p;            // row pitch (reals) of the G / H work arrays: P0 rounded up to 128 bytes
 };
 
 // a_t at logical (k0, k1, k2), without the noise factor: returns the factor
@@ -131,15 +132,16 @@ __global__ void kd_gen_last(DctGeom g, const T* sq, const double* noise, uint64_
   const double* nz = noise ? noise + (int64_t)b * g.noise_stride : nullptr;
   const int ncomb = 1 << (g.dim - 1);
   const int oddlast = 1 << (g.dim - 1);
-  const int64_t plane = (int64_t)Pm * g.P0;
+  const int64_t plane = (int64_t)Pm * g.P0p;
   T* out = G + (int64_t)b * ncomb * plane * nlast;
   for (int c = 0; c < ncomb; c++) {
     const int t0 = c, t1 = c + oddlast;
     if (NOISE == 1 && !edge && g.sx == 1) {
       // consecutive kx are consecutive samples: one Philox block (4 normals)
       // feeds up to 4 entries of the tile; NB blocks cover the L samples of a
-      // (type, m) row whatever its alignment
-      const int NB = L / 4 + 1;
+      // (type, m) row whatever its alignment: samples q0 .. q0+L-1 touch at
+      // most ceil((L+3)/4) = (L+6)/4 blocks
+      const int NB = (L + 6) / 4;
       for (int idx = threadIdx.x; idx < 2 * NB * (M + 1); idx += NT) {
         const int j = idx % NB, rest = idx / NB;
         const int half = rest & 1, m = rest >> 1;
@@ -208,7 +210,7 @@ __global__ void kd_gen_last(DctGeom g, const T* sq, const double* noise, uint64_
       const int km = edge ? km0 + l : km0;
       if (km < Pm) {
         const cplx<T> v = buf[(j >> 1) * SE + l];
-        out[(int64_t)c * plane * nlast + (int64_t)j * plane + (int64_t)km * g.P0 + kx] =
+        out[(int64_t)c * plane * nlast + (int64_t)j * plane + (int64_t)km * g.P0p + kx] =
           (j & 1) ? v.y : v.x;
       }
     }
@@ -241,9 +243,9 @@ __global__ void kd_mid(DctGeom g, const T* G, T* Hh, const cplx<T>* tw, Sched s,
       return;
   }
   const int b = blockIdx.z;
-  const int64_t gplane = (int64_t)g.P1 * g.P0;      // one z slice of G
+  const int64_t gplane = (int64_t)g.P1 * g.P0p;     // one z slice of G
   const int64_t gsize = gplane * g.n2;               // one G array
-  const int64_t hplane = (int64_t)g.n1 * g.P0;
+  const int64_t hplane = (int64_t)g.n1 * g.P0p;
   const int64_t hsize = hplane * g.n2;
   const T* Gb = G + (int64_t)b * 4 * gsize;
   T* Hb = Hh + (int64_t)b * 2 * hsize;
@@ -256,7 +258,7 @@ __global__ void kd_mid(DctGeom g, const T* G, T* Hh, const cplx<T>* tw, Sched s,
       const int z = edge ? z0 + l : z0;
       cplx<T> x = { 0, 0 };
       if (z < g.n2) {
-        const int64_t o = (int64_t)z * gplane + (int64_t)m * g.P0 + kx;
+        const int64_t o = (int64_t)z * gplane + (int64_t)m * g.P0p + kx;
         x = { ge[o], -go[o] };
       }
       if (m < M)
@@ -272,7 +274,7 @@ __global__ void kd_mid(DctGeom g, const T* G, T* Hh, const cplx<T>* tw, Sched s,
       const int z = edge ? z0 + l : z0;
       if (z < g.n2) {
         const cplx<T> v = buf[(y >> 1) * SE + l];
-        Hb[(int64_t)tx * hsize + (int64_t)z * hplane + (int64_t)y * g.P0 + kx] =
+        Hb[(int64_t)tx * hsize + (int64_t)z * hplane + (int64_t)y * g.P0p + kx] =
           (y & 1) ? v.y : v.x;
       }
     }
@@ -295,7 +297,7 @@ __global__ void kd_x(DctGeom g, const T* Hh, T* out, int64_t rows, const cplx<T>
   cplx<T>* xM = buf + (int64_t)SE * M;
   const int P0 = M + 1;
   const int b = blockIdx.y;
-  const int64_t hsize = rows * P0;
+  const int64_t hsize = rows * g.P0p;
   const T* h0 = Hh + (int64_t)b * 2 * hsize;
   const T* h1 = h0 + hsize;
   out += (int64_t)b * rows * g.n0;
@@ -305,7 +307,7 @@ __global__ void kd_x(DctGeom g, const T* Hh, T* out, int64_t rows, const cplx<T>
     const int64_t r = row0 + l;
     cplx<T> x = { 0, 0 };
     if (r < rows)
-      x = { h0[r * P0 + m], -h1[r * P0 + m] };
+      x = { h0[r * g.P0p + m], -h1[r * g.P0p + m] };
     if (m < M)
       buf[m * SE + l] = x;
     else

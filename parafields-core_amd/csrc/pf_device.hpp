@@ -48,6 +48,17 @@ __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s)
 // key = 64-bit seed; uniforms u = (x + 0.5) 2^-32; Box-Muller on (u0,u1) and
 // (u2,u3): n_a = sqrt(-2 ln u_a) cos(2 pi u_b), n_b = ... sin(2 pi u_b).
 // ---------------------------------------------------------------------------
+// a ^ b ^ k in one gfx950 three-input bit operation (register-only ALU
+// instruction; truth table 0x96 is the parity of the three inputs, symmetric
+// in their order). k must be wave-uniform: every caller passes a key derived
+// from a kernel argument, held in an SGPR.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t k)
+{
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(k));
+  return r;
+}
+
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -61,8 +72,8 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
     const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
     const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-    const uint32_t n0 = hi1 ^ c[1] ^ k0;
-    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    const uint32_t n0 = xor3(hi1, c[1], k0);
+    const uint32_t n2 = xor3(hi0, c[3], k1);
     c[0] = n0;
     c[1] = lo1;
     c[2] = n2;
@@ -98,8 +109,11 @@ struct BoxMuller<float>
     // u in (0, 1]: float of the 32-bit integer scaled by 2^-32 plus half an ulp
     const float ua = fmaf((float)xa, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
     const float ub = fmaf((float)xb, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
-    // ln(u) = log2(u) * ln 2 with the hardware log2
-    const float r = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(ua));
+    // ln(u) = log2(u) * ln 2 with the hardware log2; the argument of the
+    // root is in [0, 44.4] (never denormal), so the hardware square root
+    // (1 ulp) needs none of the IEEE wrapper's scaling steps
+    const float r =
+      __builtin_amdgcn_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(ua));
     // hardware sin/cos take their argument in revolutions: sin(2 pi ub)
     const float s = __builtin_amdgcn_sinf(ub);
     const float c = __builtin_amdgcn_cosf(ub);

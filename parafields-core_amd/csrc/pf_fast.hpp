@@ -140,11 +140,27 @@ __device__ __forceinline__ void tfft(cplx<T>* buf, const cplx<T>* tw)
   }
 }
 
+// Tile loops with a compile-time trip count: thread t handles elements
+// t, t + NT, ... of a tile of TOTAL elements. With the count known, every
+// global load of a tile is issued before the first wait (a loop bounded by
+// threadIdx.x would wait out one memory latency per element).
+template <int TOTAL, int NT>
+struct TileLoop
+{
+  static constexpr int IT = (TOTAL + NT - 1) / NT;
+  static __device__ __forceinline__ bool live(int it)
+  {
+    return TOTAL % NT == 0 || (int)threadIdx.x + it * NT < TOTAL;
+  }
+  static __device__ __forceinline__ int idx(int it) { return (int)threadIdx.x + it * NT; }
+};
+
 // Geometry of the half spectrum as the generation / matvec last-axis passes
 // see it: line (kx, km) with elements e along the last axis.
 struct LastAxis
 {
   int H, N0h;        // H = N0/2 + 1, N0h = N0/2
+  int Hp;            // row pitch of the work arrays (H rounded up to 128-byte rows)
   int Nmid;          // rows of lines (3D: N1, 2D: 1)
   int nlast;         // pruned output length
   int64_t sA, sB;    // storage index base = H * (e*sA + km*sB)
@@ -157,7 +173,7 @@ __device__ __forceinline__ cplx<T> make_entry(T lambda, T r1, T r2, bool selfcon
 {
   if (selfconj)
     return { lambda * r1, T(0) };
-  const T ls = lambda / (T)1.41421356237309504880;
+  const T ls = lambda * (T)0.70710678118654752440; // lambda / sqrt(2)
   return { ls * r1, ls * r2 };
 }
 
@@ -201,34 +217,68 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
   if (NOISE == 1 && !nyq) {
     // Philox pairs: entries s and s+1 (s even) share one block; with the row
     // base parity odd the pairs start one entry to the left of the tile.
+    // Per block: k_mid is fixed, so the storage index and the eigenvalue row
+    // are affine in e; kx < N0/2 needs no folding.
+    // Slots (p, e) are walked incrementally (slot += NT moves e by NT/SLOTS
+    // and p by NT%SLOTS with one carry), so no division or 32-bit multiply
+    // is left in the loop; the two root-eigenvalue loads of a slot are issued
+    // before its Philox rounds, which hide their latency.
     constexpr int SLOTS = L / 2 + 1;
-    for (int slot = threadIdx.x; slot < SLOTS * N; slot += NT) {
-      const int p = slot % SLOTS, e = slot / SLOTS;
-      int k1, k2;
-      line_k(g, kx0, km0, e, k1, k2);
-      const int64_t base = (int64_t)a.H * ((int64_t)e * a.sA + (int64_t)km0 * a.sB) + kx0;
+    constexpr int ITER = (SLOTS * N + NT - 1) / NT;
+    constexpr int DE = NT / SLOTS, DP = NT % SLOTS;
+    const bool d3 = g.dim == 3;
+    const int Pe = d3 ? g.P2 : g.P1, Ne = d3 ? g.N2 : g.N1;
+    int64_t eu, es; // eigen row offset = eu + es * e' (e' = e folded for the DCT half)
+    if (g.dct) {
+      const int f1 = km0 >= g.P1 ? g.N1 - km0 : km0;
+      eu = d3 ? (int64_t)g.P0 * f1 : 0;
+      es = d3 ? (int64_t)g.P0 * g.P1 : g.P0;
+    } else {
+      eu = d3 ? (int64_t)g.H * km0 : 0;
+      es = d3 ? (int64_t)g.H * g.N1 : g.H;
+    }
+    const bool sc1 = d3 ? self_conj(km0, g.N1) : true;
+    const uint32_t sk0 = (uint32_t)seed, sk1 = (uint32_t)(seed >> 32);
+    const int64_t rowstep = (int64_t)a.H * a.sA; // storage index step per e
+    const T* sqt = sq + eu + kx0;
+    int p = (int)threadIdx.x % SLOTS, e = (int)threadIdx.x / SLOTS;
+    int64_t base = (int64_t)a.H * ((int64_t)km0 * a.sB) + kx0 + rowstep * e;
+    int64_t up = es * e, dn = es * (int64_t)(Ne - e); // es*e and es*(Ne-e)
+#pragma unroll
+    for (int it = 0; it < ITER; it++) {
       const int odd = (int)(base & 1);
       const int la = 2 * p - odd, lb = la + 1;
-      if (la >= L || lb < 0)
-        continue;
-      const uint64_t blk = (uint64_t)(base + la) >> 1;
-      uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
-                        (uint32_t)(real >> 32) };
-      philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-      const bool sc12 = self_conj(k1, g.N1) && self_conj(k2, g.N2);
-      if (la >= 0) {
-        T r1, r2;
-        BoxMuller<T>::pair(c[0], c[1], r1, r2);
-        const int kx = kx0 + la;
-        const T lam = sq[eig_index(g, kx, k1, k2)];
-        buf[e * L + la] = make_entry<T>(lam, r1, r2, sc12 && self_conj(kx, g.N0));
+      if (e < N && la < L && lb >= 0) {
+        const T* sqrow = sqt + ((g.dct && e >= Pe) ? dn : up);
+        const T sa = sqrow[la < 0 ? 0 : la];
+        const T sb = sqrow[lb >= L ? L - 1 : lb];
+        const uint64_t blk = (uint64_t)(base + la) >> 1;
+        uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
+                          (uint32_t)(real >> 32) };
+        philox4x32_10(c, sk0, sk1);
+        const bool sc0 = sc1 && kx0 == 0 && self_conj(e, Ne); // only kx = 0 is on a plane
+        if (la >= 0) {
+          T r1, r2;
+          BoxMuller<T>::pair(c[0], c[1], r1, r2);
+          buf[e * L + la] = make_entry<T>(sa, r1, r2, sc0 && la == 0);
+        }
+        if (lb < L) {
+          T r1, r2;
+          BoxMuller<T>::pair(c[2], c[3], r1, r2);
+          buf[e * L + lb] = make_entry<T>(sb, r1, r2, sc0 && lb == 0);
+        }
       }
-      if (lb < L) {
-        T r1, r2;
-        BoxMuller<T>::pair(c[2], c[3], r1, r2);
-        const int kx = kx0 + lb;
-        const T lam = sq[eig_index(g, kx, k1, k2)];
-        buf[e * L + lb] = make_entry<T>(lam, r1, r2, sc12 && self_conj(kx, g.N0));
+      p += DP;
+      e += DE;
+      base += rowstep * DE;
+      up += es * DE;
+      dn -= es * DE;
+      if (p >= SLOTS) {
+        p -= SLOTS;
+        e += 1;
+        base += rowstep;
+        up += es;
+        dn -= es;
       }
     }
   } else {
@@ -257,14 +307,17 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
   }
   __syncthreads();
   tfft<T, N, L, NT, 0, true, 1>(buf, tw);
-  const int64_t plane = (int64_t)a.H * a.Nmid;
+  const int64_t plane = (int64_t)a.Hp * a.Nmid;
   cplx<T>* out = B1 + (int64_t)b * plane * a.nlast;
-  for (int idx = threadIdx.x; idx < L * a.nlast; idx += NT) {
+  using TL = TileLoop<L * N, NT>;
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
     const int l = idx % L, j = idx / L;
     const int kx = nyq ? kx0 : kx0 + l;
     const int km = nyq ? km0 + l : km0;
-    if (km < a.Nmid)
-      out[j * plane + (int64_t)km * a.H + kx] = buf[j * L + l];
+    if (TL::live(it) && j < a.nlast && km < a.Nmid)
+      out[j * plane + (int64_t)km * a.Hp + kx] = buf[j * L + l];
   }
 }
 
@@ -272,7 +325,7 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
 // rows per z, n_in leading points, rest zero), DFT, first n_out points ->
 // out[z][j][kx] (out_rows rows per z).
 template <typename T, int N, int L, int NT, bool INV>
-__global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, int H, int N0h,
+__global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, int Hp, int N0h,
                                              int nz, int in_rows, int n_in, int out_rows,
                                              int n_out, const cplx<T>* tw)
 {
@@ -291,65 +344,90 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
       return;
   }
   const int b = blockIdx.z;
-  in += (int64_t)b * H * nz * in_rows;
-  out += (int64_t)b * H * nz * out_rows;
-  for (int idx = threadIdx.x; idx < L * N; idx += NT) {
+  in += (int64_t)b * Hp * nz * in_rows;
+  out += (int64_t)b * Hp * nz * out_rows;
+  using TL = TileLoop<L * N, NT>;
+  cplx<T> v[TL::IT];
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
     const int l = idx % L, e = idx / L;
     const int kx = nyq ? kx0 : kx0 + l;
     const int z = nyq ? z0 + l : z0;
-    cplx<T> v = { 0, 0 };
-    if (e < n_in && z < nz)
-      v = in[((int64_t)z * in_rows + e) * H + kx];
-    buf[e * L + l] = v;
+    v[it] = { 0, 0 };
+    if (TL::live(it) && e < n_in && z < nz)
+      v[it] = in[((int64_t)z * in_rows + e) * Hp + kx];
   }
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++)
+    if (TL::live(it))
+      buf[TL::idx(it)] = v[it]; // element (e, l) sits at e * L + l = idx
   __syncthreads();
   tfft<T, N, L, NT, 0, INV, 1>(buf, tw);
-  for (int idx = threadIdx.x; idx < L * n_out; idx += NT) {
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
     const int l = idx % L, j = idx / L;
     const int kx = nyq ? kx0 : kx0 + l;
     const int z = nyq ? z0 + l : z0;
-    if (z < nz)
-      out[((int64_t)z * out_rows + j) * H + kx] = buf[j * L + l];
+    if (TL::live(it) && j < n_out && z < nz)
+      out[((int64_t)z * out_rows + j) * Hp + kx] = buf[idx];
   }
 }
 
 // c2r along x (see k_c2r_x for the algebra), M = N0/2 point inverse FFT on a
 // padded tile of L rows; first n0 outputs with the value transform.
 template <typename T, typename TO, int M, int L, int NT>
-__global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H,
+__global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int Hp,
                                                int n0, const cplx<T>* tw, int transform, T param)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int N0 = 2 * M, SE = L + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
   const int b = blockIdx.y;
-  in += (int64_t)b * rows * H;
+  in += (int64_t)b * rows * Hp;
   out += (int64_t)b * rows * n0;
   const int64_t row0 = (int64_t)blockIdx.x * L;
-  for (int idx = threadIdx.x; idx < L * M; idx += NT) {
+  using TL = TileLoop<L * M, NT>;
+  cplx<T> xk[TL::IT], xm[TL::IT];
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
     const int l = idx / M, k = idx % M;
     const int64_t r = row0 + l;
-    cplx<T> z = { 0, 0 };
-    if (r < rows) {
-      cplx<T> xk = in[r * H + k];
-      cplx<T> xm = in[r * H + (M - k)];
-      if (k == 0) {
-        xk.y = 0;
-        xm.y = 0;
-      }
-      const cplx<T> s = { xk.x + xm.x, xk.y - xm.y };
-      const cplx<T> d = { xk.x - xm.x, xk.y + xm.y };
-      const cplx<T> td = cmul(cconj(tw[k]), d);
-      z = { s.x - td.y, s.y + td.x };
+    xk[it] = { 0, 0 };
+    xm[it] = { 0, 0 };
+    if (TL::live(it) && r < rows) {
+      xk[it] = in[r * Hp + k];
+      xm[it] = in[r * Hp + (M - k)];
     }
-    buf[k * SE + l] = z;
+  }
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
+    const int l = idx / M, k = idx % M;
+    if (!TL::live(it))
+      continue;
+    if (k == 0) {
+      xk[it].y = 0;
+      xm[it].y = 0;
+    }
+    const cplx<T> s = { xk[it].x + xm[it].x, xk[it].y - xm[it].y };
+    const cplx<T> d = { xk[it].x - xm[it].x, xk[it].y + xm[it].y };
+    const cplx<T> td = cmul(cconj(tw[k]), d);
+    buf[k * SE + l] = { s.x - td.y, s.y + td.x };
   }
   __syncthreads();
   tfft<T, M, L, NT, 1, true, 2>(buf, tw);
-  for (int idx = threadIdx.x; idx < L * n0; idx += NT) {
-    const int l = idx / n0, x = idx % n0;
+  using TO_ = TileLoop<L * N0, NT>;
+  // partial unroll: the value transform (a runtime switch, Box-Cox's pow
+  // included) would otherwise be replicated for every element
+#pragma unroll 4
+  for (int it = 0; it < TO_::IT; it++) {
+    const int idx = TO_::idx(it);
+    const int l = idx / N0, x = idx % N0;
     const int64_t r = row0 + l;
-    if (r < rows) {
+    if (TO_::live(it) && x < n0 && r < rows) {
       const cplx<T> v = buf[(x >> 1) * SE + l];
       out[r * n0 + x] = (TO)value_transform<T>(transform, (x & 1) ? v.y : v.x, param);
     }
@@ -358,39 +436,52 @@ __global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64
 
 // r2c along x of zero-padded real rows (n0 of N0 values), see k_r2c_x.
 template <typename T, typename TI, int M, int L, int NT>
-__global__ void __launch_bounds__(NT) kf_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H,
+__global__ void __launch_bounds__(NT) kf_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int Hp,
                                                int n0, const cplx<T>* tw)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int N0 = 2 * M, SE = L + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
   const int b = blockIdx.y;
+  constexpr int H = M + 1;
   in += (int64_t)b * rows * n0;
-  out += (int64_t)b * rows * H;
+  out += (int64_t)b * rows * Hp;
   const int64_t row0 = (int64_t)blockIdx.x * L;
-  for (int idx = threadIdx.x; idx < L * M; idx += NT) {
+  using TL = TileLoop<L * M, NT>;
+  cplx<T> z[TL::IT];
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
     const int l = idx / M, j = idx % M;
     const int64_t r = row0 + l;
-    cplx<T> z = { 0, 0 };
-    if (r < rows) {
+    z[it] = { 0, 0 };
+    if (TL::live(it) && r < rows) {
       const int x0 = 2 * j, x1 = 2 * j + 1;
-      z.x = x0 < n0 ? (T)in[r * n0 + x0] : T(0);
-      z.y = x1 < n0 ? (T)in[r * n0 + x1] : T(0);
+      z[it].x = x0 < n0 ? (T)in[r * n0 + x0] : T(0);
+      z[it].y = x1 < n0 ? (T)in[r * n0 + x1] : T(0);
     }
-    buf[j * SE + l] = z;
+  }
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
+    if (TL::live(it))
+      buf[(idx % M) * SE + idx / M] = z[it];
   }
   __syncthreads();
   tfft<T, M, L, NT, 1, false, 2>(buf, tw);
-  for (int idx = threadIdx.x; idx < L * H; idx += NT) {
+  using TH = TileLoop<L * H, NT>;
+#pragma unroll
+  for (int it = 0; it < TH::IT; it++) {
+    const int idx = TH::idx(it);
     const int l = idx / H, k = idx % H;
     const int64_t r = row0 + l;
-    if (r < rows) {
+    if (TH::live(it) && r < rows) {
       const cplx<T> zk = buf[(k % M) * SE + l];
       const cplx<T> zm = buf[((M - k) % M) * SE + l];
       const cplx<T> s = { (zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5) };
       const cplx<T> d = { (zk.x - zm.x) * T(0.5), (zk.y + zm.y) * T(0.5) };
       const cplx<T> wd = cmul(tw[k], d);
-      out[r * H + k] = { s.x + wd.y, s.y - wd.x };
+      out[r * Hp + k] = { s.x + wd.y, s.y - wd.x };
     }
   }
 }
@@ -417,16 +508,25 @@ __global__ void __launch_bounds__(NT) kf_fused_mult(cplx<T>* C, GenGeom g, LastA
     if (km0 >= a.Nmid)
       return;
   }
-  const int64_t plane = (int64_t)a.H * a.Nmid;
+  const int64_t plane = (int64_t)a.Hp * a.Nmid;
   C += (int64_t)blockIdx.z * plane * a.nlast;
-  for (int idx = threadIdx.x; idx < L * N; idx += NT) {
-    const int l = idx % L, e = idx / L;
-    const int kx = nyq ? kx0 : kx0 + l;
-    const int km = nyq ? km0 + l : km0;
-    cplx<T> v = { 0, 0 };
-    if (e < a.nlast && km < a.Nmid)
-      v = C[e * plane + (int64_t)km * a.H + kx];
-    buf[e * L + l] = v;
+  using TL = TileLoop<L * N, NT>;
+  {
+    cplx<T> v[TL::IT];
+#pragma unroll
+    for (int it = 0; it < TL::IT; it++) {
+      const int idx = TL::idx(it);
+      const int l = idx % L, e = idx / L;
+      const int kx = nyq ? kx0 : kx0 + l;
+      const int km = nyq ? km0 + l : km0;
+      v[it] = { 0, 0 };
+      if (TL::live(it) && e < a.nlast && km < a.Nmid)
+        v[it] = C[e * plane + (int64_t)km * a.Hp + kx];
+    }
+#pragma unroll
+    for (int it = 0; it < TL::IT; it++)
+      if (TL::live(it))
+        buf[TL::idx(it)] = v[it];
   }
   __syncthreads();
   tfft<T, N, L, NT, 0, false, 1>(buf, tw);
@@ -450,12 +550,14 @@ __global__ void __launch_bounds__(NT) kf_fused_mult(cplx<T>* C, GenGeom g, LastA
   }
   __syncthreads();
   tfft<T, N, L, NT, 0, true, 1>(buf, tw);
-  for (int idx = threadIdx.x; idx < L * a.nlast; idx += NT) {
+#pragma unroll
+  for (int it = 0; it < TL::IT; it++) {
+    const int idx = TL::idx(it);
     const int l = idx % L, j = idx / L;
     const int kx = nyq ? kx0 : kx0 + l;
     const int km = nyq ? km0 + l : km0;
-    if (km < a.Nmid)
-      C[j * plane + (int64_t)km * a.H + kx] = buf[j * L + l];
+    if (TL::live(it) && j < a.nlast && km < a.Nmid)
+      C[j * plane + (int64_t)km * a.Hp + kx] = buf[idx];
   }
 }
 

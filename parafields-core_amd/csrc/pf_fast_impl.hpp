@@ -8,7 +8,8 @@
 namespace pfamd {
 
 // Lines per tile: the largest power of two with L*N complex values in 64 KB
-// (FP32: 16 lines x 512; FP64: 8 x 512), one thread per MAXE values.
+// (FP32: 16 lines x 512; FP64: 8 x 512), one thread per 16 values.
+// The complex/real "H" arguments of the launchers below are row pitches.
 template <typename T>
 constexpr int fast_L(int N)
 {
@@ -22,7 +23,7 @@ constexpr int fast_L(int N)
 template <typename T>
 constexpr int fast_NT(int N, int L)
 {
-  const int me = sizeof(T) == 4 ? 32 : 16;
+  const int me = 16; // values per thread and stage (FP32: 512 threads for a 16 x 512 tile)
   int nt = (L * N + me - 1) / me;
   nt = ((nt + 63) / 64) * 64;
   return nt < 64 ? 64 : nt;

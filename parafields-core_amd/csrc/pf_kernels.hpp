@@ -325,7 +325,7 @@ __device__ __forceinline__ cplx<T> spectrum_entry(const GenGeom& g, const T* sq,
                            ((2 * k2) % g.N2 == 0);
   if (allMultiple)
     return { lambda * r1, T(0) };
-  const T ls = lambda / (T)1.41421356237309504880;
+  const T ls = lambda * (T)0.70710678118654752440; // lambda / sqrt(2)
   return { ls * r1, ls * r2 };
 }
 
@@ -362,12 +362,13 @@ __global__ void k_gen_last(GenGeom g, const T* sq, const double* noise, uint64_t
   }
   __syncthreads();
   fft_lds<T, MAXE, false, true>(buf, twl, 1, s, L, L, 1);
-  cplx<T>* out = B1 + (int64_t)b * nlines * nlast;
+  const int64_t plane = (int64_t)g.Hp * (g.dim == 3 ? g.N1 : 1);
+  cplx<T>* out = B1 + (int64_t)b * plane * nlast;
   for (int idx = threadIdx.x; idx < L * nlast; idx += blockDim.x) {
     const int l = idx % L, j = idx / L;
     const int64_t u = line0 + l;
     if (u < nlines)
-      out[j * nlines + u] = buf[j * L + l];
+      out[j * plane + (u / g.H) * g.Hp + (u % g.H)] = buf[j * L + l];
   }
 }
 
@@ -375,7 +376,7 @@ __global__ void k_gen_last(GenGeom g, const T* sq, const double* noise, uint64_t
 // forward with FWD); first n_out outputs -> B2[z][y][kx]. The input holds
 // n_in <= N1 leading points (rest zero).
 template <typename T, int MAXE, bool INV>
-__global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int nz, int in_rows,
+__global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int Hp, int nz, int in_rows,
                            int n_in, int out_rows, int n_out, Sched s, const cplx<T>* tw, int L)
 {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -386,8 +387,8 @@ __global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int nz, int i
     twl[i] = tw[i];
   const int b = blockIdx.y;
   const int64_t nlines = (int64_t)H * nz;
-  in += (int64_t)b * nlines * in_rows;
-  out += (int64_t)b * nlines * out_rows;
+  in += (int64_t)b * Hp * nz * in_rows;
+  out += (int64_t)b * Hp * nz * out_rows;
   const int64_t line0 = (int64_t)blockIdx.x * L;
   for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
     const int l = idx % L, e = idx / L;
@@ -395,7 +396,7 @@ __global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int nz, int i
     cplx<T> v = { 0, 0 };
     if (u < nlines && e < n_in) {
       const int64_t kx = u % H, z = u / H;
-      v = in[(z * in_rows + e) * H + kx];
+      v = in[(z * in_rows + e) * Hp + kx];
     }
     buf[e * L + l] = v;
   }
@@ -406,7 +407,7 @@ __global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int nz, int i
     const int64_t u = line0 + l;
     if (u < nlines) {
       const int64_t kx = u % H, z = u / H;
-      out[(z * out_rows + j) * H + kx] = buf[j * L + l];
+      out[(z * out_rows + j) * Hp + kx] = buf[j * L + l];
     }
   }
 }
@@ -417,7 +418,7 @@ __global__ void k_pass_mid(const cplx<T>* in, cplx<T>* out, int H, int nz, int i
 // out[2j] = Re z_j, out[2j+1] = Im z_j; first n0 values, value transform.
 // tw is the N0-point table (the M-point FFT uses every second entry).
 template <typename T, typename TO, int MAXE>
-__global__ void k_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H, int n0, Sched s,
+__global__ void k_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int Hp, int n0, Sched s,
                         const cplx<T>* tw, int L, int transform, T param)
 {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -428,7 +429,7 @@ __global__ void k_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H, int n0,
     twl[i] = tw[i];
   __syncthreads();
   const int b = blockIdx.y;
-  in += (int64_t)b * rows * H;
+  in += (int64_t)b * rows * Hp;
   out += (int64_t)b * rows * n0;
   const int64_t row0 = (int64_t)blockIdx.x * L;
   for (int idx = threadIdx.x; idx < L * M; idx += blockDim.x) {
@@ -436,8 +437,8 @@ __global__ void k_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H, int n0,
     const int64_t r = row0 + l;
     cplx<T> z = { 0, 0 };
     if (r < rows) {
-      cplx<T> xk = in[r * H + k];
-      cplx<T> xm = in[r * H + (M - k)];
+      cplx<T> xk = in[r * Hp + k];
+      cplx<T> xm = in[r * Hp + (M - k)];
       if (k == 0) {
         xk.y = 0;
         xm.y = 0;
@@ -466,7 +467,7 @@ __global__ void k_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int H, int n0,
 // complex via an M-point FFT of z_j = x_{2j} + i x_{2j+1}:
 // X_k = (Z_k + conj Z_{M-k})/2 - i e^{-2 pi i k/N0} (Z_k - conj Z_{M-k})/2.
 template <typename T, typename TI, int MAXE>
-__global__ void k_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H, int n0, Sched s,
+__global__ void k_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int Hp, int n0, Sched s,
                         const cplx<T>* tw, int L)
 {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -476,8 +477,9 @@ __global__ void k_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H, int n0,
   for (int i = threadIdx.x; i < N0; i += blockDim.x)
     twl[i] = tw[i];
   const int b = blockIdx.y;
+  const int H = M + 1;
   in += (int64_t)b * rows * n0;
-  out += (int64_t)b * rows * H;
+  out += (int64_t)b * rows * Hp;
   const int64_t row0 = (int64_t)blockIdx.x * L;
   for (int idx = threadIdx.x; idx < L * M; idx += blockDim.x) {
     const int l = idx / M, j = idx % M;
@@ -503,7 +505,7 @@ __global__ void k_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int H, int n0,
       const cplx<T> w = twl[k];                                           // e^{-2 pi i k/N0}
       const cplx<T> wd = cmul(w, d);
       // X_k = a - i * wd
-      out[r * H + k] = { a.x + wd.y, a.y - wd.x };
+      out[r * Hp + k] = { a.x + wd.y, a.y - wd.x };
     }
   }
 }
@@ -524,14 +526,15 @@ __global__ void k_fused_mult(cplx<T>* C, GenGeom g, const double* eig, int op, d
   const int b = blockIdx.y;
   const int64_t nlines = (g.dim == 3) ? (int64_t)g.H * g.N1 : g.H;
   const int nlast = (g.dim == 3) ? g.n2 : g.n1;
-  C += (int64_t)b * nlines * nlast;
+  const int64_t plane = (int64_t)g.Hp * (g.dim == 3 ? g.N1 : 1);
+  C += (int64_t)b * plane * nlast;
   const int64_t line0 = (int64_t)blockIdx.x * L;
   for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
     const int l = idx % L, e = idx / L;
     const int64_t u = line0 + l;
     cplx<T> v = { 0, 0 };
     if (u < nlines && e < nlast)
-      v = C[e * nlines + u];
+      v = C[e * plane + (u / g.H) * g.Hp + (u % g.H)];
     buf[e * L + l] = v;
   }
   __syncthreads();
@@ -559,7 +562,7 @@ __global__ void k_fused_mult(cplx<T>* C, GenGeom g, const double* eig, int op, d
     const int l = idx % L, j = idx / L;
     const int64_t u = line0 + l;
     if (u < nlines)
-      C[j * nlines + u] = buf[j * L + l];
+      C[j * plane + (u / g.H) * g.Hp + (u % g.H)] = buf[j * L + l];
   }
 }
 

@@ -198,6 +198,9 @@ def load(path: str = LIB_PATH):
         "pf_batch_destroy": (i32, [vp]),
         "pf_batch_offsets": (i32, [vp, vp]),
         "pf_batch_generate": (i32, [vp, u64, u64, i32, dp, vp, i32, i32, vp]),
+        "pf_batch_set_branches": (i32, [vp, i32]),
+        "pf_batch_autotune": (i32, [vp, u64, u64, i32, dp, vp, i32, ctypes.POINTER(i32),
+                                    ctypes.POINTER(dp)]),
         "pf_last_timings": (i32, [vp, ctypes.c_char_p, i32, ctypes.POINTER(dp), i32,
                                   ctypes.POINTER(i32)]),
     })
@@ -217,7 +220,8 @@ def exported_symbols():
             "pf_spectral_targets", "pf_mlp_create", "pf_mlp_destroy", "pf_mlp_nparams",
             "pf_mlp_params", "pf_mlp_forward", "pf_mlp_gradients", "pf_mlp_train",
             "pf_set_profiling", "pf_last_timings", "pf_batch_create", "pf_batch_destroy",
-            "pf_batch_offsets", "pf_batch_generate"]
+            "pf_batch_offsets", "pf_batch_generate", "pf_batch_set_branches",
+            "pf_batch_autotune"]
 
 
 def check(status: int):
@@ -463,6 +467,19 @@ class Batch:
         if not (allow_negative and status == PF_ENEG):
             check(status)
         return out, [s.as_dict() for s in st]
+
+    def set_branches(self, k: int):
+        check(self.lib.pf_batch_set_branches(self.h, k))
+
+    def autotune(self, seed=0, first=0, transform="none", param=0.0, out=None):
+        if out is None:
+            out = np.empty(int(self.offsets[-1]), dtype=self.dtype)
+        best = ctypes.c_int32()
+        ms = ctypes.c_double()
+        check(self.lib.pf_batch_autotune(self.h, seed, first, TRANSFORMS[transform], param,
+                                         _ptr(out), _mem(out), ctypes.byref(best),
+                                         ctypes.byref(ms)))
+        return best.value, ms.value
 
     def field(self, out, i):
         return out[int(self.offsets[i]):int(self.offsets[i + 1])]

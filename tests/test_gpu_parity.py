@@ -502,3 +502,22 @@ def test_uneven_batch_graph_matches_single_plans():
         want = orc.generate(d64, orc.noise_philox(21, 100 + i, 0,
                                                   int(orc.sizes(d64).noise_per_field)))
         assert np.abs(b.field(out_g, i) - want).max() / want.std() < FP32_TOL
+
+
+def test_uneven_batch_branches_and_autotune():
+    """Forked graph branches (fields bin-packed over concurrent streams) give
+    the same bits as the serial graph; the autotuner picks one of the
+    candidate branch counts and keeps the result unchanged."""
+    descs = _uneven_descs(40, seed=5)
+    b = pfb.Batch(descs)
+    ref, _ = b.generate(seed=9, first=3, use_graph=True)
+    for k in (2, 4, 16):
+        b.set_branches(k)
+        out, _ = b.generate(seed=9, first=3, use_graph=True)
+        out2, _ = b.generate(seed=9, first=3, use_graph=True)
+        assert np.array_equal(out, ref), k
+        assert np.array_equal(out2, ref), k
+    best, ms = b.autotune(seed=9, first=3)
+    assert best in (1, 2, 4, 8, 16) and ms > 0
+    out, _ = b.generate(seed=9, first=3, use_graph=True)
+    assert np.array_equal(out, ref)
