@@ -299,7 +299,10 @@ def main():
                     help="first run a 1 GiB device-to-device copy of known size (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     ap.add_argument("--extra", action="store_true",
-                    help="also run the uneven-batch, CG and DCT/DST legs")
+                    help="also run the uneven-batch and CG legs")
+    ap.add_argument("--config5", action="store_true",
+                    help="also run the 1024^3 DCT/DST leg (kept out of --extra: two runs of it after "
+                         "the other legs ended in GPU memory faults of unidentified cause)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine holds the whole extended grid on one "
@@ -333,9 +336,13 @@ def main():
     traffic = measured_traffic(dom, fields_per_launch[dom])
     fp64_fps = None if a.no_fp64 else run_gpu_fp64_single(3)
     extra = {}
+    legs = []
     if a.extra:
-        for name, fn in (("uneven_batch_2d", run_uneven), ("cg_vs_learned_pcg", run_cg),
-                         ("dctdst_1024", run_dctdst)):
+        legs += [("uneven_batch_2d", run_uneven), ("cg_vs_learned_pcg", run_cg)]
+    if a.config5:
+        legs += [("dctdst_1024", run_dctdst)]
+    if legs:
+        for name, fn in legs:
             try:
                 print(f"bench: {name} leg", file=sys.stderr, flush=True)
                 extra[name] = fn()

@@ -7,6 +7,24 @@
 
 namespace pfamd {
 
+// XCD-aware block order. The hardware hands linear block b (x fastest) to
+// XCD b % 8 (MI355X: 8 XCDs of 32 CUs, one L2 each). This returns the
+// logical block (bx, by, bz) so that the blocks one XCD receives form one
+// contiguous run of logical indices: tiles that share cache lines (the
+// neighbouring kx tiles of a row) then meet in the same L2.
+__device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
+{
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned n = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned x = b & 7u, q = n >> 3, r = n & 7u;
+  const unsigned lam = x * q + (x < r ? x : r) + (b >> 3);
+  bx = (int)(lam % gx);
+  const unsigned t = lam / gx;
+  by = (int)(t % gy);
+  bz = (int)(t / gy);
+}
+
 struct GenGeom
 {
   int dim, transposed;

@@ -12,6 +12,8 @@
 // consecutive rows, which keeps H = N0/2+1 (odd) out of every division.
 #pragma once
 
+#include <type_traits>
+
 #include "pf_common.hpp"
 #include "pf_fft.hpp"
 
@@ -85,7 +87,7 @@ constexpr int ct_nst(int n)
 // One Stockham stage; tile layout buf[e*SE + l], SE = L + PAD; lane mapping
 // l fastest. Ends with a barrier.
 template <typename T, int N, int L, int NT, int PAD, int R, int NS, bool INV, int TWM>
-__device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw)
+__device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid)
 {
   constexpr int SE = L + PAD;
   constexpr int NR = N / R;
@@ -95,7 +97,7 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw)
   cplx<T> v[IT][R];
 #pragma unroll
   for (int it = 0; it < IT; it++) {
-    const int w = threadIdx.x + it * NT;
+    const int w = tid + it * NT;
     if (TOTAL % NT == 0 || w < TOTAL) {
       const int l = w % L, j = w / L;
       const int k = j % NS;
@@ -117,7 +119,7 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw)
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < IT; it++) {
-    const int w = threadIdx.x + it * NT;
+    const int w = tid + it * NT;
     if (TOTAL % NT == 0 || w < TOTAL) {
       const int l = w % L, j = w / L;
       const int k = j % NS;
@@ -130,14 +132,26 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw)
   __syncthreads();
 }
 
+// tid: the thread's index in the block. Callers inside a loop pass a value
+// from opaque_tid() so the per-element index math is not hoisted out of the
+// loop (where it would stay live in registers).
 template <typename T, int N, int L, int NT, int PAD, bool INV, int TWM, int ST = 0, int NS = 1>
-__device__ __forceinline__ void tfft(cplx<T>* buf, const cplx<T>* tw)
+__device__ __forceinline__ void tfft(cplx<T>* buf, const cplx<T>* tw, int tid = threadIdx.x)
 {
   if constexpr (ST < ct_nst(N)) {
     constexpr int R = ct_radix(N, ST);
-    tstage<T, N, L, NT, PAD, R, NS, INV, TWM>(buf, tw);
-    tfft<T, N, L, NT, PAD, INV, TWM, ST + 1, NS * R>(buf, tw);
+    tstage<T, N, L, NT, PAD, R, NS, INV, TWM>(buf, tw, tid);
+    tfft<T, N, L, NT, PAD, INV, TWM, ST + 1, NS * R>(buf, tw, tid);
   }
+}
+
+// threadIdx.x through an empty asm statement: each call yields a value the
+// compiler cannot prove equal to the previous one.
+__device__ __forceinline__ int opaque_tid()
+{
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
 }
 
 // Tile loops with a compile-time trip count: thread t handles elements
@@ -148,11 +162,11 @@ template <int TOTAL, int NT>
 struct TileLoop
 {
   static constexpr int IT = (TOTAL + NT - 1) / NT;
-  static __device__ __forceinline__ bool live(int it)
+  static __device__ __forceinline__ bool live(int it, int tid = threadIdx.x)
   {
-    return TOTAL % NT == 0 || (int)threadIdx.x + it * NT < TOTAL;
+    return TOTAL % NT == 0 || tid + it * NT < TOTAL;
   }
-  static __device__ __forceinline__ int idx(int it) { return (int)threadIdx.x + it * NT; }
+  static __device__ __forceinline__ int idx(int it, int tid = threadIdx.x) { return tid + it * NT; }
 };
 
 // Geometry of the half spectrum as the generation / matvec last-axis passes
@@ -191,7 +205,9 @@ __device__ __forceinline__ void line_k(const GenGeom& g, int kx, int km, int e, 
 
 // Pass G: spectrum generated in registers along the last axis (length N),
 // inverse DFT, first nlast outputs -> B1[j][km][kx].
-template <typename T, int N, int L, int NT, int NOISE>
+// IDX64: storage indices and eigenvalue offsets in 64 bits (fields with 2^31
+// or more half-spectrum entries); otherwise the slot walk runs in 32 bits.
+template <typename T, int N, int L, int NT, int NOISE, bool IDX64>
 __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T* sq,
                                                   const double* noise, uint64_t seed,
                                                   uint64_t first_real, cplx<T>* B1,
@@ -200,18 +216,21 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
   const int ntile = a.N0h / L;
-  const bool nyq = (int)blockIdx.x >= ntile;
+  int bx, by, b;
+  xcd_block(bx, by, b);
+  const bool nyq = bx >= ntile;
   int kx0, km0;
   if (!nyq) {
-    kx0 = blockIdx.x * L;
-    km0 = blockIdx.y;
+    kx0 = bx * L;
+    // DCT-half eigenvalues: rows km and Nmid-km read the same table row, so
+    // they take neighbouring logical positions (and one XCD's L2)
+    km0 = (g.dct && g.dim == 3) ? ((by & 1) ? a.Nmid - ((by + 1) >> 1) : (by >> 1)) : by;
   } else {
     kx0 = a.N0h;
-    km0 = blockIdx.y * L;
+    km0 = by * L;
     if (km0 >= a.Nmid)
       return;
   }
-  const int b = blockIdx.z;
   const uint64_t real = first_real + b;
   const double* nz = noise ? noise + (int64_t)b * g.noise_stride : nullptr;
   if (NOISE == 1 && !nyq) {
@@ -239,11 +258,13 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
     }
     const bool sc1 = d3 ? self_conj(km0, g.N1) : true;
     const uint32_t sk0 = (uint32_t)seed, sk1 = (uint32_t)(seed >> 32);
-    const int64_t rowstep = (int64_t)a.H * a.sA; // storage index step per e
+    using IX = typename std::conditional<IDX64, int64_t, uint32_t>::type;
+    const IX rowstep = (IX)((int64_t)a.H * a.sA); // storage index step per e
+    const IX est = (IX)es;
     const T* sqt = sq + eu + kx0;
     int p = (int)threadIdx.x % SLOTS, e = (int)threadIdx.x / SLOTS;
-    int64_t base = (int64_t)a.H * ((int64_t)km0 * a.sB) + kx0 + rowstep * e;
-    int64_t up = es * e, dn = es * (int64_t)(Ne - e); // es*e and es*(Ne-e)
+    IX base = (IX)((int64_t)a.H * ((int64_t)km0 * a.sB) + kx0) + rowstep * (IX)e;
+    IX up = est * (IX)e, dn = est * (IX)(Ne - e); // es*e and es*(Ne-e)
 #pragma unroll
     for (int it = 0; it < ITER; it++) {
       const int odd = (int)(base & 1);
@@ -252,7 +273,7 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
         const T* sqrow = sqt + ((g.dct && e >= Pe) ? dn : up);
         const T sa = sqrow[la < 0 ? 0 : la];
         const T sb = sqrow[lb >= L ? L - 1 : lb];
-        const uint64_t blk = (uint64_t)(base + la) >> 1;
+        const uint64_t blk = (uint64_t)(IX)(base + (IX)la) >> 1;
         uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
                           (uint32_t)(real >> 32) };
         philox4x32_10(c, sk0, sk1);
@@ -270,15 +291,15 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
       }
       p += DP;
       e += DE;
-      base += rowstep * DE;
-      up += es * DE;
-      dn -= es * DE;
+      base += rowstep * (IX)DE;
+      up += est * (IX)DE;
+      dn -= est * (IX)DE;
       if (p >= SLOTS) {
         p -= SLOTS;
         e += 1;
         base += rowstep;
-        up += es;
-        dn -= es;
+        up += est;
+        dn -= est;
       }
     }
   } else {
@@ -308,16 +329,22 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
   __syncthreads();
   tfft<T, N, L, NT, 0, true, 1>(buf, tw);
   const int64_t plane = (int64_t)a.Hp * a.Nmid;
-  cplx<T>* out = B1 + (int64_t)b * plane * a.nlast;
+  // block-uniform base; element (j, l) at j*plane + l*lstep
+  cplx<T>* ob = B1 + (int64_t)b * plane * a.nlast + (int64_t)km0 * a.Hp + kx0;
+  const int64_t lstep = nyq ? a.Hp : 1;
+  const int lmax = nyq ? a.Nmid - km0 : L; // live columns of the tile
   using TL = TileLoop<L * N, NT>;
+  static_assert(NT % L == 0, "a thread keeps its column across iterations");
+  constexpr int JS = NT / L; // rows advanced per iteration
+  const int l = (int)threadIdx.x % L, j0 = (int)threadIdx.x / L;
+  cplx<T>* op = ob + (int64_t)j0 * plane + l * lstep;
+  const bool lok = l < lmax;
 #pragma unroll
   for (int it = 0; it < TL::IT; it++) {
-    const int idx = TL::idx(it);
-    const int l = idx % L, j = idx / L;
-    const int kx = nyq ? kx0 : kx0 + l;
-    const int km = nyq ? km0 + l : km0;
-    if (TL::live(it) && j < a.nlast && km < a.Nmid)
-      out[j * plane + (int64_t)km * a.Hp + kx] = buf[j * L + l];
+    if (it * JS >= a.nlast) // uniform: the pruned rows are skipped whole
+      break;
+    if (TL::live(it) && lok && j0 + it * JS < a.nlast)
+      op[(int64_t)it * JS * plane] = buf[TL::idx(it)];
   }
 }
 
@@ -347,16 +374,20 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
   in += (int64_t)b * Hp * nz * in_rows;
   out += (int64_t)b * Hp * nz * out_rows;
   using TL = TileLoop<L * N, NT>;
+  static_assert(NT % L == 0, "a thread keeps its column across iterations");
+  constexpr int JS = NT / L; // rows advanced per iteration
+  // block-uniform bases; a thread's column l is fixed, its row is j0 + it*JS
+  const int l = (int)threadIdx.x % L, j0 = (int)threadIdx.x / L;
+  const int lmax = nyq ? nz - z0 : L; // live columns of the tile
+  const bool lok = l < lmax;
+  const cplx<T>* ip = in + ((int64_t)z0 * in_rows + j0) * Hp + kx0 +
+                      (nyq ? (int64_t)l * in_rows * Hp : (int64_t)l);
   cplx<T> v[TL::IT];
 #pragma unroll
   for (int it = 0; it < TL::IT; it++) {
-    const int idx = TL::idx(it);
-    const int l = idx % L, e = idx / L;
-    const int kx = nyq ? kx0 : kx0 + l;
-    const int z = nyq ? z0 + l : z0;
     v[it] = { 0, 0 };
-    if (TL::live(it) && e < n_in && z < nz)
-      v[it] = in[((int64_t)z * in_rows + e) * Hp + kx];
+    if (TL::live(it) && lok && j0 + it * JS < n_in)
+      v[it] = ip[(int64_t)it * JS * Hp];
   }
 #pragma unroll
   for (int it = 0; it < TL::IT; it++)
@@ -364,14 +395,14 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
       buf[TL::idx(it)] = v[it]; // element (e, l) sits at e * L + l = idx
   __syncthreads();
   tfft<T, N, L, NT, 0, INV, 1>(buf, tw);
+  cplx<T>* op = out + ((int64_t)z0 * out_rows + j0) * Hp + kx0 +
+                (nyq ? (int64_t)l * out_rows * Hp : (int64_t)l);
 #pragma unroll
   for (int it = 0; it < TL::IT; it++) {
-    const int idx = TL::idx(it);
-    const int l = idx % L, j = idx / L;
-    const int kx = nyq ? kx0 : kx0 + l;
-    const int z = nyq ? z0 + l : z0;
-    if (TL::live(it) && j < n_out && z < nz)
-      out[((int64_t)z * out_rows + j) * Hp + kx] = buf[idx];
+    if (it * JS >= n_out) // uniform: the pruned rows are skipped whole
+      break;
+    if (TL::live(it) && lok && j0 + it * JS < n_out)
+      op[(int64_t)it * JS * Hp] = buf[TL::idx(it)];
   }
 }
 

@@ -2,6 +2,7 @@
 // one translation unit per precision).
 #pragma once
 
+#include "pf_dctdst_fast.hpp"
 #include "pf_fast.hpp"
 #include "pf_fast_launch.hpp"
 
@@ -42,10 +43,19 @@ bool gen_last_n(const FastCtx& c, const GenGeom& g, const LastAxis& a, const T* 
   if (a.N0h % L != 0 || lds > c.lds_budget)
     return false;
   dim3 grid(a.N0h / L + 1, a.Nmid, batch);
+  // 32-bit slot walk when every storage index and eigenvalue offset of a
+  // realization stays below 2^31
+  const int64_t entries = (int64_t)a.H * a.Nmid * N;
+  const bool idx64 = entries >= ((int64_t)1 << 31) || g.noise_stride >= ((int64_t)1 << 32);
   if (noise)
-    kf_gen_last<T, N, L, NT, 0><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first, B1, tw);
+    kf_gen_last<T, N, L, NT, 0, true><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first,
+                                                                   B1, tw);
+  else if (idx64)
+    kf_gen_last<T, N, L, NT, 1, true><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first,
+                                                                   B1, tw);
   else
-    kf_gen_last<T, N, L, NT, 1><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first, B1, tw);
+    kf_gen_last<T, N, L, NT, 1, false><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first,
+                                                                    B1, tw);
   return true;
 }
 
@@ -185,6 +195,115 @@ bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const Las
   }
 }
 
+// --- DCT/DST field passes ---------------------------------------------------
+// Tiles of L lines of M+1 points: the largest power of two L <= 32 (FP32) /
+// 16 (FP64) whose tile fits in 80 KB, so two blocks share a CU's 160 KB.
+template <typename T>
+constexpr int dct_L(int M)
+{
+  int L = sizeof(T) == 4 ? 32 : 16;
+  while (L > 1 && (size_t)((L + 1) * M + L) * 2 * sizeof(T) > (size_t)80 * 1024)
+    L /= 2;
+  return L;
+}
+
+template <typename T, int M>
+constexpr size_t dct_lds()
+{
+  return (size_t)((dct_L<T>(M) + 1) * M + dct_L<T>(M)) * 2 * sizeof(T);
+}
+
+#define PF_DCT_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+
+template <typename T, int M>
+bool dct_gen_last_n(const FastCtx& c, const DctGeom& g, const T* sq, const double* noise,
+                    uint64_t seed, uint64_t first, T* G, const cplx<T>* tw, int batch)
+{
+  constexpr int L = dct_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr size_t lds = dct_lds<T, M>();
+  const int Pm = g.dim == 3 ? g.P1 : 1;
+  if ((g.P0 - 1) % L != 0 || lds > c.lds_budget)
+    return false;
+  dim3 grid((g.P0 - 1) / L + 1, Pm, batch);
+  if (noise)
+    kdf_gen_last<T, M, L, NT, 0><<<grid, NT, lds, c.stream>>>(g, sq, noise, seed, first, G, tw);
+  else
+    kdf_gen_last<T, M, L, NT, 1><<<grid, NT, lds, c.stream>>>(g, sq, noise, seed, first, G, tw);
+  return true;
+}
+
+template <typename T>
+bool fast_dct_gen_last(const FastCtx& c, int M, const DctGeom& g, const T* sq,
+                       const double* noise, uint64_t seed, uint64_t first, T* G,
+                       const cplx<T>* tw, int batch)
+{
+  switch (M) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return dct_gen_last_n<T, n>(c, g, sq, noise, seed, first, G, tw, batch);
+    PF_DCT_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int M>
+bool dct_mid_n(const FastCtx& c, const DctGeom& g, const T* G, T* Hh, const cplx<T>* tw,
+               int batch)
+{
+  constexpr int L = dct_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr size_t lds = dct_lds<T, M>();
+  if ((g.P0 - 1) % L != 0 || lds > c.lds_budget)
+    return false;
+  dim3 grid((g.P0 - 1) / L + 1, g.n2, batch);
+  kdf_mid<T, M, L, NT><<<grid, NT, lds, c.stream>>>(g, G, Hh, tw);
+  return true;
+}
+
+template <typename T>
+bool fast_dct_mid(const FastCtx& c, int M, const DctGeom& g, const T* G, T* Hh,
+                  const cplx<T>* tw, int batch)
+{
+  switch (M) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return dct_mid_n<T, n>(c, g, G, Hh, tw, batch);
+    PF_DCT_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int M>
+bool dct_x_n(const FastCtx& c, const DctGeom& g, const T* Hh, T* out, int64_t rows,
+             const cplx<T>* tw, int transform, T param, int batch)
+{
+  constexpr int L = dct_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr size_t lds = dct_lds<T, M>();
+  if (lds > c.lds_budget)
+    return false;
+  dim3 grid((unsigned)((rows + L - 1) / L), batch);
+  kdf_x<T, M, L, NT><<<grid, NT, lds, c.stream>>>(g, Hh, out, rows, tw, transform, param);
+  return true;
+}
+
+template <typename T>
+bool fast_dct_x(const FastCtx& c, int M, const DctGeom& g, const T* Hh, T* out, int64_t rows,
+                const cplx<T>* tw, int transform, T param, int batch)
+{
+  switch (M) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return dct_x_n<T, n>(c, g, Hh, out, rows, tw, transform, param, batch);
+    PF_DCT_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
 #define PF_INSTANTIATE(T)                                                                          \
   template bool fast_gen_last<T>(const FastCtx&, int, const GenGeom&, const LastAxis&, const T*,   \
                                  const double*, uint64_t, uint64_t, cplx<T>*, const cplx<T>*,      \
@@ -197,6 +316,12 @@ bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const Las
                             const cplx<T>*, int);                                                  \
   template bool fast_fused<T>(const FastCtx&, int, cplx<T>*, const GenGeom&, const LastAxis&,      \
                               const double*, int, double, const cplx<T>*, int, const double*,      \
-                              int);
+                              int);                                                                \
+  template bool fast_dct_gen_last<T>(const FastCtx&, int, const DctGeom&, const T*, const double*, \
+                                     uint64_t, uint64_t, T*, const cplx<T>*, int);                 \
+  template bool fast_dct_mid<T>(const FastCtx&, int, const DctGeom&, const T*, T*,                 \
+                                const cplx<T>*, int);                                              \
+  template bool fast_dct_x<T>(const FastCtx&, int, const DctGeom&, const T*, T*, int64_t,          \
+                              const cplx<T>*, int, T, int);
 
 } // namespace pfamd

@@ -9,6 +9,7 @@
 namespace pfamd {
 
 struct LastAxis;
+struct DctGeom;
 
 struct FastCtx
 {
@@ -34,5 +35,18 @@ template <typename T>
 bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const LastAxis& a,
                 const double* eig, int op, double inv_next, const cplx<T>* tw, int batch,
                 const double* spec, int nbins);
+
+// DCT/DST field passes (line half-length M along the pass axis); each
+// returns false when M has no specialisation or its tile does not fit.
+template <typename T>
+bool fast_dct_gen_last(const FastCtx& c, int M, const DctGeom& g, const T* sq,
+                       const double* noise, uint64_t seed, uint64_t first, T* G,
+                       const cplx<T>* tw, int batch);
+template <typename T>
+bool fast_dct_mid(const FastCtx& c, int M, const DctGeom& g, const T* G, T* Hh,
+                  const cplx<T>* tw, int batch);
+template <typename T>
+bool fast_dct_x(const FastCtx& c, int M, const DctGeom& g, const T* Hh, T* out, int64_t rows,
+                const cplx<T>* tw, int transform, T param, int batch);
 
 } // namespace pfamd
