@@ -537,3 +537,31 @@ def test_uneven_batch_branches_and_autotune():
     assert best in (1, 2, 4, 8, 16) and ms > 0
     out, _ = b.generate(seed=9, first=3, use_graph=True)
     assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("precision,matrix", [("fp32", "dct"), ("fp64", "r2c")])
+def test_philox_fields_mean_and_covariance(precision, matrix):
+    """Throughput mode statistics: 40000 device-Philox realizations of a small
+    2D field have the generator's mean (0) and covariance A·Aᵀ, where A is the
+    oracle's noise→field map (its columns are the fields of unit noise
+    vectors; it carries the reference's kx-plane variance deficit). Every
+    entry must sit within 6 standard errors of the sample covariance of a
+    Gaussian vector, Var(S_ij) = (C_ii C_jj + C_ij^2) / B."""
+    d = desc(2, [10, 8], covariance="exponential", corr_length=0.3, variance=1.5,
+             matrix=matrix, precision=precision)
+    d64 = desc(2, [10, 8], covariance="exponential", corr_length=0.3, variance=1.5,
+               matrix=matrix)
+    npf = int(orc.sizes(d64).noise_per_field)
+    A = np.stack([orc.generate(d64, np.eye(npf)[i]) for i in range(npf)], axis=1)
+    C = A @ A.T
+    p = pfb.Plan(d)
+    B = 40000
+    f = p.generate(B, seed=2024, first=0).astype(np.float64)
+    mean = f.mean(axis=0)
+    S = np.cov(f, rowvar=False, bias=True) + np.outer(mean, mean)  # E[x xᵀ] about the true mean 0
+    sd = np.sqrt(np.diag(C))
+    assert np.all(np.abs(mean) < 6 * sd / np.sqrt(B))
+    se = np.sqrt((np.outer(np.diag(C), np.diag(C)) + C ** 2) / B)
+    assert np.all(np.abs(S - C) < 6 * se)
+    # the deficit is real: the model covariance differs from A·Aᵀ on the diagonal
+    assert abs(np.diag(C).mean() - 1.5) > 6 * np.sqrt(2 * 1.5 ** 2 / (B * len(sd)))
