@@ -731,7 +731,9 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
     HIPCHECK(hipEventRecord(ev, sa));
     HIPCHECK(hipStreamWaitEvent(sb, ev, 0));
   }
-  FastCtx fb{ sb, fast_budget() };
+  // PF_TILE_DIV=2: half-width tiles for the y and x passes of the generation
+  const char* td = getenv("PF_TILE_DIV");
+  FastCtx fb{ sb, fast_budget(), (td && td[0] == '2') ? 2 : 1 };
   const cplx<T>* xin = B1;
   if (dim == 3) {
     Scope sc(&P, "gen_mid", sb);

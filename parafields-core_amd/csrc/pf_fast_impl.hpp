@@ -75,11 +75,11 @@ bool fast_gen_last(const FastCtx& c, int N, const GenGeom& g, const LastAxis& a,
   }
 }
 
-template <typename T, int N>
-bool mid_n(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, int N0h, int nz,
-           int in_rows, int n_in, int out_rows, int n_out, const cplx<T>* tw, int batch)
+template <typename T, int N, int L>
+bool mid_nl(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, int N0h, int nz,
+            int in_rows, int n_in, int out_rows, int n_out, const cplx<T>* tw, int batch)
 {
-  constexpr int L = fast_L<T>(N), NT = fast_NT<T>(N, L);
+  constexpr int NT = fast_NT<T>(N, L);
   const size_t lds = (size_t)L * N * sizeof(cplx<T>);
   if (N0h % L != 0 || lds > c.lds_budget)
     return false;
@@ -91,6 +91,22 @@ bool mid_n(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, i
     kf_mid<T, N, L, NT, false><<<grid, NT, lds, c.stream>>>(in, out, H, N0h, nz, in_rows, n_in,
                                                             out_rows, n_out, tw);
   return true;
+}
+
+// Half-width tiles (L/2 lines, half the LDS) let a CU hold more blocks next to
+// the generation pass; the two tile halves of a 128-byte row segment run on
+// one XCD (neighbouring kx tiles, x-fastest block order).
+template <typename T, int N>
+bool mid_n(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, int N0h, int nz,
+           int in_rows, int n_in, int out_rows, int n_out, const cplx<T>* tw, int batch)
+{
+  constexpr int L = fast_L<T>(N);
+  if constexpr (L >= 8) {
+    if (c.ldiv == 2)
+      return mid_nl<T, N, L / 2>(c, inv, in, out, H, N0h, nz, in_rows, n_in, out_rows, n_out, tw,
+                                 batch);
+  }
+  return mid_nl<T, N, L>(c, inv, in, out, H, N0h, nz, in_rows, n_in, out_rows, n_out, tw, batch);
 }
 
 template <typename T>
@@ -109,17 +125,29 @@ bool fast_mid(const FastCtx& c, int N, bool inv, const cplx<T>* in, cplx<T>* out
   }
 }
 
-template <typename T, int M>
-bool c2r_n(const FastCtx& c, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
-           const cplx<T>* tw, int transform, T param, int batch)
+template <typename T, int M, int L>
+bool c2r_nl(const FastCtx& c, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
+            const cplx<T>* tw, int transform, T param, int batch)
 {
-  constexpr int L = fast_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr int NT = fast_NT<T>(M, L);
   const size_t lds = (size_t)(L + 1) * M * sizeof(cplx<T>);
   if (lds > c.lds_budget)
     return false;
   dim3 grid((unsigned)((rows + L - 1) / L), batch);
   kf_c2r_x<T, T, M, L, NT><<<grid, NT, lds, c.stream>>>(in, out, rows, H, n0, tw, transform, param);
   return true;
+}
+
+template <typename T, int M>
+bool c2r_n(const FastCtx& c, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
+           const cplx<T>* tw, int transform, T param, int batch)
+{
+  constexpr int L = fast_L<T>(M);
+  if constexpr (L >= 8) {
+    if (c.ldiv == 2)
+      return c2r_nl<T, M, L / 2>(c, in, out, rows, H, n0, tw, transform, param, batch);
+  }
+  return c2r_nl<T, M, L>(c, in, out, rows, H, n0, tw, transform, param, batch);
 }
 
 template <typename T>

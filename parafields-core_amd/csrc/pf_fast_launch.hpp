@@ -15,6 +15,7 @@ struct FastCtx
 {
   hipStream_t stream;
   size_t lds_budget;
+  int ldiv = 1; // y and x passes: tiles of L/ldiv lines (1 or 2)
 };
 
 template <typename T>

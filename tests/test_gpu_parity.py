@@ -291,12 +291,15 @@ FASTPATH_CASES = [
 
 
 @pytest.mark.parametrize("dim,cells,kw", FASTPATH_CASES)
-@pytest.mark.parametrize("path", ["fast", "generic"])
+@pytest.mark.parametrize("path", ["fast", "fast_half_tiles", "generic"])
 def test_specialised_and_generic_kernels(dim, cells, kw, path, monkeypatch):
-    """The compile-time specialised kernels and the generic kernels both match
-    the oracle (PF_NO_FAST=1 routes every pass through the generic ones)."""
+    """The compile-time specialised kernels (also with half-width tiles in the
+    y and x passes, PF_TILE_DIV=2) and the generic kernels all match the oracle
+    (PF_NO_FAST=1 routes every pass through the generic ones)."""
     if path == "generic":
         monkeypatch.setenv("PF_NO_FAST", "1")
+    if path == "fast_half_tiles":
+        monkeypatch.setenv("PF_TILE_DIV", "2")
     d = desc(dim, cells, **kw)
     p = pfb.Plan(d)
     noise = orc.noise_cpp(5, int(p.sizes.noise_per_field))
