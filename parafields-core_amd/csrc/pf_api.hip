@@ -529,6 +529,79 @@ pf_eig_stats reduce_partials(Plan& P, int nb)
   return s;
 }
 
+// R2C matrix, 2D, full real-part check (r2c_check_all): covariance on the
+// padded grid, compile-time r2c along x and forward DFT along y (in place,
+// row pitch Hp), then one fused pass for scaling, classification, the real
+// part and the square-root table. Returns false (nothing changed but the
+// scratch contents) when a line length has no specialised kernel.
+bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial* dev_stats,
+                             const CovParams& cp, double inv_next, int nbp)
+{
+  const Geo& g = P.g;
+  const pf_desc& d = P.d;
+  const int64_t H = g.H, N0 = g.N[0], N1 = g.N[1], rows = N1;
+  const int64_t count = H * rows;
+  P.scratch.ensure((size_t)N0 * rows * sizeof(double) +
+                   (size_t)g.Hp * rows * sizeof(cplx<double>));
+  double* real = P.scratch.as<double>();
+  cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + N0 * rows);
+  if (custom_host) {
+    std::vector<double> compact(N0 * rows);
+    for (int64_t r = 0; r < rows; r++)
+      std::memcpy(&compact[r * N0], custom_host + r * 2 * H, N0 * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(real, compact.data(), compact.size() * sizeof(double),
+                            hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+  } else {
+    Scope sc(&P, "eig_cov");
+    k_eval_cov_full<<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+    check_launch();
+  }
+  const FastCtx fc{ P.stream, fast_budget() };
+  {
+    Scope sc(&P, "eig_r2c_x");
+    if (!fast_r2c<double>(fc, (int)(N0 / 2), real, spec, rows, g.Hp, (int)N0,
+                          P.table<double>((int)N0), 1))
+      return false;
+    check_launch();
+  }
+  {
+    Scope sc(&P, "eig_c2c");
+    if (!fast_mid<double>(fc, (int)N1, false, spec, spec, g.Hp, (int)(N0 / 2), 1, (int)N1,
+                          (int)N1, (int)N1, (int)N1, P.table<double>((int)N1), 1))
+      return false;
+    check_launch();
+  }
+  P.eig.ensure(count * sizeof(double));
+  const int64_t ne = g.eig_entries;
+  const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
+  {
+    Scope sc(&P, "eig_classify");
+    if (d.precision == PF_FP32) {
+      P.sq32.ensure(ne * sizeof(float));
+      k_finish_r2c_eigs<float><<<nb, 256, 0, P.stream>>>(spec, g.Hp, (int)H, rows, inv_next,
+                                                         d.threshold, P.eig.as<double>(),
+                                                         P.sq32.as<float>(),
+                                                         P.partial.as<ClassifyPartial>());
+    } else {
+      P.sq64.ensure(ne * sizeof(double));
+      k_finish_r2c_eigs<double><<<nb, 256, 0, P.stream>>>(spec, g.Hp, (int)H, rows, inv_next,
+                                                          d.threshold, P.eig.as<double>(),
+                                                          P.sq64.as<double>(),
+                                                          P.partial.as<ClassifyPartial>());
+    }
+    check_launch();
+  }
+  if (dev_stats)
+    k_reduce_partials<<<1, 64, 0, P.stream>>>(P.partial.as<ClassifyPartial>(), nb, dev_stats);
+  else {
+    P.st = reduce_partials(P, (int)nb);
+    P.st.entries = count;
+    P.scratch.release();
+  }
+  return true;
+}
+
 // dev_stats != nullptr: capture-safe mode for batches (no host syncs, no
 // frees); the classification lands in *dev_stats and the caller finalises.
 void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats = nullptr)
@@ -581,6 +654,16 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     else
       P.st = reduce_partials(P, nb);
     P.st.entries = total;
+  } else if (g.dim == 2 && d.r2c_check_all && fast_budget() > 0 &&
+             compute_eigs_r2c2d_fast(P, custom_host, dev_stats, cp, inv_next, nbp)) {
+    // eigenvalues, classification and square roots done by the specialised
+    // 2D path (full real-part check)
+    if (dev_stats)
+      return;
+    P.sync();
+    P.eig_valid = true;
+    P.eig_status = (P.st.negative > 0 && !d.approximate) ? PF_ENEG : PF_OK;
+    return;
   } else {
     const int64_t H = g.H, N0 = g.N[0], rows = (int64_t)g.N[1] * g.N[2];
     const int64_t count = H * rows;

@@ -21,6 +21,17 @@ constexpr int fast_L(int N)
   return L;
 }
 
+// Lines per tile of the x passes, whose LDS tile is padded to L + 1 lines:
+// the largest power of two <= fast_L with (L + 1) * M complex values in 64 KB.
+template <typename T>
+constexpr int pad_L(int M)
+{
+  int L = fast_L<T>(M);
+  while (L > 1 && (L + 1) * M * 2 * (int)sizeof(T) > 65536)
+    L /= 2;
+  return L;
+}
+
 template <typename T>
 constexpr int fast_NT(int N, int L)
 {
@@ -142,7 +153,7 @@ template <typename T, int M>
 bool c2r_n(const FastCtx& c, const cplx<T>* in, T* out, int64_t rows, int H, int n0,
            const cplx<T>* tw, int transform, T param, int batch)
 {
-  constexpr int L = fast_L<T>(M);
+  constexpr int L = pad_L<T>(M);
   if constexpr (L >= 8) {
     if (c.ldiv == 2)
       return c2r_nl<T, M, L / 2>(c, in, out, rows, H, n0, tw, transform, param, batch);
@@ -169,7 +180,7 @@ template <typename T, int M>
 bool r2c_n(const FastCtx& c, const T* in, cplx<T>* out, int64_t rows, int H, int n0,
            const cplx<T>* tw, int batch)
 {
-  constexpr int L = fast_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr int L = pad_L<T>(M), NT = fast_NT<T>(M, L);
   const size_t lds = (size_t)(L + 1) * M * sizeof(cplx<T>);
   if (lds > c.lds_budget)
     return false;

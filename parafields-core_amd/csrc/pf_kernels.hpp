@@ -272,6 +272,29 @@ __global__ void k_classify_r2c(cplx<double>* spec, int64_t count, int H, int N1,
   block_reduce_write(c, partial);
 }
 
+// R2C backend, full real-part check, spectrum with row pitch Hp: in one pass
+// scale by 1/N_ext, classify and clamp the real part, write the eigenvalue
+// (layout [rows][H]) and its square root in the field precision (the work of
+// k_scale_complex, k_classify_r2c(check_all), k_real_part and k_sqrt_table).
+template <typename T>
+__global__ void k_finish_r2c_eigs(const cplx<double>* spec, int Hp, int H, int64_t rows,
+                                  double inv_next, double thr, double* eig, T* sq,
+                                  ClassifyPartial* partial)
+{
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  const int64_t count = (int64_t)H * rows;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / H;
+    const int kx = (int)(i - r * H);
+    double v = spec[r * Hp + kx].x * inv_next;
+    classify_one(v, thr, c);
+    eig[i] = v;
+    sq[i] = (T)sqrt(v);
+  }
+  block_reduce_write(c, partial);
+}
+
 // Sum `n` block partials into one record on the device (capture-safe).
 __global__ void k_reduce_partials(const ClassifyPartial* partial, int n, ClassifyPartial* out)
 {

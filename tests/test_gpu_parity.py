@@ -82,6 +82,33 @@ def test_eigs_match_oracle(dim, cells, kw):
     assert st["smallest"] == pytest.approx(wst["smallest"], rel=1e-9, abs=1e-15)
 
 
+R2C_CHECK_ALL_CASES = [
+    (2, [48, 96], dict(covariance="gaussian", anisotropy="geometric",
+                       corr_length=[0.08, 0.02, -0.01, 0.03], approximate=True)),
+    (2, [1024, 64], dict(covariance="exponential", corr_length=0.05)),
+]
+
+
+@pytest.mark.parametrize("path", ["fast", "generic"])
+@pytest.mark.parametrize("dim,cells,kw", R2C_CHECK_ALL_CASES)
+def test_eigs_r2c_check_all_match_oracle(dim, cells, kw, path, monkeypatch):
+    """R2C matrix with the full real-part check (the uneven batch's setting):
+    the specialised 2D path (FP64 r2c along x, forward DFT along y, one fused
+    classify/square-root pass) and the generic path both match the oracle."""
+    if path == "generic":
+        monkeypatch.setenv("PF_NO_FAST", "1")
+    d = desc(dim, cells, matrix="r2c", r2c_check_all=True, **kw)
+    p = pfb.Plan(d)
+    status, st = p.eigs_stats_nothrow()
+    want, wst, wstatus = orc.eigs(d, allow_negative=True)
+    assert status == wstatus
+    assert rel_max(p.eig_values(), want) < FP64_TOL
+    for k in ("small", "zero", "negative", "entries"):
+        assert st[k] == wst[k], (k, st, wst)
+    noise = orc.noise_cpp(3, int(p.sizes.noise_per_field))
+    assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < FP64_TOL
+
+
 def test_negative_eigenvalue_error():
     d = desc(2, [16, 16], covariance="gaussian", corr_length=0.5)
     p = pfb.Plan(d)
@@ -287,6 +314,9 @@ FASTPATH_CASES = [
     (3, [64, 32, 24], dict(covariance="exponential", corr_length=0.1, transposed=0)),
     (2, [96, 192], dict(covariance="matern52", anisotropy="geometric",
                         corr_length=[0.1, 0.03, -0.02, 0.05], approximate=True)),
+    # 1024-point x pass (padded tiles of 4 FP32 / 2 FP64 lines) and 512-point
+    (2, [1024, 48], dict(covariance="exponential", corr_length=0.05)),
+    (2, [512, 96], dict(covariance="matern32", corr_length=0.1, transposed=0)),
 ]
 
 
