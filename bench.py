@@ -160,10 +160,12 @@ def run_gpu_fp64_single(reps):
 
 
 def field_error(got, want):
-    """max |got - want| / std(want) and ||got - want|| / ||want|| (FP64)."""
+    """max |got - want| relative to std(want) and to max |want|, and
+    ||got - want|| / ||want|| (in FP64)."""
     g = np.asarray(got, dtype=np.float64)
     w = np.asarray(want, dtype=np.float64)
-    return {"max_abs_over_std": float(np.abs(g - w).max() / w.std()),
+    e = np.abs(g - w).max()
+    return {"max_abs_over_std": float(e / w.std()), "max_abs_over_max": float(e / np.abs(w).max()),
             "rel_l2": float(np.linalg.norm(g - w) / np.linalg.norm(w))}
 
 
@@ -239,6 +241,7 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     b.close()
     del out
     torch.cuda.empty_cache()
+    print("bench: uneven batch done, per-size classes next", file=sys.stderr, flush=True)
     # per-size throughput: the fields of each size class (larger side) as a
     # batch of their own, autotuned and replayed as one graph
     per_size = {}
@@ -258,6 +261,7 @@ def run_uneven(nfields=1000, seed=3, reps=3):
         el_s = (time.perf_counter() - t0) / reps
         per_size[str(side)] = {"fields": len(idx), "fields_per_s": round(len(idx) / el_s, 1),
                                "ms": round(el_s * 1e3, 3), "branches": k}
+        print(f"bench: size class {side} ({len(idx)} fields) done", file=sys.stderr, flush=True)
         bs.close()
         del o
     torch.cuda.empty_cache()
@@ -445,7 +449,8 @@ def main():
         "accuracy_vs_fp64_oracle": None if accuracy is None else dict(
             accuracy, realization=0,
             note="Gaussian field (before the log-normal transform) against the FP64 CPU oracle "
-                 "on identical Philox noise; bar: max_abs_over_std <= 2e-5 (FP32), 1e-11 (FP64)"),
+                 "on identical Philox noise; bars: FP32 max_abs_over_std <= 2e-5, FP64 "
+                 "max_abs_over_max <= 1e-11"),
         "other_configs": extra,
     }
     print(json.dumps(line), flush=True)

@@ -279,8 +279,9 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
 
 /* Batch packing: the fields are bin-packed by algorithmic bytes onto
  * `branches` concurrent graph branches (streams forked from and joined to the
- * batch stream). pf_batch_autotune times one replay for 1, 2, 4, 8 and 16
- * branches on the real batch and keeps the fastest (best, best_ms out). */
+ * batch stream; never more branches than fields). pf_batch_autotune times
+ * one replay for 1, 2, 4, 8 and 16 branches (up to the number of fields) on
+ * the real batch and keeps the fastest (best, best_ms out). */
 int pf_batch_set_branches(void* batch, int32_t branches);
 int pf_batch_autotune(void* batch, uint64_t seed, uint64_t first_realization, int32_t transform,
                       double transform_param, void* out, int32_t out_mem, int32_t* best,

@@ -1468,6 +1468,9 @@ struct Batch
   void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out, int k)
   {
     const size_t n = plans.size();
+    // never fork a branch that would receive no field: an empty forked
+    // branch joins the capture without any node of its own
+    k = std::min<int>(k, (int)n);
     if (k <= 1) {
       for (size_t i = 0; i < n; i++) {
         plans[i]->stream = stream;
@@ -2132,7 +2135,7 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
       B->stage.ensure(outb);
       dout = B->stage.p;
     }
-    const int k = std::max(1, B->branches);
+    const int k = std::max(1, std::min<int>(B->branches, (int)B->plans.size()));
     if (!B->prepared) {
       // eager pass: allocates every buffer and twiddle table the capture needs
       B->enqueue(seed, first_realization, transform, transform_param, dout, 1);
@@ -2219,6 +2222,8 @@ int pf_batch_autotune(void* batch, uint64_t seed, uint64_t first_realization, in
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     for (int k : { 1, 2, 4, 8, 16 }) {
+      if (k > 1 && k > (int)B->plans.size())
+        break; // more branches than fields: nothing left to try
       B->branches = k;
       int st = pf_batch_generate(batch, seed, first_realization, transform, transform_param, out,
                                  out_mem, 1, nullptr); // builds the graph for k

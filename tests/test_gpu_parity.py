@@ -105,8 +105,13 @@ def test_eigs_r2c_check_all_match_oracle(dim, cells, kw, path, monkeypatch):
     assert rel_max(p.eig_values(), want) < FP64_TOL
     for k in ("small", "zero", "negative", "entries"):
         assert st[k] == wst[k], (k, st, wst)
+    # fields: FP64 bar, except where eigenvalues sit at the rounding level of
+    # the transform (the Gaussian case has thousands at |lambda| ~ 1e-17 of
+    # lambda_max); their square roots carry FFT rounding differences of
+    # sqrt(1e-17) ~ 3e-9 into the field
+    tol = 1e-7 if kw["covariance"] == "gaussian" else FP64_TOL
     noise = orc.noise_cpp(3, int(p.sizes.noise_per_field))
-    assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < FP64_TOL
+    assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < tol
 
 
 def test_negative_eigenvalue_error():
@@ -569,6 +574,21 @@ def test_uneven_batch_branches_and_autotune():
     best, ms = b.autotune(seed=9, first=3)
     assert best in (1, 2, 4, 8, 16) and ms > 0
     out, _ = b.generate(seed=9, first=3, use_graph=True)
+    assert np.array_equal(out, ref)
+
+
+def test_uneven_batch_more_branches_than_fields():
+    """A batch of 3 fields asked for 16 branches runs on 3 (no empty branch)
+    and keeps the bits of the serial graph; the autotuner stops at 3."""
+    descs = _uneven_descs(3, seed=8)
+    b = pfb.Batch(descs)
+    ref, _ = b.generate(seed=4, first=0, use_graph=True)
+    b.set_branches(16)
+    out, _ = b.generate(seed=4, first=0, use_graph=True)
+    assert np.array_equal(out, ref)
+    best, _ = b.autotune(seed=4, first=0)
+    assert best in (1, 2)
+    out, _ = b.generate(seed=4, first=0, use_graph=True)
     assert np.array_equal(out, ref)
 
 
