@@ -206,10 +206,9 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     import torch
     rng = np.random.default_rng(seed)
     sides = [64, 96, 128, 192, 256, 384, 512, 768, 1024]
-    descs, alg, shapes = [], 0, []
+    descs, alg = [], 0
     for _ in range(nfields):
         n0, n1 = (int(rng.choice(sides)) for _ in range(2))
-        shapes.append((n0, n1))
         th, r, L = rng.uniform(0, np.pi), rng.uniform(1, 8), rng.uniform(0.02, 0.2)
         R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
         A = R @ np.diag([L, L / r])
@@ -241,38 +240,13 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     b.close()
     del out
     torch.cuda.empty_cache()
-    print("bench: uneven batch done, per-size classes next", file=sys.stderr, flush=True)
-    # per-size throughput: the fields of each size class (larger side) as a
-    # batch of their own, autotuned and replayed as one graph
-    per_size = {}
-    for side in sides:
-        idx = [i for i, (n0, n1) in enumerate(shapes) if max(n0, n1) == side]
-        if not idx:
-            continue
-        bs = pfb.Batch([descs[i] for i in idx])
-        o = torch.empty(int(bs.offsets[-1]), dtype=torch.float32, device="cuda")
-        bs.generate(seed=11, out=o, use_graph=True)
-        k, _ = bs.autotune(seed=11, out=o)
-        bs.generate(seed=11, out=o, use_graph=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            bs.generate(seed=11, out=o, use_graph=True)
-        el_s = (time.perf_counter() - t0) / reps
-        per_size[str(side)] = {"fields": len(idx), "fields_per_s": round(len(idx) / el_s, 1),
-                               "ms": round(el_s * 1e3, 3), "branches": k}
-        print(f"bench: size class {side} ({len(idx)} fields) done", file=sys.stderr, flush=True)
-        bs.close()
-        del o
     torch.cuda.empty_cache()
     return {"fields": nfields, "fields_per_s": round(nfields / el, 1),
             "ms_per_batch": round(el * 1e3, 2), "eager_ms_per_batch": round(eager * 1e3, 2),
             "serial_graph_ms_per_batch": round(serial * 1e3, 2), "branches": best,
             "alg_GBps": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / HBM_PEAK, 4),
-            "setup_s": round(setup, 2), "per_size_class": per_size,
-            "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch; "
-                    "per_size_class: the fields whose larger side is the key, as a batch of "
-                    "their own"}
+            "setup_s": round(setup, 2),
+            "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch"}
 
 
 def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):

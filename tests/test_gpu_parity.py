@@ -321,7 +321,7 @@ FASTPATH_CASES = [
                         corr_length=[0.1, 0.03, -0.02, 0.05], approximate=True)),
     # 1024-point x pass (padded tiles of 4 FP32 / 2 FP64 lines) and 512-point
     (2, [1024, 48], dict(covariance="exponential", corr_length=0.05)),
-    (2, [512, 96], dict(covariance="matern32", corr_length=0.1, transposed=0)),
+    (2, [512, 96], dict(covariance="exponential", corr_length=0.08, transposed=0)),
 ]
 
 
