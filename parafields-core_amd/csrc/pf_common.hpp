@@ -12,13 +12,19 @@ namespace pfamd {
 // logical block (bx, by, bz) so that the blocks one XCD receives form one
 // contiguous run of logical indices: tiles that share cache lines (the
 // neighbouring kx tiles of a row) then meet in the same L2.
+// Logical index of linear block b of n (a bijection on [0, n)).
+__device__ __forceinline__ unsigned xcd_linear(unsigned b, unsigned n)
+{
+  const unsigned x = b & 7u, q = n >> 3, r = n & 7u;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 {
   const unsigned gx = gridDim.x, gy = gridDim.y;
   const unsigned n = gx * gy * gridDim.z;
   const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  const unsigned x = b & 7u, q = n >> 3, r = n & 7u;
-  const unsigned lam = x * q + (x < r ? x : r) + (b >> 3);
+  const unsigned lam = xcd_linear(b, n);
   bx = (int)(lam % gx);
   const unsigned t = lam / gx;
   by = (int)(t % gy);

@@ -104,21 +104,25 @@ struct BoxMuller<double>
 template <>
 struct BoxMuller<float>
 {
+  // u in (0, 1]: float of the 32-bit integer scaled by 2^-32 plus half an ulp
+  static __device__ __forceinline__ float turn(uint32_t x)
+  {
+    return fmaf((float)x, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+  }
+  // sqrt(-2 ln u): ln(u) = log2(u) * ln 2 with the hardware log2; the
+  // argument of the root is in [0, 44.4] (never denormal), so the hardware
+  // square root (1 ulp) needs none of the IEEE wrapper's scaling steps
+  static __device__ __forceinline__ float radius(uint32_t xa)
+  {
+    return __builtin_amdgcn_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(turn(xa)));
+  }
   static __device__ __forceinline__ void pair(uint32_t xa, uint32_t xb, float& na, float& nb)
   {
-    // u in (0, 1]: float of the 32-bit integer scaled by 2^-32 plus half an ulp
-    const float ua = fmaf((float)xa, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
-    const float ub = fmaf((float)xb, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
-    // ln(u) = log2(u) * ln 2 with the hardware log2; the argument of the
-    // root is in [0, 44.4] (never denormal), so the hardware square root
-    // (1 ulp) needs none of the IEEE wrapper's scaling steps
-    const float r =
-      __builtin_amdgcn_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(ua));
+    const float r = radius(xa);
     // hardware sin/cos take their argument in revolutions: sin(2 pi ub)
-    const float s = __builtin_amdgcn_sinf(ub);
-    const float c = __builtin_amdgcn_cosf(ub);
-    na = r * c;
-    nb = r * s;
+    const float ub = turn(xb);
+    na = r * __builtin_amdgcn_cosf(ub);
+    nb = r * __builtin_amdgcn_sinf(ub);
   }
 };
 
@@ -339,6 +343,22 @@ __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
 
 // Value transforms of mutators.hh:35-163 (sign threshold is a parameter,
 // 0 in the reference).
+// value_transform with the kind fixed at compile time
+template <typename T, int K>
+__device__ __forceinline__ T value_transform_k(T v, T param)
+{
+  if constexpr (K == PF_TRANSFORM_LOGNORMAL)
+    return exp(v);
+  else if constexpr (K == PF_TRANSFORM_FOLDEDNORMAL)
+    return fabs(v);
+  else if constexpr (K == PF_TRANSFORM_SIGN)
+    return (v < param) ? T(-1) : T(1);
+  else if constexpr (K == PF_TRANSFORM_BOXCOX)
+    return pow(v * param + T(1), -param);
+  else
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T value_transform(int kind, T v, T param)
 {

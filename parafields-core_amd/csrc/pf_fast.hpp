@@ -191,6 +191,30 @@ __device__ __forceinline__ cplx<T> make_entry(T lambda, T r1, T r2, bool selfcon
   return { ls * r1, ls * r2 };
 }
 
+// Spectrum entry of one Box-Muller pair: make_entry(lambda, normals of
+// (xa, xb)). FP32 folds the 1/sqrt(2) of the entries off the self-conjugate
+// set into the root's argument and lambda into the radius, ahead of the
+// cosine / sine products.
+template <typename T>
+__device__ __forceinline__ cplx<T> gen_entry(uint32_t xa, uint32_t xb, T lambda, bool selfconj)
+{
+  T r1, r2;
+  BoxMuller<T>::pair(xa, xb, r1, r2);
+  return make_entry<T>(lambda, r1, r2, selfconj);
+}
+
+template <>
+__device__ __forceinline__ cplx<float> gen_entry<float>(uint32_t xa, uint32_t xb, float lambda,
+                                                        bool selfconj)
+{
+  // lambda * sqrt(-2 ln u) / sqrt(2) = lambda * sqrt(-ln u); ln u = log2(u) ln 2
+  const float k = selfconj ? -2.0f * 0.69314718055994531f : -0.69314718055994531f;
+  const float t =
+    lambda * __builtin_amdgcn_sqrtf(k * __builtin_amdgcn_logf(BoxMuller<float>::turn(xa)));
+  const float ub = BoxMuller<float>::turn(xb);
+  return { t * __builtin_amdgcn_cosf(ub), selfconj ? 0.0f : t * __builtin_amdgcn_sinf(ub) };
+}
+
 // logical (k0, k1, k2) of line (kx, km) element e
 __device__ __forceinline__ void line_k(const GenGeom& g, int kx, int km, int e, int& k1, int& k2)
 {
@@ -207,17 +231,17 @@ __device__ __forceinline__ void line_k(const GenGeom& g, int kx, int km, int e, 
 // inverse DFT, first nlast outputs -> B1[j][km][kx].
 // IDX64: storage indices and eigenvalue offsets in 64 bits (fields with 2^31
 // or more half-spectrum entries); otherwise the slot walk runs in 32 bits.
+// Body of one tile (bx = kx tile or the Nyquist column, by = k_mid row or
+// group, b = realization of the launch).
 template <typename T, int N, int L, int NT, int NOISE, bool IDX64>
-__global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T* sq,
-                                                  const double* noise, uint64_t seed,
-                                                  uint64_t first_real, cplx<T>* B1,
-                                                  const cplx<T>* tw)
+__device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& a, const T* sq,
+                                              const double* noise, uint64_t seed,
+                                              uint64_t first_real, cplx<T>* B1,
+                                              const cplx<T>* tw, int bx, int by, int b)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
   const int ntile = a.N0h / L;
-  int bx, by, b;
-  xcd_block(bx, by, b);
   const bool nyq = bx >= ntile;
   int kx0, km0;
   if (!nyq) {
@@ -234,17 +258,21 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
   const uint64_t real = first_real + b;
   const double* nz = noise ? noise + (int64_t)b * g.noise_stride : nullptr;
   if (NOISE == 1 && !nyq) {
-    // Philox pairs: entries s and s+1 (s even) share one block; with the row
-    // base parity odd the pairs start one entry to the left of the tile.
-    // Per block: k_mid is fixed, so the storage index and the eigenvalue row
-    // are affine in e; kx < N0/2 needs no folding.
-    // Slots (p, e) are walked incrementally (slot += NT moves e by NT/SLOTS
-    // and p by NT%SLOTS with one carry), so no division or 32-bit multiply
-    // is left in the loop; the two root-eigenvalue loads of a slot are issued
-    // before its Philox rounds, which hide their latency.
-    constexpr int SLOTS = L / 2 + 1;
-    constexpr int ITER = (SLOTS * N + NT - 1) / NT;
-    constexpr int DE = NT / SLOTS, DP = NT % SLOTS;
+    // One Philox block yields the entries 2m and 2m+1 of the storage order.
+    // Row e of the tile holds entries base(e) + l, l < L (k_mid is fixed,
+    // so the storage index and the eigenvalue row are affine in e, and
+    // kx < N0/2 needs no folding). An even row base makes the row L/2 whole
+    // blocks; an odd one makes it L/2 - 1 whole blocks plus a block whose
+    // second entry is column 0 and one whose first entry is column L-1.
+    // Main pass: thread t owns block slot q = t % S (S = L/2) of the rows
+    // e = t/S + it*ES. ES is even, so a thread's rows share one parity and
+    // its two columns 2q - odd and 2q - odd + 1 (column -1 is dropped).
+    // Tail pass: column L-1 of the odd rows, one block each. Every lane of
+    // the main pass does useful work, and no index test is left in its loop.
+    static_assert(L % 2 == 0, "entries are generated in pairs");
+    constexpr int S = L / 2, ES = NT / S;
+    static_assert(NT % S == 0 && ES % 2 == 0, "a thread's rows share one parity");
+    constexpr int ITM = (N + ES - 1) / ES;
     const bool d3 = g.dim == 3;
     const int Pe = d3 ? g.P2 : g.P1, Ne = d3 ? g.N2 : g.N1;
     int64_t eu, es; // eigen row offset = eu + es * e' (e' = e folded for the DCT half)
@@ -256,50 +284,73 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
       eu = d3 ? (int64_t)g.H * km0 : 0;
       es = d3 ? (int64_t)g.H * g.N1 : g.H;
     }
-    const bool sc1 = d3 ? self_conj(km0, g.N1) : true;
+    // kx = 0 (column 0 of the first tile) is the only self-conjugate column
+    const bool scq = (d3 ? self_conj(km0, g.N1) : true) && kx0 == 0;
     const uint32_t sk0 = (uint32_t)seed, sk1 = (uint32_t)(seed >> 32);
     using IX = typename std::conditional<IDX64, int64_t, uint32_t>::type;
     const IX rowstep = (IX)((int64_t)a.H * a.sA); // storage index step per e
     const IX est = (IX)es;
     const T* sqt = sq + eu + kx0;
-    int p = (int)threadIdx.x % SLOTS, e = (int)threadIdx.x / SLOTS;
-    IX base = (IX)((int64_t)a.H * ((int64_t)km0 * a.sB) + kx0) + rowstep * (IX)e;
-    IX up = est * (IX)e, dn = est * (IX)(Ne - e); // es*e and es*(Ne-e)
+    const IX base0 = (IX)((int64_t)a.H * ((int64_t)km0 * a.sB) + kx0);
+    const int par0 = (int)(base0 & 1), rs = (int)(rowstep & 1);
+    const int tid = (int)threadIdx.x;
+    // the tile of kx = 0 holds self-conjugate entries; every other tile runs
+    // a copy of the main pass without their tests
+    auto main_pass = [&](auto with_sc) {
+      constexpr bool SC = decltype(with_sc)::value;
+      const int q = tid % S;
+      int e = tid / S;
+      const int odd = (par0 + rs * e) & 1;
+      const int la = 2 * q - odd, lb = la + 1, lac = la < 0 ? 0 : la, ia = la < 0 ? lb : la;
+      IX base = base0 + rowstep * (IX)e + (IX)la; // storage index of column la: even
+      IX up = est * (IX)e, dn = est * (IX)(Ne - e); // es*e and es*(Ne-e)
 #pragma unroll
-    for (int it = 0; it < ITER; it++) {
-      const int odd = (int)(base & 1);
-      const int la = 2 * p - odd, lb = la + 1;
-      if (e < N && la < L && lb >= 0) {
-        const T* sqrow = sqt + ((g.dct && e >= Pe) ? dn : up);
-        const T sa = sqrow[la < 0 ? 0 : la];
-        const T sb = sqrow[lb >= L ? L - 1 : lb];
-        const uint64_t blk = (uint64_t)(IX)(base + (IX)la) >> 1;
-        uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
-                          (uint32_t)(real >> 32) };
-        philox4x32_10(c, sk0, sk1);
-        const bool sc0 = sc1 && kx0 == 0 && self_conj(e, Ne); // only kx = 0 is on a plane
-        if (la >= 0) {
-          T r1, r2;
-          BoxMuller<T>::pair(c[0], c[1], r1, r2);
-          buf[e * L + la] = make_entry<T>(sa, r1, r2, sc0 && la == 0);
+      for (int it = 0; it < ITM; it++) {
+        if (N % ES == 0 || e < N) {
+          const T* sqrow = sqt + ((g.dct && e >= Pe) ? dn : up);
+          const T sa = sqrow[lac], sb = sqrow[lb];
+          const uint64_t blk = (uint64_t)base >> 1;
+          uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
+                            (uint32_t)(real >> 32) };
+          philox4x32_10(c, sk0, sk1);
+          const bool sce = SC && self_conj(e, Ne);
+          const cplx<T> va = gen_entry<T>(c[0], c[1], sa, sce && la == 0);
+          const cplx<T> vb = gen_entry<T>(c[2], c[3], sb, sce && lb == 0);
+          // column -1 is dropped by storing it to column lb first: a store
+          // left unconditional keeps va (and its eigenvalue load) out of a
+          // branch, where the load's latency would be waited out at once
+          buf[e * L + ia] = va;
+          buf[e * L + lb] = vb;
         }
-        if (lb < L) {
-          T r1, r2;
-          BoxMuller<T>::pair(c[2], c[3], r1, r2);
-          buf[e * L + lb] = make_entry<T>(sb, r1, r2, sc0 && lb == 0);
-        }
+        e += ES;
+        base += rowstep * (IX)ES;
+        up += est * (IX)ES;
+        dn -= est * (IX)ES;
       }
-      p += DP;
-      e += DE;
-      base += rowstep * (IX)DE;
-      up += est * (IX)DE;
-      dn -= est * (IX)DE;
-      if (p >= SLOTS) {
-        p -= SLOTS;
-        e += 1;
-        base += rowstep;
-        up += est;
-        dn -= est;
+    };
+    if (scq)
+      main_pass(std::true_type{});
+    else
+      main_pass(std::false_type{});
+    {
+      // odd rows: every row when the row step is even and base(0) odd, every
+      // other row from the first odd one when the row step is odd
+      const int e0 = rs ? 1 - par0 : 0, estep = rs ? 2 : 1;
+      const int nodd = rs ? (N - e0 + 1) / 2 : (par0 ? N : 0);
+      constexpr int ITT = (N + NT - 1) / NT;
+#pragma unroll
+      for (int it = 0; it < ITT; it++) {
+        const int i = tid + it * NT;
+        if (i < nodd) {
+          const int e = e0 + estep * i;
+          const T* sqrow = sqt + ((g.dct && e >= Pe) ? est * (IX)(Ne - e) : est * (IX)e);
+          const T sa = sqrow[L - 1];
+          const uint64_t blk = (uint64_t)(IX)(base0 + rowstep * (IX)e + (IX)(L - 1)) >> 1;
+          uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
+                            (uint32_t)(real >> 32) };
+          philox4x32_10(c, sk0, sk1);
+          buf[e * L + L - 1] = gen_entry<T>(c[0], c[1], sa, false);
+        }
       }
     }
   } else {
@@ -348,29 +399,40 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
   }
 }
 
+template <typename T, int N, int L, int NT, int NOISE, bool IDX64>
+__global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T* sq,
+                                                  const double* noise, uint64_t seed,
+                                                  uint64_t first_real, cplx<T>* B1,
+                                                  const cplx<T>* tw)
+{
+  int bx, by, b;
+  xcd_block(bx, by, b);
+  gen_last_tile<T, N, L, NT, NOISE, IDX64>(g, a, sq, noise, seed, first_real, B1, tw, bx, by, b);
+}
+
 // Pass along the middle axis (3D): lines (kx, z) of in[z][e][kx] (in_rows
 // rows per z, n_in leading points, rest zero), DFT, first n_out points ->
 // out[z][j][kx] (out_rows rows per z).
+// Body of tile (bx, by) of realization b.
 template <typename T, int N, int L, int NT, bool INV>
-__global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, int Hp, int N0h,
-                                             int nz, int in_rows, int n_in, int out_rows,
-                                             int n_out, const cplx<T>* tw)
+__device__ __forceinline__ void mid_tile(const cplx<T>* in, cplx<T>* out, int Hp, int N0h, int nz,
+                                         int in_rows, int n_in, int out_rows, int n_out,
+                                         const cplx<T>* tw, int bx, int by, int b)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
   const int ntile = N0h / L;
-  const bool nyq = (int)blockIdx.x >= ntile;
+  const bool nyq = bx >= ntile;
   int kx0, z0;
   if (!nyq) {
-    kx0 = blockIdx.x * L;
-    z0 = blockIdx.y;
+    kx0 = bx * L;
+    z0 = by;
   } else {
     kx0 = N0h;
-    z0 = blockIdx.y * L;
+    z0 = by * L;
     if (z0 >= nz)
       return;
   }
-  const int b = blockIdx.z;
   in += (int64_t)b * Hp * nz * in_rows;
   out += (int64_t)b * Hp * nz * out_rows;
   using TL = TileLoop<L * N, NT>;
@@ -406,31 +468,50 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
   }
 }
 
+template <typename T, int N, int L, int NT, bool INV>
+__global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, int Hp, int N0h,
+                                             int nz, int in_rows, int n_in, int out_rows,
+                                             int n_out, const cplx<T>* tw)
+{
+  mid_tile<T, N, L, NT, INV>(in, out, Hp, N0h, nz, in_rows, n_in, out_rows, n_out, tw, blockIdx.x,
+                             blockIdx.y, blockIdx.z);
+}
+
+// Two adjacent outputs stored as one vector.
+template <typename U>
+struct alignas(2 * sizeof(U)) pair2
+{
+  U x, y;
+};
+
 // c2r along x (see k_c2r_x for the algebra), M = N0/2 point inverse FFT on a
 // padded tile of L rows; first n0 outputs with the value transform.
+// Body of tile bx of realization b.
 template <typename T, typename TO, int M, int L, int NT>
-__global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int Hp,
-                                               int n0, const cplx<T>* tw, int transform, T param)
+__device__ __forceinline__ void c2r_x_tile(const cplx<T>* in, TO* out, int64_t rows, int Hp,
+                                           int n0, const cplx<T>* tw, int transform, T param,
+                                           int bx, int b)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int N0 = 2 * M, SE = L + 1;
+  constexpr int SE = L + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
-  const int b = blockIdx.y;
   in += (int64_t)b * rows * Hp;
   out += (int64_t)b * rows * n0;
-  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int64_t row0 = (int64_t)bx * L;
+  const int lrows = (int)(rows - row0 < L ? rows - row0 : L); // live rows of the tile
+  const cplx<T>* ib = in + row0 * Hp;
   using TL = TileLoop<L * M, NT>;
   cplx<T> xk[TL::IT], xm[TL::IT];
 #pragma unroll
   for (int it = 0; it < TL::IT; it++) {
     const int idx = TL::idx(it);
     const int l = idx / M, k = idx % M;
-    const int64_t r = row0 + l;
     xk[it] = { 0, 0 };
     xm[it] = { 0, 0 };
-    if (TL::live(it) && r < rows) {
-      xk[it] = in[r * Hp + k];
-      xm[it] = in[r * Hp + (M - k)];
+    if (TL::live(it) && l < lrows) {
+      const uint32_t off = (uint32_t)(l * Hp); // 32-bit offsets within the tile
+      xk[it] = ib[off + k];
+      xm[it] = ib[off + (M - k)];
     }
   }
 #pragma unroll
@@ -450,19 +531,71 @@ __global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64
   }
   __syncthreads();
   tfft<T, M, L, NT, 1, true, 2>(buf, tw);
-  using TO_ = TileLoop<L * N0, NT>;
-  // partial unroll: the value transform (a runtime switch, Box-Cox's pow
-  // included) would otherwise be replicated for every element
-#pragma unroll 4
-  for (int it = 0; it < TO_::IT; it++) {
-    const int idx = TO_::idx(it);
-    const int l = idx / N0, x = idx % N0;
-    const int64_t r = row0 + l;
-    if (TO_::live(it) && x < n0 && r < rows) {
-      const cplx<T> v = buf[(x >> 1) * SE + l];
-      out[r * n0 + x] = (TO)value_transform<T>(transform, (x & 1) ? v.y : v.x, param);
+  // outputs 2m and 2m+1 of a row are the real and imaginary parts of entry
+  // m: one LDS read and, for even n0 on an aligned row, one vector store.
+  // Offsets within the tile are 32-bit (L * n0 is far below 2^31), so the
+  // stores address a uniform 64-bit base plus a per-lane offset.
+  TO* ob = out + row0 * n0;
+  const bool vec = n0 % 2 == 0 && (reinterpret_cast<uintptr_t>(out) & (2 * sizeof(TO) - 1)) == 0;
+  auto emit = [&](auto kind) {
+    constexpr int K = decltype(kind)::value;
+    auto one = [&](int it, bool pairs) {
+      const int idx = TL::idx(it);
+      const int l = idx / M, m = idx % M;
+      if (TL::live(it) && l < lrows && 2 * m < n0) {
+        const cplx<T> v = buf[m * SE + l];
+        const TO lo = (TO)value_transform_k<T, K>(v.x, param);
+        const TO hi = (TO)value_transform_k<T, K>(v.y, param);
+        const uint32_t off = (uint32_t)(l * n0 + 2 * m);
+        if (pairs) {
+          *reinterpret_cast<pair2<TO>*>(ob + off) = { lo, hi };
+        } else {
+          ob[off] = lo;
+          if (2 * m + 1 < n0)
+            ob[off + 1] = hi;
+        }
+      }
+    };
+    // Box-Cox's power function is long: its loop stays rolled
+    if constexpr (K == PF_TRANSFORM_BOXCOX) {
+#pragma unroll 1
+      for (int it = 0; it < TL::IT; it++)
+        one(it, vec);
+    } else if (vec) {
+#pragma unroll
+      for (int it = 0; it < TL::IT; it++)
+        one(it, true);
+    } else {
+#pragma unroll
+      for (int it = 0; it < TL::IT; it++)
+        one(it, false);
     }
+  };
+  // the transform is dispatched once per tile, not per output
+  switch (transform) {
+    case PF_TRANSFORM_LOGNORMAL:
+      emit(std::integral_constant<int, PF_TRANSFORM_LOGNORMAL>{});
+      break;
+    case PF_TRANSFORM_FOLDEDNORMAL:
+      emit(std::integral_constant<int, PF_TRANSFORM_FOLDEDNORMAL>{});
+      break;
+    case PF_TRANSFORM_SIGN:
+      emit(std::integral_constant<int, PF_TRANSFORM_SIGN>{});
+      break;
+    case PF_TRANSFORM_BOXCOX:
+      emit(std::integral_constant<int, PF_TRANSFORM_BOXCOX>{});
+      break;
+    default:
+      emit(std::integral_constant<int, PF_TRANSFORM_NONE>{});
   }
+}
+
+template <typename T, typename TO, int M, int L, int NT>
+__global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int Hp,
+                                               int n0, const cplx<T>* tw, int transform, T param)
+{
+  c2r_x_tile<T, TO, M, L, NT>(in, out, rows, Hp, n0, tw, transform, param, blockIdx.x,
+                              blockIdx.y);
 }
 
 // r2c along x of zero-padded real rows (n0 of N0 values), see k_r2c_x.
