@@ -206,9 +206,10 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     import torch
     rng = np.random.default_rng(seed)
     sides = [64, 96, 128, 192, 256, 384, 512, 768, 1024]
-    descs, alg = [], 0
+    descs, alg, shapes, falg = [], 0, [], []
     for _ in range(nfields):
         n0, n1 = (int(rng.choice(sides)) for _ in range(2))
+        shapes.append((n0, n1))
         th, r, L = rng.uniform(0, np.pi), rng.uniform(1, 8), rng.uniform(0.02, 0.2)
         R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
         A = R @ np.diag([L, L / r])
@@ -216,7 +217,8 @@ def run_uneven(nfields=1000, seed=3, reps=3):
                                    anisotropy="geometric", corr_length=list(A.ravel()),
                                    approximate=True, r2c_check_all=True, precision="fp32"))
         H, N1 = n0 + 1, 2 * n1
-        alg += 48 * H * N1 + 16 * H * n1 + 4 * n0 * n1
+        falg.append(48 * H * N1 + 16 * H * n1 + 4 * n0 * n1)
+        alg += falg[-1]
     t0 = time.perf_counter()
     b = pfb.Batch(descs)
     out = torch.empty(int(b.offsets[-1]), dtype=torch.float32, device="cuda")
@@ -237,6 +239,17 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     t0 = time.perf_counter()
     b.generate(seed=11, out=out, use_graph=False)
     eager = time.perf_counter() - t0
+    # per-size throughput: device time of every field in one timed eager
+    # (serial) pass, summed over the fields whose larger side is the key
+    ms = b.field_times(seed=11, out=out)
+    per_size = {}
+    for side in sides:
+        idx = [i for i, (n0, n1) in enumerate(shapes) if max(n0, n1) == side]
+        if idx:
+            t = float(ms[idx].sum()) * 1e-3
+            per_size[str(side)] = {"fields": len(idx), "device_ms": round(t * 1e3, 3),
+                                   "fields_per_s": round(len(idx) / t, 1),
+                                   "alg_GBps": round(sum(falg[i] for i in idx) / t / 1e9, 1)}
     b.close()
     del out
     torch.cuda.empty_cache()
@@ -245,8 +258,10 @@ def run_uneven(nfields=1000, seed=3, reps=3):
             "ms_per_batch": round(el * 1e3, 2), "eager_ms_per_batch": round(eager * 1e3, 2),
             "serial_graph_ms_per_batch": round(serial * 1e3, 2), "branches": best,
             "alg_GBps": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / HBM_PEAK, 4),
-            "setup_s": round(setup, 2),
-            "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch"}
+            "setup_s": round(setup, 2), "per_size_class": per_size,
+            "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch; "
+                    "per_size_class: fields keyed by their larger side, device time of each "
+                    "field in one eager serial pass (HIP events around every field)"}
 
 
 def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):

@@ -287,6 +287,15 @@ int pf_batch_autotune(void* batch, uint64_t seed, uint64_t first_realization, in
                       double transform_param, void* out, int32_t out_mem, int32_t* best,
                       double* best_ms);
 
+/* Per-field device times (no counterpart in the reference): with timing on,
+ * an eager pf_batch_generate (use_graph = 0) brackets every field
+ * (eigenvalues + realization, serially on the batch stream) by HIP events
+ * (on a batch's first call, in a second pass after the one that allocates);
+ * pf_batch_field_times then writes the n durations in ms (PF_ESTATE if the
+ * last generate call was not such a pass). */
+int pf_batch_set_field_timing(void* batch, int32_t on);
+int pf_batch_field_times(void* batch, double* ms);
+
 /* Observability (the analogue of the reference's randomField.verbose
  * prints, fieldtraits.hh:180, matrix.hh:301-305): when on, every kernel
  * launch of a call is bracketed by HIP events on the plan's stream and

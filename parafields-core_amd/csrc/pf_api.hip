@@ -1403,6 +1403,8 @@ struct Batch
   std::vector<hipStream_t> branch;    // extra streams of the forked graph branches
   std::vector<hipEvent_t> ev;         // fork/join events (one per branch)
   hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> fev;        // per-field timing of the eager pass (2 per field)
+  bool field_timing = false, timed = false;
   int branches = 1;
   DevBuf stats, stage;
   bool prepared = false;
@@ -1422,6 +1424,8 @@ struct Batch
     for (auto s : branch)
       (void)hipStreamDestroy(s);
     for (auto e : ev)
+      (void)hipEventDestroy(e);
+    for (auto e : fev)
       (void)hipEventDestroy(e);
     if (ev_fork)
       (void)hipEventDestroy(ev_fork);
@@ -1465,17 +1469,31 @@ struct Batch
   // With k > 1 branches the fields are bin-packed by algorithmic bytes
   // (largest first onto the least loaded branch) and each branch runs on its
   // own stream forked from and joined to the origin stream by events.
-  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out, int k)
+  // time_fields (eager, serial only): an event pair around every field.
+  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out, int k,
+               bool time_fields = false)
   {
     const size_t n = plans.size();
     // never fork a branch that would receive no field: an empty forked
     // branch joins the capture without any node of its own
     k = std::min<int>(k, (int)n);
     if (k <= 1) {
+      if (time_fields) {
+        while (fev.size() < 2 * n) {
+          hipEvent_t e;
+          HIPCHECK(hipEventCreate(&e));
+          fev.push_back(e);
+        }
+      }
       for (size_t i = 0; i < n; i++) {
         plans[i]->stream = stream;
+        if (time_fields)
+          HIPCHECK(hipEventRecord(fev[2 * i], stream));
         enqueue_field(i, seed, first, transform, param, out);
+        if (time_fields)
+          HIPCHECK(hipEventRecord(fev[2 * i + 1], stream));
       }
+      timed = time_fields;
       return;
     }
     ensure_branches(k);
@@ -2136,13 +2154,16 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
       dout = B->stage.p;
     }
     const int k = std::max(1, std::min<int>(B->branches, (int)B->plans.size()));
+    B->timed = false;
     if (!B->prepared) {
       // eager pass: allocates every buffer and twiddle table the capture needs
       B->enqueue(seed, first_realization, transform, transform_param, dout, 1);
       HIPCHECK(hipStreamSynchronize(B->stream));
       B->prepared = true;
+      if (!use_graph && B->field_timing) // the timed pass runs without allocations
+        B->enqueue(seed, first_realization, transform, transform_param, dout, 1, true);
     } else if (!use_graph) {
-      B->enqueue(seed, first_realization, transform, transform_param, dout, 1);
+      B->enqueue(seed, first_realization, transform, transform_param, dout, 1, B->field_timing);
     }
     if (use_graph) {
       const bool same = B->exec && B->key_seed == seed && B->key_first == first_realization &&
@@ -2194,6 +2215,29 @@ int pf_batch_generate(void* batch, uint64_t seed, uint64_t first_realization, in
     if (status != PF_OK)
       g_err = "negative eigenvalues in covariance matrix";
     return status;
+  })
+}
+
+int pf_batch_set_field_timing(void* batch, int32_t on)
+{
+  PF_GUARD({
+    static_cast<Batch*>(batch)->field_timing = on != 0;
+    return PF_OK;
+  })
+}
+
+int pf_batch_field_times(void* batch, double* ms)
+{
+  PF_GUARD({
+    Batch* B = static_cast<Batch*>(batch);
+    if (!B->timed)
+      throw PfError(PF_ESTATE, "the last pf_batch_generate was not a timed eager pass");
+    for (size_t i = 0; i < B->plans.size(); i++) {
+      float t = 0.f;
+      HIPCHECK(hipEventElapsedTime(&t, B->fev[2 * i], B->fev[2 * i + 1]));
+      ms[i] = t;
+    }
+    return PF_OK;
   })
 }
 

@@ -199,6 +199,8 @@ def load(path: str = LIB_PATH):
         "pf_batch_offsets": (i32, [vp, vp]),
         "pf_batch_generate": (i32, [vp, u64, u64, i32, dp, vp, i32, i32, vp]),
         "pf_batch_set_branches": (i32, [vp, i32]),
+        "pf_batch_set_field_timing": (i32, [vp, i32]),
+        "pf_batch_field_times": (i32, [vp, ctypes.POINTER(dp)]),
         "pf_batch_autotune": (i32, [vp, u64, u64, i32, dp, vp, i32, ctypes.POINTER(i32),
                                     ctypes.POINTER(dp)]),
         "pf_last_timings": (i32, [vp, ctypes.c_char_p, i32, ctypes.POINTER(dp), i32,
@@ -221,7 +223,7 @@ def exported_symbols():
             "pf_mlp_params", "pf_mlp_forward", "pf_mlp_gradients", "pf_mlp_train",
             "pf_set_profiling", "pf_last_timings", "pf_batch_create", "pf_batch_destroy",
             "pf_batch_offsets", "pf_batch_generate", "pf_batch_set_branches",
-            "pf_batch_autotune"]
+            "pf_batch_autotune", "pf_batch_set_field_timing", "pf_batch_field_times"]
 
 
 def check(status: int):
@@ -480,6 +482,22 @@ class Batch:
                                          _ptr(out), _mem(out), ctypes.byref(best),
                                          ctypes.byref(ms)))
         return best.value, ms.value
+
+    def field_times(self, seed=0, first=0, transform="none", param=0.0, out=None):
+        """Per-field device times (ms) of one eager serial pass (each field's
+        eigenvalues and realization between two HIP events)."""
+        if out is None:
+            out = np.empty(int(self.offsets[-1]), dtype=self.dtype)
+        check(self.lib.pf_batch_set_field_timing(self.h, 1))
+        try:
+            self.generate(seed=seed, first=first, transform=transform, param=param, out=out,
+                          use_graph=False)
+            ms = np.empty(len(self.offsets) - 1, dtype=np.float64)
+            check(self.lib.pf_batch_field_times(self.h, ms.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_double))))
+        finally:
+            check(self.lib.pf_batch_set_field_timing(self.h, 0))
+        return ms
 
     def field(self, out, i):
         return out[int(self.offsets[i]):int(self.offsets[i + 1])]

@@ -1,6 +1,8 @@
 """Parity of the HIP path (through the C ABI of libparafields_amd.so) with the
 FP64 CPU oracle on identical inputs. FP64 paths: <= 1e-11 relative; FP32
 paths: <= 2e-5 of the field standard deviation; integer/bit work: exact."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -575,6 +577,25 @@ def test_uneven_batch_branches_and_autotune():
     assert best in (1, 2, 4, 8, 16) and ms > 0
     out, _ = b.generate(seed=9, first=3, use_graph=True)
     assert np.array_equal(out, ref)
+
+
+def test_uneven_batch_field_times():
+    """Per-field device times of a timed eager pass: one positive time per
+    field, the timed pass writes the same bits as the graph, and asking for
+    times after a graph replay is a state error."""
+    descs = _uneven_descs(6, seed=12)
+    b = pfb.Batch(descs)
+    ms = b.field_times(seed=2, first=1)          # first call: allocating pass, then the timed one
+    assert ms.shape == (6,) and np.all(ms > 0) and np.all(np.isfinite(ms))
+    ref, _ = b.generate(seed=2, first=1, use_graph=True)
+    out = np.empty_like(ref)
+    ms2 = b.field_times(seed=2, first=1, out=out)
+    assert np.all(ms2 > 0) and np.array_equal(out, ref)
+    b.generate(seed=2, first=1, use_graph=True)
+    buf = np.empty(6)
+    with pytest.raises(RuntimeError):
+        pfb.check(b.lib.pf_batch_field_times(b.h, buf.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
 
 
 def test_uneven_batch_more_branches_than_fields():
