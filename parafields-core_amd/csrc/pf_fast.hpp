@@ -84,11 +84,25 @@ constexpr int ct_nst(int n)
   return s;
 }
 
-// One Stockham stage; tile layout buf[e*SE + l], SE = L + PAD; lane mapping
-// l fastest. Ends with a barrier.
-template <typename T, int N, int L, int NT, int PAD, int R, int NS, bool INV, int TWM>
-__device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid)
+// Source of a transform's first stage / sink of its last stage: the tile in
+// LDS (the default), or a callable in(e, l) -> value / out(e, l, value) that
+// reads the input from or writes the output to global memory directly.
+struct TileLds
 {
+};
+
+// One Stockham stage; tile layout buf[e*SE + l], SE = L + PAD; lane mapping
+// l fastest. With In = TileLds it reads the tile (and waits at a barrier
+// before overwriting it), else it takes its inputs from in(e, l); with
+// Out = TileLds it writes the tile and ends with a barrier, else it hands
+// each output to out(e, l, v) and leaves the LDS tile unused.
+template <typename T, int N, int L, int NT, int PAD, int R, int NS, bool INV, int TWM,
+          typename In = TileLds, typename Out = TileLds>
+__device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
+                                       const In& in = In{}, const Out& out = Out{})
+{
+  constexpr bool LDS_IN = std::is_same<In, TileLds>::value;
+  constexpr bool LDS_OUT = std::is_same<Out, TileLds>::value;
   constexpr int SE = L + PAD;
   constexpr int NR = N / R;
   constexpr int TOTAL = L * NR;
@@ -102,8 +116,12 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid)
       const int l = w % L, j = w / L;
       const int k = j % NS;
 #pragma unroll
-      for (int r = 0; r < R; r++)
-        v[it][r] = buf[(j + r * NR) * SE + l];
+      for (int r = 0; r < R; r++) {
+        if constexpr (LDS_IN)
+          v[it][r] = buf[(j + r * NR) * SE + l];
+        else
+          v[it][r] = in(j + r * NR, l);
+      }
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) {
@@ -116,7 +134,8 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid)
       Butterfly<T, R, INV>::run(v[it]);
     }
   }
-  __syncthreads();
+  if constexpr (LDS_IN && LDS_OUT)
+    __syncthreads();
 #pragma unroll
   for (int it = 0; it < IT; it++) {
     const int w = tid + it * NT;
@@ -125,11 +144,16 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid)
       const int k = j % NS;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++)
-        buf[(base + r * NS) * SE + l] = v[it][r];
+      for (int r = 0; r < R; r++) {
+        if constexpr (LDS_OUT)
+          buf[(base + r * NS) * SE + l] = v[it][r];
+        else
+          out(base + r * NS, l, v[it][r]);
+      }
     }
   }
-  __syncthreads();
+  if constexpr (LDS_OUT)
+    __syncthreads();
 }
 
 // tid: the thread's index in the block. Callers inside a loop pass a value
@@ -142,6 +166,30 @@ __device__ __forceinline__ void tfft(cplx<T>* buf, const cplx<T>* tw, int tid = 
     constexpr int R = ct_radix(N, ST);
     tstage<T, N, L, NT, PAD, R, NS, INV, TWM>(buf, tw, tid);
     tfft<T, N, L, NT, PAD, INV, TWM, ST + 1, NS * R>(buf, tw, tid);
+  }
+}
+
+// tfft with the first stage reading in(e, l) and the last stage writing
+// out(e, l, v) (either may be TileLds). A first stage that reads global
+// memory needs no barrier before it writes the tile; a last stage that writes
+// global memory leaves the tile without a closing barrier.
+template <typename T, int N, int L, int NT, int PAD, bool INV, int TWM, typename In, typename Out,
+          int ST = 0, int NS = 1>
+__device__ __forceinline__ void tfft_io(cplx<T>* buf, const cplx<T>* tw, int tid, const In& in,
+                                        const Out& out)
+{
+  if constexpr (ST < ct_nst(N)) {
+    constexpr int R = ct_radix(N, ST);
+    constexpr bool first = ST == 0, last = ST + 1 == ct_nst(N);
+    if constexpr (first && last)
+      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, In, Out>(buf, tw, tid, in, out);
+    else if constexpr (first)
+      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, In, TileLds>(buf, tw, tid, in, TileLds{});
+    else if constexpr (last)
+      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, TileLds, Out>(buf, tw, tid, TileLds{}, out);
+    else
+      tstage<T, N, L, NT, PAD, R, NS, INV, TWM>(buf, tw, tid);
+    tfft_io<T, N, L, NT, PAD, INV, TWM, In, Out, ST + 1, NS * R>(buf, tw, tid, in, out);
   }
 }
 
@@ -378,25 +426,17 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
     }
   }
   __syncthreads();
-  tfft<T, N, L, NT, 0, true, 1>(buf, tw);
+  // the last stage writes its outputs j < nlast straight to B1: element
+  // (j, l) of the tile at j*plane + l*lstep from a block-uniform base
   const int64_t plane = (int64_t)a.Hp * a.Nmid;
-  // block-uniform base; element (j, l) at j*plane + l*lstep
   cplx<T>* ob = B1 + (int64_t)b * plane * a.nlast + (int64_t)km0 * a.Hp + kx0;
   const int64_t lstep = nyq ? a.Hp : 1;
   const int lmax = nyq ? a.Nmid - km0 : L; // live columns of the tile
-  using TL = TileLoop<L * N, NT>;
-  static_assert(NT % L == 0, "a thread keeps its column across iterations");
-  constexpr int JS = NT / L; // rows advanced per iteration
-  const int l = (int)threadIdx.x % L, j0 = (int)threadIdx.x / L;
-  cplx<T>* op = ob + (int64_t)j0 * plane + l * lstep;
-  const bool lok = l < lmax;
-#pragma unroll
-  for (int it = 0; it < TL::IT; it++) {
-    if (it * JS >= a.nlast) // uniform: the pruned rows are skipped whole
-      break;
-    if (TL::live(it) && lok && j0 + it * JS < a.nlast)
-      op[(int64_t)it * JS * plane] = buf[TL::idx(it)];
-  }
+  auto store = [&](int j, int l, const cplx<T>& v) {
+    if (j < a.nlast && l < lmax)
+      ob[(int64_t)j * plane + l * lstep] = v;
+  };
+  tfft_io<T, N, L, NT, 0, true, 1>(buf, tw, (int)threadIdx.x, TileLds{}, store);
 }
 
 template <typename T, int N, int L, int NT, int NOISE, bool IDX64>
@@ -435,37 +475,24 @@ __device__ __forceinline__ void mid_tile(const cplx<T>* in, cplx<T>* out, int Hp
   }
   in += (int64_t)b * Hp * nz * in_rows;
   out += (int64_t)b * Hp * nz * out_rows;
-  using TL = TileLoop<L * N, NT>;
-  static_assert(NT % L == 0, "a thread keeps its column across iterations");
-  constexpr int JS = NT / L; // rows advanced per iteration
-  // block-uniform bases; a thread's column l is fixed, its row is j0 + it*JS
-  const int l = (int)threadIdx.x % L, j0 = (int)threadIdx.x / L;
+  // column l of the tile: kx0 + l (or row group z0 + l of the Nyquist column);
+  // element e of a line at e*Hp from the column's base. The first stage
+  // reads the lines from B1 and the last stage writes rows j < n_out to B2.
   const int lmax = nyq ? nz - z0 : L; // live columns of the tile
-  const bool lok = l < lmax;
-  const cplx<T>* ip = in + ((int64_t)z0 * in_rows + j0) * Hp + kx0 +
-                      (nyq ? (int64_t)l * in_rows * Hp : (int64_t)l);
-  cplx<T> v[TL::IT];
-#pragma unroll
-  for (int it = 0; it < TL::IT; it++) {
-    v[it] = { 0, 0 };
-    if (TL::live(it) && lok && j0 + it * JS < n_in)
-      v[it] = ip[(int64_t)it * JS * Hp];
-  }
-#pragma unroll
-  for (int it = 0; it < TL::IT; it++)
-    if (TL::live(it))
-      buf[TL::idx(it)] = v[it]; // element (e, l) sits at e * L + l = idx
-  __syncthreads();
-  tfft<T, N, L, NT, 0, INV, 1>(buf, tw);
-  cplx<T>* op = out + ((int64_t)z0 * out_rows + j0) * Hp + kx0 +
-                (nyq ? (int64_t)l * out_rows * Hp : (int64_t)l);
-#pragma unroll
-  for (int it = 0; it < TL::IT; it++) {
-    if (it * JS >= n_out) // uniform: the pruned rows are skipped whole
-      break;
-    if (TL::live(it) && lok && j0 + it * JS < n_out)
-      op[(int64_t)it * JS * Hp] = buf[TL::idx(it)];
-  }
+  const cplx<T>* ib = in + (int64_t)z0 * in_rows * Hp + kx0;
+  cplx<T>* ob = out + (int64_t)z0 * out_rows * Hp + kx0;
+  const int64_t lin = nyq ? (int64_t)in_rows * Hp : 1, lout = nyq ? (int64_t)out_rows * Hp : 1;
+  auto load = [&](int e, int l) -> cplx<T> {
+    cplx<T> v = { 0, 0 };
+    if (e < n_in && l < lmax)
+      v = ib[(int64_t)e * Hp + l * lin];
+    return v;
+  };
+  auto store = [&](int j, int l, const cplx<T>& v) {
+    if (j < n_out && l < lmax)
+      ob[(int64_t)j * Hp + l * lout] = v;
+  };
+  tfft_io<T, N, L, NT, 0, INV, 1>(buf, tw, (int)threadIdx.x, load, store);
 }
 
 template <typename T, int N, int L, int NT, bool INV>
@@ -500,6 +527,8 @@ __device__ __forceinline__ void c2r_x_tile(const cplx<T>* in, TO* out, int64_t r
   const int64_t row0 = (int64_t)bx * L;
   const int lrows = (int)(rows - row0 < L ? rows - row0 : L); // live rows of the tile
   const cplx<T>* ib = in + row0 * Hp;
+  // rows are contiguous along k, the tile's line direction: the loads run
+  // along k (coalesced) and the tile is transposed through LDS
   using TL = TileLoop<L * M, NT>;
   cplx<T> xk[TL::IT], xm[TL::IT];
 #pragma unroll
