@@ -499,6 +499,35 @@ GenGeom gen_geom(const Plan& P)
   return gg;
 }
 
+// f(std::integral_constant<int, family>) for the covariance family `cov`
+// (-1 for families without an instantiation of their own).
+template <typename F>
+void with_cov(int cov, F&& f)
+{
+  switch (cov) {
+#define PF_COV_CASE(c)                                                                             \
+  case c:                                                                                          \
+    f(std::integral_constant<int, c>{});                                                           \
+    break;
+    PF_COV_CASE(PF_COV_EXPONENTIAL)
+    PF_COV_CASE(PF_COV_GAUSSIAN)
+    PF_COV_CASE(PF_COV_SPHERICAL)
+    PF_COV_CASE(PF_COV_SEPARABLE_EXPONENTIAL)
+    PF_COV_CASE(PF_COV_MATERN)
+    PF_COV_CASE(PF_COV_MATERN32)
+    PF_COV_CASE(PF_COV_MATERN52)
+    PF_COV_CASE(PF_COV_DAMPED_OSCILLATION)
+    PF_COV_CASE(PF_COV_GAMMA_EXPONENTIAL)
+    PF_COV_CASE(PF_COV_CAUCHY)
+    PF_COV_CASE(PF_COV_GENERALIZED_CAUCHY)
+    PF_COV_CASE(PF_COV_CUBIC)
+    PF_COV_CASE(PF_COV_WHITE_NOISE)
+#undef PF_COV_CASE
+    default:
+      f(std::integral_constant<int, -1>{});
+  }
+}
+
 unsigned grid1d(int64_t n, int threads = 256)
 {
   int64_t b = (n + threads - 1) / threads;
@@ -554,7 +583,10 @@ bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial
     HIPCHECK(hipStreamSynchronize(P.stream));
   } else {
     Scope sc(&P, "eig_cov");
-    k_eval_cov_full<<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+    with_cov(cp.covariance, [&](auto c) {
+      k_eval_cov_full<decltype(c)::value>
+        <<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+    });
     check_launch();
   }
   const FastCtx fc{ P.stream, fast_budget() };
@@ -621,7 +653,10 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
                               P.stream));
     else {
       Scope sc(&P, "eig_cov");
-      k_eval_cov_dct<<<grid1d(total), 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+      with_cov(cp.covariance, [&](auto c) {
+        k_eval_cov_dct<decltype(c)::value>
+          <<<grid1d(total), 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+      });
       check_launch();
     }
     int64_t stride = 1;
@@ -682,7 +717,10 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
       HIPCHECK(hipStreamSynchronize(P.stream));
     } else {
       Scope sc(&P, "eig_cov");
-      k_eval_cov_full<<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      with_cov(cp.covariance, [&](auto c) {
+      k_eval_cov_full<decltype(c)::value>
+        <<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+    });
       check_launch();
     }
     {
@@ -1394,11 +1432,55 @@ struct Mlp
 // ---------------------------------------------------------------------------
 // Uneven batches: one stream, optional hipGraph capture of the whole batch
 // ---------------------------------------------------------------------------
+// A field the grouped path can take: 2D, R2C matrix with the full real-part
+// check, R2C field, FP32 realizations, and a specialised kernel for every
+// pass of its shape.
+bool groupable(const Plan& P)
+{
+  const pf_desc& d = P.d;
+  const Geo& g = P.g;
+  if (g.dim != 2 || d.matrix_backend != PF_MATRIX_R2C || !d.r2c_check_all ||
+      d.field_backend != PF_FIELD_R2C || d.precision != PF_FP32 || fast_budget() == 0)
+    return false;
+  FastCtx q{ nullptr, fast_budget() };
+  q.query = true;
+  const int N0 = g.N[0], N1 = g.N[1];
+  const GenGeom gg = gen_geom(P);
+  const LastAxis la = last_axis(g);
+  return fast_r2c<double>(q, N0 / 2, nullptr, nullptr, N1, g.Hp, N0, nullptr, 1) &&
+         fast_mid<double>(q, N1, false, nullptr, nullptr, g.Hp, N0 / 2, 1, N1, N1, N1, N1,
+                          nullptr, 1) &&
+         fast_gen_last<float>(q, N1, gg, la, nullptr, nullptr, 0, 0, nullptr, nullptr, 1) &&
+         fast_c2r<float>(q, N0 / 2, nullptr, nullptr, g.n[1], g.Hp, g.n[0], nullptr, 0, 0.f, 1);
+}
+
+// Fields of one shape that run together: every kernel of their eigenvalues
+// and realizations is one launch over the group (field f = launch index f),
+// with per-field covariance parameters, thresholds, Philox realizations,
+// root tables and output offsets. 2D R2C fields (full real-part check, FP32
+// realizations) with specialised kernels for every pass are grouped.
+struct FieldGroup
+{
+  std::vector<size_t> fields; // batch indices
+  DevBuf cov, fidx, ooff, thr; // per field: CovParams, batch index, output offset, threshold
+  DevBuf scratch, eig, sq, partial, w1;
+};
+
+// A unit of work of the batch: one group, or one field through its own plan.
+struct BatchUnit
+{
+  int group; // index into Batch::groups, or -1
+  size_t field;
+  double cost;
+};
+
 struct Batch
 {
   std::vector<std::unique_ptr<Plan>> plans;
   std::vector<int64_t> offsets;
   std::vector<double> cost;           // algorithmic bytes per field (bin packing weight)
+  std::vector<std::unique_ptr<FieldGroup>> groups;
+  std::vector<BatchUnit> units;
   hipStream_t stream = nullptr;       // origin stream
   std::vector<hipStream_t> branch;    // extra streams of the forked graph branches
   std::vector<hipEvent_t> ev;         // fork/join events (one per branch)
@@ -1464,21 +1546,86 @@ struct Batch
         generate_r2c<double>(P, nullptr, seed, first + i, 1, transform, param, o);
     }
   }
-  // Every field: FP64 eigenvalues (classification reduced on the device),
-  // then one realization. No host synchronisation, so it can be captured.
-  // With k > 1 branches the fields are bin-packed by algorithmic bytes
-  // (largest first onto the least loaded branch) and each branch runs on its
-  // own stream forked from and joined to the origin stream by events.
-  // time_fields (eager, serial only): an event pair around every field.
-  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out, int k,
-               bool time_fields = false)
+  // One group on stream s: FP64 eigenvalues of all its fields (covariance,
+  // r2c along x, forward DFT along y, scale/classify/root, per-field
+  // reduction into stats), then one FP32 realization of each (pass G along
+  // y, c2r along x into the field's place in the output).
+  void enqueue_group(FieldGroup& G, hipStream_t s, uint64_t seed, uint64_t first, int transform,
+                     double param, void* out)
   {
-    const size_t n = plans.size();
-    // never fork a branch that would receive no field: an empty forked
+    Plan& P = *plans[G.fields[0]]; // shape: geometry and twiddle tables
+    P.stream = s;
+    const Geo& g = P.g;
+    const int nf = (int)G.fields.size();
+    const int64_t H = g.H, N0 = g.N[0], N1 = g.N[1], rows = N1, count = H * rows;
+    const double inv_next = 1.0 / (double)g.Next;
+    G.scratch.ensure((size_t)nf * (N0 * rows * sizeof(double) + g.Hp * rows * sizeof(cplx<double>)));
+    double* real = G.scratch.as<double>();
+    cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + nf * N0 * rows);
+    with_cov(P.d.covariance, [&](auto c) {
+      k_eval_cov_full_group<decltype(c)::value><<<dim3(grid1d(N0 * rows), nf), 256, 0, s>>>(
+        G.cov.as<CovParams>(), (int)N0, (int)N1, 1, real, N0 * rows);
+    });
+    check_launch();
+    const FastCtx fc{ s, fast_budget() };
+    if (!fast_r2c<double>(fc, (int)(N0 / 2), real, spec, rows, g.Hp, (int)N0,
+                          P.table<double>((int)N0), nf) ||
+        !fast_mid<double>(fc, (int)N1, false, spec, spec, g.Hp, (int)(N0 / 2), 1, (int)N1,
+                          (int)N1, (int)N1, (int)N1, P.table<double>((int)N1), nf))
+      throw PfError(PF_EHIP, "grouped eigenvalue launch rejected");
+    check_launch();
+    const int nbp = 1024;
+    const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
+    G.eig.ensure((size_t)nf * count * sizeof(double));
+    G.sq.ensure((size_t)nf * count * sizeof(float));
+    G.partial.ensure((size_t)nf * nb * sizeof(ClassifyPartial));
+    k_finish_r2c_eigs_group<float><<<dim3(nb, nf), 256, 0, s>>>(
+      spec, g.Hp, (int)H, rows, inv_next, G.thr.as<double>(), G.eig.as<double>(), G.sq.as<float>(),
+      G.partial.as<ClassifyPartial>());
+    check_launch();
+    k_reduce_partials_group<<<nf, 256, 0, s>>>(G.partial.as<ClassifyPartial>(), (int)nb,
+                                               G.fidx.as<int64_t>(), stats.as<ClassifyPartial>());
+    check_launch();
+    GenGeom gg = gen_geom(P);
+    gg.fidx = G.fidx.as<int64_t>();
+    gg.sq_stride = count;
+    const LastAxis la = last_axis(g);
+    G.w1.ensure((size_t)nf * g.Hp * la.nlast * sizeof(cplx<float>));
+    cplx<float>* B1 = G.w1.as<cplx<float>>();
+    FastCtx fx = fc;
+    fx.ooff = G.ooff.as<int64_t>();
+    if (!fast_gen_last<float>(fc, (int)N1, gg, la, G.sq.as<float>(), nullptr, seed, first, B1,
+                              P.table<float>((int)N1), nf) ||
+        !fast_c2r<float>(fx, (int)(N0 / 2), B1, (float*)out, g.n[1], g.Hp, g.n[0],
+                         P.table<float>((int)N0), transform, (float)param, nf))
+      throw PfError(PF_EHIP, "grouped generation launch rejected");
+    check_launch();
+  }
+  void enqueue_unit(const BatchUnit& u, hipStream_t s, uint64_t seed, uint64_t first,
+                    int transform, double param, void* out)
+  {
+    if (u.group >= 0) {
+      enqueue_group(*groups[u.group], s, seed, first, transform, param, out);
+    } else {
+      plans[u.field]->stream = s;
+      enqueue_field(u.field, seed, first, transform, param, out);
+    }
+  }
+  // Every unit: FP64 eigenvalues (classification reduced on the device),
+  // then one realization per field. No host synchronisation, so it can be
+  // captured. With k > 1 branches the units are bin-packed by algorithmic
+  // bytes (largest first onto the least loaded branch) and each branch runs
+  // on its own stream forked from and joined to the origin stream by events.
+  // time_units (eager, serial only): an event pair around every unit.
+  void enqueue(uint64_t seed, uint64_t first, int transform, double param, void* out, int k,
+               bool time_units = false)
+  {
+    const size_t n = units.size();
+    // never fork a branch that would receive no unit: an empty forked
     // branch joins the capture without any node of its own
     k = std::min<int>(k, (int)n);
     if (k <= 1) {
-      if (time_fields) {
+      if (time_units) {
         while (fev.size() < 2 * n) {
           hipEvent_t e;
           HIPCHECK(hipEventCreate(&e));
@@ -1486,26 +1633,26 @@ struct Batch
         }
       }
       for (size_t i = 0; i < n; i++) {
-        plans[i]->stream = stream;
-        if (time_fields)
+        if (time_units)
           HIPCHECK(hipEventRecord(fev[2 * i], stream));
-        enqueue_field(i, seed, first, transform, param, out);
-        if (time_fields)
+        enqueue_unit(units[i], stream, seed, first, transform, param, out);
+        if (time_units)
           HIPCHECK(hipEventRecord(fev[2 * i + 1], stream));
       }
-      timed = time_fields;
+      timed = time_units;
       return;
     }
     ensure_branches(k);
     std::vector<size_t> order(n);
     for (size_t i = 0; i < n; i++)
       order[i] = i;
-    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::sort(order.begin(), order.end(),
+              [&](size_t a, size_t b) { return units[a].cost > units[b].cost; });
     std::vector<double> load(k, 0.);
     std::vector<std::vector<size_t>> bins(k);
     for (size_t i : order) {
       const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      load[j] += cost[i];
+      load[j] += units[i].cost;
       bins[j].push_back(i);
     }
     HIPCHECK(hipEventRecord(ev_fork, stream));
@@ -1513,10 +1660,8 @@ struct Batch
       HIPCHECK(hipStreamWaitEvent(branch[j - 1], ev_fork, 0));
     for (int j = 0; j < k; j++) {
       hipStream_t s = j == 0 ? stream : branch[j - 1];
-      for (size_t i : bins[j]) {
-        plans[i]->stream = s;
-        enqueue_field(i, seed, first, transform, param, out);
-      }
+      for (size_t i : bins[j])
+        enqueue_unit(units[i], s, seed, first, transform, param, out);
     }
     for (int j = 1; j < k; j++) {
       HIPCHECK(hipEventRecord(ev[j - 1], branch[j - 1]));
@@ -2111,6 +2256,53 @@ int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
     }
     B->offsets.push_back(off);
     B->stats.ensure((size_t)n * sizeof(ClassifyPartial));
+    // group the fields of one shape that have specialised kernels for every
+    // pass (PF_BATCH_GROUP=0: every field through its own plan)
+    const char* ge = getenv("PF_BATCH_GROUP");
+    const bool grouping = !(ge && ge[0] == '0');
+    std::map<std::array<int64_t, 6>, int> key_group;
+    for (int i = 0; i < n; i++) {
+      const Plan& P = *B->plans[i];
+      if (!grouping || !groupable(P)) {
+        B->units.push_back({ -1, (size_t)i, B->cost[i] });
+        continue;
+      }
+      const std::array<int64_t, 6> key = { P.g.n[0], P.g.n[1], P.g.N[0], P.g.N[1],
+                                           (int64_t)P.g.transposed, (int64_t)P.d.covariance };
+      auto it = key_group.find(key);
+      if (it == key_group.end()) {
+        it = key_group.emplace(key, (int)B->groups.size()).first;
+        B->groups.push_back(std::make_unique<FieldGroup>());
+        B->units.push_back({ it->second, 0, 0. });
+      }
+      B->groups[it->second]->fields.push_back(i);
+    }
+    for (auto& u : B->units) {
+      if (u.group < 0)
+        continue;
+      FieldGroup& G = *B->groups[u.group];
+      const size_t nf = G.fields.size();
+      u.field = G.fields[0];
+      std::vector<CovParams> cps(nf);
+      std::vector<int64_t> fidx(nf), ooff(nf);
+      std::vector<double> thr(nf);
+      for (size_t f = 0; f < nf; f++) {
+        const Plan& P = *B->plans[G.fields[f]];
+        cps[f] = make_cov(P.d, P.g);
+        fidx[f] = (int64_t)G.fields[f];
+        ooff[f] = B->offsets[G.fields[f]];
+        thr[f] = P.d.threshold;
+        u.cost += B->cost[G.fields[f]];
+      }
+      G.cov.ensure(nf * sizeof(CovParams));
+      G.fidx.ensure(nf * sizeof(int64_t));
+      G.ooff.ensure(nf * sizeof(int64_t));
+      G.thr.ensure(nf * sizeof(double));
+      HIPCHECK(hipMemcpy(G.cov.p, cps.data(), nf * sizeof(CovParams), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(G.fidx.p, fidx.data(), nf * sizeof(int64_t), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(G.ooff.p, ooff.data(), nf * sizeof(int64_t), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(G.thr.p, thr.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+    }
     *batch = B.release();
     return PF_OK;
   })
@@ -2232,10 +2424,18 @@ int pf_batch_field_times(void* batch, double* ms)
     Batch* B = static_cast<Batch*>(batch);
     if (!B->timed)
       throw PfError(PF_ESTATE, "the last pf_batch_generate was not a timed eager pass");
-    for (size_t i = 0; i < B->plans.size(); i++) {
+    for (size_t i = 0; i < B->units.size(); i++) {
       float t = 0.f;
       HIPCHECK(hipEventElapsedTime(&t, B->fev[2 * i], B->fev[2 * i + 1]));
-      ms[i] = t;
+      const BatchUnit& u = B->units[i];
+      if (u.group < 0) {
+        ms[u.field] = t;
+      } else {
+        // a group's fields share its time equally
+        const auto& f = B->groups[u.group]->fields;
+        for (size_t j : f)
+          ms[j] = t / (double)f.size();
+      }
     }
     return PF_OK;
   })

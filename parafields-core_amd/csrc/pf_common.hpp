@@ -41,6 +41,11 @@ struct GenGeom
   int64_t noise_stride; // samples per realization
   int Hp;               // row pitch (complex entries) of the spectrum work arrays, H rounded up
                         // to 128-byte rows
+  // fields of one shape generated together (uneven batches): realization b
+  // draws Philox realization first + fidx[b] and reads its own root table at
+  // sq + b * sq_stride; nullptr / 0 for the realizations of one plan
+  const int64_t* fidx = nullptr;
+  int64_t sq_stride = 0;
 };
 
 // Reference storage index of half-spectrum entry (kx, k1, k2): the order in

@@ -241,6 +241,8 @@ __device__ inline double bessel_k(double nu, double x)
 // Covariance at the embedding grid position idx (integer cell offsets):
 // coordinates idx*h wrapped by the classical rule, geometry transform,
 // covariance functor (+ nugget at the origin).
+// COV >= 0: the family is fixed at compile time (p.covariance is not read).
+template <int COV = -1>
 __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
 {
 #pragma clang fp contract(off)
@@ -269,7 +271,7 @@ __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
   const double hh = sqrt(sum);
   const double var = p.variance;
   double v = 0.;
-  switch (p.covariance) {
+  switch (COV >= 0 ? COV : p.covariance) {
     case PF_COV_SPHERICAL:
       if (hh > 1.)
         v = 0.;

@@ -303,7 +303,8 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
     if (km0 >= a.Nmid)
       return;
   }
-  const uint64_t real = first_real + b;
+  const uint64_t real = first_real + (g.fidx ? (uint64_t)g.fidx[b] : (uint64_t)b);
+  sq += (int64_t)b * g.sq_stride;
   const double* nz = noise ? noise + (int64_t)b * g.noise_stride : nullptr;
   if (NOISE == 1 && !nyq) {
     // One Philox block yields the entries 2m and 2m+1 of the storage order.
@@ -500,8 +501,13 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
                                              int nz, int in_rows, int n_in, int out_rows,
                                              int n_out, const cplx<T>* tw)
 {
-  mid_tile<T, N, L, NT, INV>(in, out, Hp, N0h, nz, in_rows, n_in, out_rows, n_out, tw, blockIdx.x,
-                             blockIdx.y, blockIdx.z);
+  // tile rows narrower than a 128-byte line (FP64 tiles of 2 or 4 lines):
+  // XCD-aware order, so neighbouring kx tiles, which share lines, run on one
+  // XCD and meet in its L2 (whole-line rows keep the hardware order)
+  int bx = blockIdx.x, by = blockIdx.y, b = blockIdx.z;
+  if constexpr (L * sizeof(cplx<T>) < 128)
+    xcd_block(bx, by, b);
+  mid_tile<T, N, L, NT, INV>(in, out, Hp, N0h, nz, in_rows, n_in, out_rows, n_out, tw, bx, by, b);
 }
 
 // Two adjacent outputs stored as one vector.
@@ -517,13 +523,13 @@ struct alignas(2 * sizeof(U)) pair2
 template <typename T, typename TO, int M, int L, int NT>
 __device__ __forceinline__ void c2r_x_tile(const cplx<T>* in, TO* out, int64_t rows, int Hp,
                                            int n0, const cplx<T>* tw, int transform, T param,
-                                           int bx, int b)
+                                           const int64_t* ooff, int bx, int b)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int SE = L + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
   in += (int64_t)b * rows * Hp;
-  out += (int64_t)b * rows * n0;
+  out += ooff ? ooff[b] : (int64_t)b * rows * n0; // ooff: output offset of each realization
   const int64_t row0 = (int64_t)bx * L;
   const int lrows = (int)(rows - row0 < L ? rows - row0 : L); // live rows of the tile
   const cplx<T>* ib = in + row0 * Hp;
@@ -621,9 +627,10 @@ __device__ __forceinline__ void c2r_x_tile(const cplx<T>* in, TO* out, int64_t r
 
 template <typename T, typename TO, int M, int L, int NT>
 __global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64_t rows, int Hp,
-                                               int n0, const cplx<T>* tw, int transform, T param)
+                                               int n0, const cplx<T>* tw, int transform, T param,
+                                               const int64_t* ooff)
 {
-  c2r_x_tile<T, TO, M, L, NT>(in, out, rows, Hp, n0, tw, transform, param, blockIdx.x,
+  c2r_x_tile<T, TO, M, L, NT>(in, out, rows, Hp, n0, tw, transform, param, ooff, blockIdx.x,
                               blockIdx.y);
 }
 

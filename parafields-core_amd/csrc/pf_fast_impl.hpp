@@ -53,6 +53,8 @@ bool gen_last_n(const FastCtx& c, const GenGeom& g, const LastAxis& a, const T* 
   const size_t lds = (size_t)L * N * sizeof(cplx<T>);
   if (a.N0h % L != 0 || lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid(a.N0h / L + 1, a.Nmid, batch);
   // 32-bit slot walk when every storage index and eigenvalue offset of a
   // realization stays below 2^31
@@ -94,6 +96,8 @@ bool mid_nl(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, 
   const size_t lds = (size_t)L * N * sizeof(cplx<T>);
   if (N0h % L != 0 || lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid(N0h / L + 1, nz, batch);
   if (inv)
     kf_mid<T, N, L, NT, true><<<grid, NT, lds, c.stream>>>(in, out, H, N0h, nz, in_rows, n_in,
@@ -144,8 +148,11 @@ bool c2r_nl(const FastCtx& c, const cplx<T>* in, T* out, int64_t rows, int H, in
   const size_t lds = (size_t)(L + 1) * M * sizeof(cplx<T>);
   if (lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid((unsigned)((rows + L - 1) / L), batch);
-  kf_c2r_x<T, T, M, L, NT><<<grid, NT, lds, c.stream>>>(in, out, rows, H, n0, tw, transform, param);
+  kf_c2r_x<T, T, M, L, NT><<<grid, NT, lds, c.stream>>>(in, out, rows, H, n0, tw, transform, param,
+                                                        c.ooff);
   return true;
 }
 
@@ -184,6 +191,8 @@ bool r2c_n(const FastCtx& c, const T* in, cplx<T>* out, int64_t rows, int H, int
   const size_t lds = (size_t)(L + 1) * M * sizeof(cplx<T>);
   if (lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid((unsigned)((rows + L - 1) / L), batch);
   kf_r2c_x<T, T, M, L, NT><<<grid, NT, lds, c.stream>>>(in, out, rows, H, n0, tw);
   return true;
@@ -212,6 +221,8 @@ bool fused_n(const FastCtx& c, cplx<T>* C, const GenGeom& g, const LastAxis& a, 
   const size_t lds = (size_t)L * N * sizeof(cplx<T>);
   if (a.N0h % L != 0 || lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid(a.N0h / L + 1, a.Nmid, batch);
   kf_fused_mult<T, N, L, NT><<<grid, NT, lds, c.stream>>>(C, g, a, eig, op, inv_next, tw, spec,
                                                           nbins);

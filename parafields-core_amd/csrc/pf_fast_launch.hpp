@@ -16,6 +16,10 @@ struct FastCtx
   hipStream_t stream;
   size_t lds_budget;
   int ldiv = 1; // y and x passes: tiles of L/ldiv lines (1 or 2)
+  // x pass of the generation: output offset of each realization (nullptr:
+  // realization b at b * rows * n0)
+  const int64_t* ooff = nullptr;
+  bool query = false; // report whether a specialised kernel applies; launch nothing
 };
 
 template <typename T>

@@ -20,26 +20,31 @@ namespace pfamd {
 
 // Covariance on the DCT half grid [P2][P1][P0] (dctmatrixbackend.hh:164-170 via
 // matrix.hh:615-638).
+// COV: the covariance family at compile time (-1: p.covariance at run time);
+// one family per instantiation keeps the other families' code (and
+// registers) out of the kernel.
+template <int COV>
 __global__ void k_eval_cov_dct(CovParams p, int P0, int P1, int P2, double* out)
 {
   const int64_t total = (int64_t)P0 * P1 * P2;
   for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
        f += (int64_t)gridDim.x * blockDim.x) {
     int64_t idx[3] = { f % P0, (f / P0) % P1, f / ((int64_t)P0 * P1) };
-    out[f] = embedded_cov(p, idx);
+    out[f] = embedded_cov<COV>(p, idx);
   }
 }
 
 // Covariance on the full real grid [N2][N1][N0] (r2cmatrixbackend.hh:150-170;
 // the r2c padding entries are never read by the transform, so they are not
 // stored).
+template <int COV>
 __global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out)
 {
   const int64_t total = (int64_t)N0 * N1 * N2;
   for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
        f += (int64_t)gridDim.x * blockDim.x) {
     int64_t idx[3] = { f % N0, (f / N0) % N1, f / ((int64_t)N0 * N1) };
-    out[f] = embedded_cov(p, idx);
+    out[f] = embedded_cov<COV>(p, idx);
   }
 }
 
@@ -184,7 +189,9 @@ __device__ __forceinline__ void classify_one(double& v, double thr, ClassifyPart
     v = 0.;
 }
 
-__device__ void block_reduce_write(ClassifyPartial c, ClassifyPartial* out)
+// Block-wide sum of the classification records; the result is valid in
+// thread 0.
+__device__ ClassifyPartial block_reduce(ClassifyPartial c)
 {
   __shared__ long long s_small[1024 / 64], s_zero[1024 / 64], s_neg[1024 / 64];
   __shared__ double s_min[1024 / 64];
@@ -202,16 +209,23 @@ __device__ void block_reduce_write(ClassifyPartial c, ClassifyPartial* out)
     s_min[wid] = c.smallest;
   }
   __syncthreads();
+  ClassifyPartial r = { 0, 0, 0, 1.7976931348623157e308 };
   if (threadIdx.x == 0) {
-    ClassifyPartial r = { 0, 0, 0, 1.7976931348623157e308 };
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++) {
       r.small += s_small[w];
       r.zero += s_zero[w];
       r.negative += s_neg[w];
       r.smallest = fmin(r.smallest, s_min[w]);
     }
-    out[blockIdx.x] = r;
   }
+  return r;
+}
+
+__device__ void block_reduce_write(ClassifyPartial c, ClassifyPartial* out)
+{
+  const ClassifyPartial r = block_reduce(c);
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = r;
 }
 
 // Divide by N_ext and classify/clamp every entry (DCT backend; order free),
@@ -293,6 +307,68 @@ __global__ void k_finish_r2c_eigs(const cplx<double>* spec, int Hp, int H, int64
     sq[i] = (T)sqrt(v);
   }
   block_reduce_write(c, partial);
+}
+
+// ---- fields of one shape in one launch (uneven batches) --------------------
+// Field f = blockIdx.y of a group: its own covariance parameters, threshold
+// and arrays at stride `stride` (entries) from the group's base.
+
+template <int COV>
+__global__ void k_eval_cov_full_group(const CovParams* cps, int N0, int N1, int N2, double* out,
+                                      int64_t stride)
+{
+  const CovParams& p = cps[blockIdx.y]; // read in place (uniform address): no local copy
+  out += (int64_t)blockIdx.y * stride;
+  const int64_t total = (int64_t)N0 * N1 * N2;
+  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int64_t idx[3] = { f % N0, (f / N0) % N1, f / ((int64_t)N0 * N1) };
+    out[f] = embedded_cov<COV>(p, idx);
+  }
+}
+
+// k_finish_r2c_eigs per field: spectrum [rows][Hp], eigenvalues and roots
+// [rows][H], gridDim.x block partials per field.
+template <typename T>
+__global__ void k_finish_r2c_eigs_group(const cplx<double>* spec, int Hp, int H, int64_t rows,
+                                        double inv_next, const double* thr, double* eig, T* sq,
+                                        ClassifyPartial* partial)
+{
+  const int f = blockIdx.y;
+  const int64_t count = (int64_t)H * rows;
+  spec += (int64_t)f * Hp * rows;
+  eig += (int64_t)f * count;
+  sq += (int64_t)f * count;
+  const double t = thr[f];
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / H;
+    const int kx = (int)(i - r * H);
+    double v = spec[r * Hp + kx].x * inv_next;
+    classify_one(v, t, c);
+    eig[i] = v;
+    sq[i] = (T)sqrt(v);
+  }
+  block_reduce_write(c, partial + (int64_t)f * gridDim.x);
+}
+
+// One block per field: the field's n partials into stats[fidx[f]].
+__global__ void k_reduce_partials_group(const ClassifyPartial* partial, int n,
+                                        const int64_t* fidx, ClassifyPartial* stats)
+{
+  const int f = blockIdx.x;
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const ClassifyPartial q = partial[(int64_t)f * n + i];
+    c.small += q.small;
+    c.zero += q.zero;
+    c.negative += q.negative;
+    c.smallest = fmin(c.smallest, q.smallest);
+  }
+  const ClassifyPartial r = block_reduce(c);
+  if (threadIdx.x == 0)
+    stats[fidx[f]] = r;
 }
 
 // Sum `n` block partials into one record on the device (capture-safe).

@@ -579,6 +579,39 @@ def test_uneven_batch_branches_and_autotune():
     assert np.array_equal(out, ref)
 
 
+def test_uneven_batch_groups_match_single_fields(monkeypatch):
+    """Fields of one shape run as one group (one launch per kernel over the
+    group, per-field covariance, threshold, Philox realization, root table and
+    output offset): bit-identical to the same batch with every field through
+    its own plan (PF_BATCH_GROUP=0), with the per-field eigenvalue statistics
+    unchanged; one field per group is checked against the oracle."""
+    rng = np.random.default_rng(17)
+    kws = []
+    for i in range(24):
+        n0, n1 = [(64, 64), (96, 128), (128, 96), (64, 192)][i % 4]
+        th, r, L = rng.uniform(0, np.pi), rng.uniform(1, 8), rng.uniform(0.02, 0.2)
+        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+        A = R @ np.diag([L, L / r])
+        kws.append(dict(cells=[n0, n1], covariance="gaussian", anisotropy="geometric",
+                        corr_length=list(A.ravel()), approximate=True, r2c_check_all=True,
+                        threshold=1e-14 * (1 + i % 3)))
+    descs = [desc(2, precision="fp32", **kw) for kw in kws]
+    got, st = pfb.Batch(descs).generate(seed=5, first=7, transform="logNormal", use_graph=True)
+    monkeypatch.setenv("PF_BATCH_GROUP", "0")
+    ref, st_ref = pfb.Batch(descs).generate(seed=5, first=7, transform="logNormal",
+                                            use_graph=True)
+    assert np.array_equal(got, ref)
+    assert st == st_ref
+    monkeypatch.delenv("PF_BATCH_GROUP")
+    b = pfb.Batch(descs)
+    gauss, _ = b.generate(seed=5, first=7)
+    for i in range(4):
+        d64 = desc(2, **kws[i])
+        n = int(pfb.Plan(d64).sizes.noise_per_field)
+        want = orc.generate(d64, orc.noise_philox(5, 7 + i, 0, n))
+        assert np.abs(b.field(gauss, i) - want).max() / want.std() < FP32_TOL
+
+
 def test_uneven_batch_field_times():
     """Per-field device times of a timed eager pass: one positive time per
     field, the timed pass writes the same bits as the graph, and asking for
