@@ -84,6 +84,22 @@ constexpr int ct_nst(int n)
   return s;
 }
 
+// Stores of data that no later access of the same kernel reads (the pass
+// outputs B1, B2 and the fields): non-temporal, so they do not displace
+// what the passes re-read from L2 (twiddles, root eigenvalues).
+template <typename T>
+__device__ __forceinline__ void store_nt(T* p, T a, T b)
+{
+  using V = T __attribute__((ext_vector_type(2)));
+  const V w = { a, b };
+  __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+}
+template <typename T>
+__device__ __forceinline__ void store_nt(cplx<T>* p, const cplx<T>& v)
+{
+  store_nt(reinterpret_cast<T*>(p), v.x, v.y);
+}
+
 // Source of a transform's first stage / sink of its last stage: the tile in
 // LDS (the default), or a callable in(e, l) -> value / out(e, l, value) that
 // reads the input from or writes the output to global memory directly.
@@ -435,7 +451,7 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
   const int lmax = nyq ? a.Nmid - km0 : L; // live columns of the tile
   auto store = [&](int j, int l, const cplx<T>& v) {
     if (j < a.nlast && l < lmax)
-      ob[(int64_t)j * plane + l * lstep] = v;
+      store_nt(&ob[(int64_t)j * plane + l * lstep], v);
   };
   tfft_io<T, N, L, NT, 0, true, 1>(buf, tw, (int)threadIdx.x, TileLds{}, store);
 }
@@ -491,7 +507,7 @@ __device__ __forceinline__ void mid_tile(const cplx<T>* in, cplx<T>* out, int Hp
   };
   auto store = [&](int j, int l, const cplx<T>& v) {
     if (j < n_out && l < lmax)
-      ob[(int64_t)j * Hp + l * lout] = v;
+      store_nt(&ob[(int64_t)j * Hp + l * lout], v);
   };
   tfft_io<T, N, L, NT, 0, INV, 1>(buf, tw, (int)threadIdx.x, load, store);
 }
@@ -509,13 +525,6 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
     xcd_block(bx, by, b);
   mid_tile<T, N, L, NT, INV>(in, out, Hp, N0h, nz, in_rows, n_in, out_rows, n_out, tw, bx, by, b);
 }
-
-// Two adjacent outputs stored as one vector.
-template <typename U>
-struct alignas(2 * sizeof(U)) pair2
-{
-  U x, y;
-};
 
 // c2r along x (see k_c2r_x for the algebra), M = N0/2 point inverse FFT on a
 // padded tile of L rows; first n0 outputs with the value transform.
@@ -583,7 +592,7 @@ __device__ __forceinline__ void c2r_x_tile(const cplx<T>* in, TO* out, int64_t r
         const TO hi = (TO)value_transform_k<T, K>(v.y, param);
         const uint32_t off = (uint32_t)(l * n0 + 2 * m);
         if (pairs) {
-          *reinterpret_cast<pair2<TO>*>(ob + off) = { lo, hi };
+          store_nt(ob + off, lo, hi);
         } else {
           ob[off] = lo;
           if (2 * m + 1 < n0)
