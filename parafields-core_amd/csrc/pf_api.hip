@@ -584,8 +584,8 @@ bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial
   } else {
     Scope sc(&P, "eig_cov");
     with_cov(cp.covariance, [&](auto c) {
-      k_eval_cov_full<decltype(c)::value>
-        <<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      k_eval_cov_full<decltype(c)::value><<<row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]), 256,
+                                            0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
     });
     check_launch();
   }
@@ -654,8 +654,8 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     else {
       Scope sc(&P, "eig_cov");
       with_cov(cp.covariance, [&](auto c) {
-        k_eval_cov_dct<decltype(c)::value>
-          <<<grid1d(total), 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+        k_eval_cov_dct<decltype(c)::value><<<row_blocks((int64_t)g.P[1] * g.P[2], g.P[0]), 256,
+                                             0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
       });
       check_launch();
     }
@@ -718,8 +718,8 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     } else {
       Scope sc(&P, "eig_cov");
       with_cov(cp.covariance, [&](auto c) {
-      k_eval_cov_full<decltype(c)::value>
-        <<<grid1d(N0 * rows), 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      k_eval_cov_full<decltype(c)::value><<<row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]), 256,
+                                            0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
     });
       check_launch();
     }
@@ -1463,7 +1463,7 @@ struct FieldGroup
 {
   std::vector<size_t> fields; // batch indices
   DevBuf cov, fidx, ooff, thr; // per field: CovParams, batch index, output offset, threshold
-  DevBuf scratch, eig, sq, partial, w1;
+  DevBuf scratch, sq, partial, w1;
 };
 
 // A unit of work of the batch: one group, or one field through its own plan.
@@ -1563,8 +1563,9 @@ struct Batch
     double* real = G.scratch.as<double>();
     cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + nf * N0 * rows);
     with_cov(P.d.covariance, [&](auto c) {
-      k_eval_cov_full_group<decltype(c)::value><<<dim3(grid1d(N0 * rows), nf), 256, 0, s>>>(
-        G.cov.as<CovParams>(), (int)N0, (int)N1, 1, real, N0 * rows);
+      k_eval_cov_full_group<decltype(c)::value><<<dim3(row_blocks(rows, (int)N0), nf), 256, 0,
+                                                  s>>>(G.cov.as<CovParams>(), (int)N0, (int)N1, 1,
+                                                       real, N0 * rows);
     });
     check_launch();
     const FastCtx fc{ s, fast_budget() };
@@ -1576,11 +1577,10 @@ struct Batch
     check_launch();
     const int nbp = 1024;
     const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
-    G.eig.ensure((size_t)nf * count * sizeof(double));
-    G.sq.ensure((size_t)nf * count * sizeof(float));
+    G.sq.ensure((size_t)nf * count * sizeof(float)); // the FP64 eigenvalues are not kept
     G.partial.ensure((size_t)nf * nb * sizeof(ClassifyPartial));
     k_finish_r2c_eigs_group<float><<<dim3(nb, nf), 256, 0, s>>>(
-      spec, g.Hp, (int)H, rows, inv_next, G.thr.as<double>(), G.eig.as<double>(), G.sq.as<float>(),
+      spec, g.Hp, (int)H, rows, inv_next, G.thr.as<double>(), nullptr, G.sq.as<float>(),
       G.partial.as<ClassifyPartial>());
     check_launch();
     k_reduce_partials_group<<<nf, 256, 0, s>>>(G.partial.as<ClassifyPartial>(), (int)nb,

@@ -18,6 +18,37 @@ namespace pfamd {
 // Eigenvalues
 // ---------------------------------------------------------------------------
 
+// Visits every (row, i) of an array of nrows rows of n entries with the
+// threads of all blocks: a block takes one row at a time when rows are at
+// least blockDim.x long, else blockDim.x / n whole rows. No 64-bit division
+// per entry; the row index is split by the caller once per row.
+template <typename F>
+__device__ __forceinline__ void for_rows(int64_t nrows, int n, const F& f)
+{
+  const int bd = blockDim.x, t = threadIdx.x;
+  const bool wide = n >= bd;
+  const int rpb = wide ? 1 : bd / n;        // rows per block and sweep
+  const int rsub = wide ? 0 : t / n;        // this thread's row within them
+  const int i0 = wide ? t : t % n, step = wide ? bd : n;
+  if (rsub >= rpb)
+    return;
+  for (int64_t r0 = (int64_t)blockIdx.x * rpb; r0 < nrows; r0 += (int64_t)gridDim.x * rpb) {
+    const int64_t row = r0 + rsub;
+    if (row >= nrows)
+      break;
+    for (int i = i0; i < n; i += step)
+      f(row, i);
+  }
+}
+
+// blocks for for_rows over nrows rows of n entries (256 threads)
+inline unsigned row_blocks(int64_t nrows, int n)
+{
+  const int64_t rpb = n >= 256 ? 1 : 256 / n;
+  const int64_t b = (nrows + rpb - 1) / rpb;
+  return (unsigned)(b < (1 << 20) ? b : (1 << 20));
+}
+
 // Covariance on the DCT half grid [P2][P1][P0] (dctmatrixbackend.hh:164-170 via
 // matrix.hh:615-638).
 // COV: the covariance family at compile time (-1: p.covariance at run time);
@@ -26,12 +57,11 @@ namespace pfamd {
 template <int COV>
 __global__ void k_eval_cov_dct(CovParams p, int P0, int P1, int P2, double* out)
 {
-  const int64_t total = (int64_t)P0 * P1 * P2;
-  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int64_t idx[3] = { f % P0, (f / P0) % P1, f / ((int64_t)P0 * P1) };
-    out[f] = embedded_cov<COV>(p, idx);
-  }
+  for_rows((int64_t)P1 * P2, P0, [&](int64_t row, int i) {
+    const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)P1),
+                             (int64_t)((uint32_t)row / (uint32_t)P1) };
+    out[row * P0 + i] = embedded_cov<COV>(p, idx);
+  });
 }
 
 // Covariance on the full real grid [N2][N1][N0] (r2cmatrixbackend.hh:150-170;
@@ -40,12 +70,11 @@ __global__ void k_eval_cov_dct(CovParams p, int P0, int P1, int P2, double* out)
 template <int COV>
 __global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out)
 {
-  const int64_t total = (int64_t)N0 * N1 * N2;
-  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int64_t idx[3] = { f % N0, (f / N0) % N1, f / ((int64_t)N0 * N1) };
-    out[f] = embedded_cov<COV>(p, idx);
-  }
+  for_rows((int64_t)N1 * N2, N0, [&](int64_t row, int i) {
+    const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)N1),
+                             (int64_t)((uint32_t)row / (uint32_t)N1) };
+    out[row * N0 + i] = embedded_cov<COV>(p, idx);
+  });
 }
 
 // REDFT00 (FFTW's DCT-I) along one axis of a real FP64 array, computed as the
@@ -296,16 +325,12 @@ __global__ void k_finish_r2c_eigs(const cplx<double>* spec, int Hp, int H, int64
                                   ClassifyPartial* partial)
 {
   ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
-  const int64_t count = (int64_t)H * rows;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / H;
-    const int kx = (int)(i - r * H);
+  for_rows(rows, H, [&](int64_t r, int kx) {
     double v = spec[r * Hp + kx].x * inv_next;
     classify_one(v, thr, c);
-    eig[i] = v;
-    sq[i] = (T)sqrt(v);
-  }
+    eig[r * H + kx] = v;
+    sq[r * H + kx] = (T)sqrt(v);
+  });
   block_reduce_write(c, partial);
 }
 
@@ -319,12 +344,11 @@ __global__ void k_eval_cov_full_group(const CovParams* cps, int N0, int N1, int 
 {
   const CovParams& p = cps[blockIdx.y]; // read in place (uniform address): no local copy
   out += (int64_t)blockIdx.y * stride;
-  const int64_t total = (int64_t)N0 * N1 * N2;
-  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int64_t idx[3] = { f % N0, (f / N0) % N1, f / ((int64_t)N0 * N1) };
-    out[f] = embedded_cov<COV>(p, idx);
-  }
+  for_rows((int64_t)N1 * N2, N0, [&](int64_t row, int i) {
+    const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)N1),
+                             (int64_t)((uint32_t)row / (uint32_t)N1) };
+    out[row * N0 + i] = embedded_cov<COV>(p, idx);
+  });
 }
 
 // k_finish_r2c_eigs per field: spectrum [rows][Hp], eigenvalues and roots
@@ -337,19 +361,18 @@ __global__ void k_finish_r2c_eigs_group(const cplx<double>* spec, int Hp, int H,
   const int f = blockIdx.y;
   const int64_t count = (int64_t)H * rows;
   spec += (int64_t)f * Hp * rows;
-  eig += (int64_t)f * count;
+  if (eig) // nullptr: the FP64 eigenvalues are not kept
+    eig += (int64_t)f * count;
   sq += (int64_t)f * count;
   const double t = thr[f];
   ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / H;
-    const int kx = (int)(i - r * H);
+  for_rows(rows, H, [&](int64_t r, int kx) {
     double v = spec[r * Hp + kx].x * inv_next;
     classify_one(v, t, c);
-    eig[i] = v;
-    sq[i] = (T)sqrt(v);
-  }
+    if (eig)
+      eig[r * H + kx] = v;
+    sq[r * H + kx] = (T)sqrt(v);
+  });
   block_reduce_write(c, partial + (int64_t)f * gridDim.x);
 }
 
