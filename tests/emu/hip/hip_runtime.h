@@ -1,0 +1,34 @@
+// Host build of the tile kernels for tests/test_kernel_emulation.py: the
+// HIP vocabulary the kernel sources use, mapped onto C++ on the CPU. A block
+// runs as blockDim.x std::threads; __syncthreads is a std::barrier.
+#pragma once
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__
+#define __align__(n) alignas(n)
+struct emu_dim3
+{
+  unsigned x, y, z;
+};
+extern thread_local emu_dim3 threadIdx;
+extern emu_dim3 blockIdx, blockDim, gridDim;
+extern std::barrier<>* emu_bar;
+inline void __syncthreads() { emu_bar->arrive_and_wait(); }
+// the hardware transcendentals (log2, sin/cos of revolutions, sqrt)
+inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
+inline float __builtin_amdgcn_sinf(float x) { return std::sin(6.283185307179586f * x); }
+inline float __builtin_amdgcn_cosf(float x) { return std::cos(6.283185307179586f * x); }
+inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
+inline void sincospi(double x, double* s, double* c)
+{
+  *s = std::sin(M_PI * x);
+  *c = std::cos(M_PI * x);
+}
+typedef void* hipStream_t;
