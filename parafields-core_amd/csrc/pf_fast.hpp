@@ -111,14 +111,18 @@ struct TileLds
 // l fastest. With In = TileLds it reads the tile (and waits at a barrier
 // before overwriting it), else it takes its inputs from in(e, l); with
 // Out = TileLds it writes the tile and ends with a barrier, else it hands
-// each output to out(e, l, v) and leaves the LDS tile unused.
+// each output to out(e, l, v) and leaves the LDS tile unused. RO < R (last
+// stage, global output only) emits outputs r < RO of each butterfly: with
+// NS * R = N those are the elements below RO * NS, and the arithmetic of the
+// others is left out.
 template <typename T, int N, int L, int NT, int PAD, int R, int NS, bool INV, int TWM,
-          typename In = TileLds, typename Out = TileLds>
+          typename In = TileLds, typename Out = TileLds, int RO = R>
 __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
                                        const In& in = In{}, const Out& out = Out{})
 {
   constexpr bool LDS_IN = std::is_same<In, TileLds>::value;
   constexpr bool LDS_OUT = std::is_same<Out, TileLds>::value;
+  static_assert(RO == R || (!LDS_OUT && NS * R == N), "pruned outputs: last stage, global sink");
   constexpr int SE = L + PAD;
   constexpr int NR = N / R;
   constexpr int TOTAL = L * NR;
@@ -160,7 +164,7 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
       const int k = j % NS;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) {
+      for (int r = 0; r < RO; r++) {
         if constexpr (LDS_OUT)
           buf[(base + r * NS) * SE + l] = v[it][r];
         else
@@ -188,24 +192,28 @@ __device__ __forceinline__ void tfft(cplx<T>* buf, const cplx<T>* tw, int tid = 
 // tfft with the first stage reading in(e, l) and the last stage writing
 // out(e, l, v) (either may be TileLds). A first stage that reads global
 // memory needs no barrier before it writes the tile; a last stage that writes
-// global memory leaves the tile without a closing barrier.
-template <typename T, int N, int L, int NT, int PAD, bool INV, int TWM, typename In, typename Out,
-          int ST = 0, int NS = 1>
+// global memory leaves the tile without a closing barrier. HALF: the sink
+// keeps no output at or above N/2, so a last stage of even radix computes
+// and hands over only the lower half of each butterfly.
+template <typename T, int N, int L, int NT, int PAD, bool INV, int TWM, bool HALF = false,
+          typename In, typename Out, int ST = 0, int NS = 1>
 __device__ __forceinline__ void tfft_io(cplx<T>* buf, const cplx<T>* tw, int tid, const In& in,
                                         const Out& out)
 {
   if constexpr (ST < ct_nst(N)) {
     constexpr int R = ct_radix(N, ST);
     constexpr bool first = ST == 0, last = ST + 1 == ct_nst(N);
+    constexpr bool lds_out = std::is_same<Out, TileLds>::value;
+    constexpr int RO = (HALF && last && !lds_out && R % 2 == 0) ? R / 2 : R;
     if constexpr (first && last)
-      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, In, Out>(buf, tw, tid, in, out);
+      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, In, Out, RO>(buf, tw, tid, in, out);
     else if constexpr (first)
       tstage<T, N, L, NT, PAD, R, NS, INV, TWM, In, TileLds>(buf, tw, tid, in, TileLds{});
     else if constexpr (last)
-      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, TileLds, Out>(buf, tw, tid, TileLds{}, out);
+      tstage<T, N, L, NT, PAD, R, NS, INV, TWM, TileLds, Out, RO>(buf, tw, tid, TileLds{}, out);
     else
       tstage<T, N, L, NT, PAD, R, NS, INV, TWM>(buf, tw, tid);
-    tfft_io<T, N, L, NT, PAD, INV, TWM, In, Out, ST + 1, NS * R>(buf, tw, tid, in, out);
+    tfft_io<T, N, L, NT, PAD, INV, TWM, HALF, In, Out, ST + 1, NS * R>(buf, tw, tid, in, out);
   }
 }
 
@@ -294,10 +302,14 @@ __device__ __forceinline__ void line_k(const GenGeom& g, int kx, int km, int e, 
 // Pass G: spectrum generated in registers along the last axis (length N),
 // inverse DFT, first nlast outputs -> B1[j][km][kx].
 // IDX64: storage indices and eigenvalue offsets in 64 bits (fields with 2^31
-// or more half-spectrum entries); otherwise the slot walk runs in 32 bits.
+// or more half-spectrum entries); otherwise the slot walk runs in 32 bits and
+// the eigenvalue loads and B1 stores take 32-bit byte offsets from a
+// block-uniform base (the launcher checks that a realization's eigenvalue
+// table and B1 slab stay below 4 GiB). HALF: nlast <= N/2, so the last FFT
+// stage computes the lower half of its outputs only.
 // Body of one tile (bx = kx tile or the Nyquist column, by = k_mid row or
 // group, b = realization of the launch).
-template <typename T, int N, int L, int NT, int NOISE, bool IDX64>
+template <typename T, int N, int L, int NT, int NOISE, bool IDX64, bool HALF = false>
 __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& a, const T* sq,
                                               const double* noise, uint64_t seed,
                                               uint64_t first_real, cplx<T>* B1,
@@ -356,6 +368,14 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
     const IX rowstep = (IX)((int64_t)a.H * a.sA); // storage index step per e
     const IX est = (IX)es;
     const T* sqt = sq + eu + kx0;
+    // eigenvalue at element offset i from sqt
+    auto sq_at = [sqt](auto i) -> T {
+      if constexpr (IDX64)
+        return sqt[i];
+      else
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(sqt) +
+                                           (uint32_t)i * (uint32_t)sizeof(T));
+    };
     const IX base0 = (IX)((int64_t)a.H * ((int64_t)km0 * a.sB) + kx0);
     const int par0 = (int)(base0 & 1), rs = (int)(rowstep & 1);
     const int tid = (int)threadIdx.x;
@@ -372,8 +392,8 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
 #pragma unroll
       for (int it = 0; it < ITM; it++) {
         if (N % ES == 0 || e < N) {
-          const T* sqrow = sqt + ((g.dct && e >= Pe) ? dn : up);
-          const T sa = sqrow[lac], sb = sqrow[lb];
+          const IX row = (g.dct && e >= Pe) ? dn : up;
+          const T sa = sq_at(row + (IX)lac), sb = sq_at(row + (IX)lb);
           const uint64_t blk = (uint64_t)base >> 1;
           uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
                             (uint32_t)(real >> 32) };
@@ -408,8 +428,8 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
         const int i = tid + it * NT;
         if (i < nodd) {
           const int e = e0 + estep * i;
-          const T* sqrow = sqt + ((g.dct && e >= Pe) ? est * (IX)(Ne - e) : est * (IX)e);
-          const T sa = sqrow[L - 1];
+          const IX row = (g.dct && e >= Pe) ? est * (IX)(Ne - e) : est * (IX)e;
+          const T sa = sq_at(row + (IX)(L - 1));
           const uint64_t blk = (uint64_t)(IX)(base0 + rowstep * (IX)e + (IX)(L - 1)) >> 1;
           uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
                             (uint32_t)(real >> 32) };
@@ -449,14 +469,25 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
   cplx<T>* ob = B1 + (int64_t)b * plane * a.nlast + (int64_t)km0 * a.Hp + kx0;
   const int64_t lstep = nyq ? a.Hp : 1;
   const int lmax = nyq ? a.Nmid - km0 : L; // live columns of the tile
-  auto store = [&](int j, int l, const cplx<T>& v) {
-    if (j < a.nlast && l < lmax)
-      store_nt(&ob[(int64_t)j * plane + l * lstep], v);
-  };
-  tfft_io<T, N, L, NT, 0, true, 1>(buf, tw, (int)threadIdx.x, TileLds{}, store);
+  if constexpr (IDX64) {
+    auto store = [&](int j, int l, const cplx<T>& v) {
+      if (j < a.nlast && l < lmax)
+        store_nt(&ob[(int64_t)j * plane + l * lstep], v);
+    };
+    tfft_io<T, N, L, NT, 0, true, 1, HALF>(buf, tw, (int)threadIdx.x, TileLds{}, store);
+  } else {
+    const uint32_t pb = (uint32_t)plane * (uint32_t)sizeof(cplx<T>);
+    const uint32_t lb = (uint32_t)lstep * (uint32_t)sizeof(cplx<T>);
+    char* obb = reinterpret_cast<char*>(ob);
+    auto store = [&](int j, int l, const cplx<T>& v) {
+      if (j < a.nlast && l < lmax)
+        store_nt(reinterpret_cast<cplx<T>*>(obb + ((uint32_t)j * pb + (uint32_t)l * lb)), v);
+    };
+    tfft_io<T, N, L, NT, 0, true, 1, HALF>(buf, tw, (int)threadIdx.x, TileLds{}, store);
+  }
 }
 
-template <typename T, int N, int L, int NT, int NOISE, bool IDX64>
+template <typename T, int N, int L, int NT, int NOISE, bool IDX64, bool HALF = false>
 __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T* sq,
                                                   const double* noise, uint64_t seed,
                                                   uint64_t first_real, cplx<T>* B1,
@@ -464,14 +495,18 @@ __global__ void __launch_bounds__(NT) kf_gen_last(GenGeom g, LastAxis a, const T
 {
   int bx, by, b;
   xcd_block(bx, by, b);
-  gen_last_tile<T, N, L, NT, NOISE, IDX64>(g, a, sq, noise, seed, first_real, B1, tw, bx, by, b);
+  gen_last_tile<T, N, L, NT, NOISE, IDX64, HALF>(g, a, sq, noise, seed, first_real, B1, tw, bx,
+                                                 by, b);
 }
 
 // Pass along the middle axis (3D): lines (kx, z) of in[z][e][kx] (in_rows
 // rows per z, n_in leading points, rest zero), DFT, first n_out points ->
 // out[z][j][kx] (out_rows rows per z).
-// Body of tile (bx, by) of realization b.
-template <typename T, int N, int L, int NT, bool INV>
+// Body of tile (bx, by) of realization b. Loads and stores take 32-bit byte
+// offsets from block-uniform bases (the launcher checks that a tile's span
+// stays below 4 GiB). HALF: n_out <= N/2, so the last FFT stage computes the
+// lower half of its outputs only.
+template <typename T, int N, int L, int NT, bool INV, bool HALF = false>
 __device__ __forceinline__ void mid_tile(const cplx<T>* in, cplx<T>* out, int Hp, int N0h, int nz,
                                          int in_rows, int n_in, int out_rows, int n_out,
                                          const cplx<T>* tw, int bx, int by, int b)
@@ -498,21 +533,25 @@ __device__ __forceinline__ void mid_tile(const cplx<T>* in, cplx<T>* out, int Hp
   const int lmax = nyq ? nz - z0 : L; // live columns of the tile
   const cplx<T>* ib = in + (int64_t)z0 * in_rows * Hp + kx0;
   cplx<T>* ob = out + (int64_t)z0 * out_rows * Hp + kx0;
-  const int64_t lin = nyq ? (int64_t)in_rows * Hp : 1, lout = nyq ? (int64_t)out_rows * Hp : 1;
+  constexpr uint32_t CB = sizeof(cplx<T>);
+  const uint32_t eb = (uint32_t)Hp * CB;
+  const uint32_t linb = nyq ? (uint32_t)in_rows * eb : CB, loutb = nyq ? (uint32_t)out_rows * eb : CB;
+  const char* ibb = reinterpret_cast<const char*>(ib);
+  char* obb = reinterpret_cast<char*>(ob);
   auto load = [&](int e, int l) -> cplx<T> {
     cplx<T> v = { 0, 0 };
     if (e < n_in && l < lmax)
-      v = ib[(int64_t)e * Hp + l * lin];
+      v = *reinterpret_cast<const cplx<T>*>(ibb + ((uint32_t)e * eb + (uint32_t)l * linb));
     return v;
   };
   auto store = [&](int j, int l, const cplx<T>& v) {
     if (j < n_out && l < lmax)
-      store_nt(&ob[(int64_t)j * Hp + l * lout], v);
+      store_nt(reinterpret_cast<cplx<T>*>(obb + ((uint32_t)j * eb + (uint32_t)l * loutb)), v);
   };
-  tfft_io<T, N, L, NT, 0, INV, 1>(buf, tw, (int)threadIdx.x, load, store);
+  tfft_io<T, N, L, NT, 0, INV, 1, HALF>(buf, tw, (int)threadIdx.x, load, store);
 }
 
-template <typename T, int N, int L, int NT, bool INV>
+template <typename T, int N, int L, int NT, bool INV, bool HALF = false>
 __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, int Hp, int N0h,
                                              int nz, int in_rows, int n_in, int out_rows,
                                              int n_out, const cplx<T>* tw)
@@ -523,7 +562,8 @@ __global__ void __launch_bounds__(NT) kf_mid(const cplx<T>* in, cplx<T>* out, in
   int bx = blockIdx.x, by = blockIdx.y, b = blockIdx.z;
   if constexpr (L * sizeof(cplx<T>) < 128)
     xcd_block(bx, by, b);
-  mid_tile<T, N, L, NT, INV>(in, out, Hp, N0h, nz, in_rows, n_in, out_rows, n_out, tw, bx, by, b);
+  mid_tile<T, N, L, NT, INV, HALF>(in, out, Hp, N0h, nz, in_rows, n_in, out_rows, n_out, tw, bx, by,
+                                   b);
 }
 
 // c2r along x (see k_c2r_x for the algebra), M = N0/2 point inverse FFT on a

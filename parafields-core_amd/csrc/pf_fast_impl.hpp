@@ -57,15 +57,24 @@ bool gen_last_n(const FastCtx& c, const GenGeom& g, const LastAxis& a, const T* 
     return true;
   dim3 grid(a.N0h / L + 1, a.Nmid, batch);
   // 32-bit slot walk when every storage index and eigenvalue offset of a
-  // realization stays below 2^31
+  // realization stays below 2^31 and its eigenvalue table (at most one entry
+  // per half-spectrum entry) and B1 slab stay below 4 GiB
   const int64_t entries = (int64_t)a.H * a.Nmid * N;
-  const bool idx64 = entries >= ((int64_t)1 << 31) || g.noise_stride >= ((int64_t)1 << 32);
+  const int64_t lim = (int64_t)1 << 32;
+  const bool idx64 = entries >= ((int64_t)1 << 31) || g.noise_stride >= lim ||
+                     entries * (int64_t)sizeof(T) >= lim ||
+                     (int64_t)a.Hp * a.Nmid * a.nlast * (int64_t)sizeof(cplx<T>) >= lim;
+  // outputs j < nlast <= N/2 only: the last stage skips the upper half
+  const bool half = 2 * a.nlast <= N;
   if (noise)
     kf_gen_last<T, N, L, NT, 0, true><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first,
                                                                    B1, tw);
   else if (idx64)
     kf_gen_last<T, N, L, NT, 1, true><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first,
                                                                    B1, tw);
+  else if (half)
+    kf_gen_last<T, N, L, NT, 1, false, true><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed,
+                                                                          first, B1, tw);
   else
     kf_gen_last<T, N, L, NT, 1, false><<<grid, NT, lds, c.stream>>>(g, a, sq, noise, seed, first,
                                                                     B1, tw);
@@ -94,12 +103,19 @@ bool mid_nl(const FastCtx& c, bool inv, const cplx<T>* in, cplx<T>* out, int H, 
 {
   constexpr int NT = fast_NT<T>(N, L);
   const size_t lds = (size_t)L * N * sizeof(cplx<T>);
-  if (N0h % L != 0 || lds > c.lds_budget)
+  // byte offsets within a tile (rows of N elements, Nyquist-column groups
+  // of L row blocks) fit in 32 bits
+  const int64_t rows = in_rows > out_rows ? in_rows : out_rows;
+  if (N0h % L != 0 || lds > c.lds_budget ||
+      ((int64_t)N + (int64_t)L * rows) * H * (int64_t)sizeof(cplx<T>) >= ((int64_t)1 << 32))
     return false;
   if (c.query)
     return true;
   dim3 grid(N0h / L + 1, nz, batch);
-  if (inv)
+  if (inv && 2 * n_out <= N)
+    kf_mid<T, N, L, NT, true, true><<<grid, NT, lds, c.stream>>>(in, out, H, N0h, nz, in_rows,
+                                                                 n_in, out_rows, n_out, tw);
+  else if (inv)
     kf_mid<T, N, L, NT, true><<<grid, NT, lds, c.stream>>>(in, out, H, N0h, nz, in_rows, n_in,
                                                            out_rows, n_out, tw);
   else

@@ -1,7 +1,8 @@
 // Host build of kf_gen_last (the real kernel source, see
 // tests/test_kernel_emulation.py): the Philox main/tail pass (NOISE=1)
-// against the per-entry noise-array path (NOISE=0) fed with the same FP32
-// Box-Muller normals, for the row-parity cases of both storage orders and 2D;
+// (with the full and, where nlast <= N/2, the half last stage) against the
+// per-entry noise-array path (NOISE=0) fed with the same FP32 Box-Muller
+// normals, for the row-parity cases of both storage orders and 2D;
 // every buffer is end-aligned against a PROT_NONE guard region, so an access
 // past its end stops the run with the buffer and offset named.
 #include <hip/hip_runtime.h>
@@ -75,7 +76,9 @@ static int run_case(const char* name, int dim, int transposed, int dct, int N0, 
   const size_t outn = (size_t)nb * g.Hp * a.Nmid * nlast;
   cplx<float>* B1 = (cplx<float>*)galloc("B1", outn * 8);
   cplx<float>* B2 = (cplx<float>*)galloc("B2", outn * 8);
-  for (size_t i = 0; i < outn; i++) { B1[i] = { NAN, NAN }; B2[i] = { NAN, NAN }; }
+  cplx<float>* B3 = (cplx<float>*)galloc("B3", outn * 8);
+  for (size_t i = 0; i < outn; i++) { B1[i] = { NAN, NAN }; B2[i] = { NAN, NAN }; B3[i] = { NAN, NAN }; }
+  const bool half = 2 * nlast <= N;
   cplx<float>* tw = (cplx<float>*)galloc("tw", N * 8);
   for (int j = 0; j < N; j++) tw[j] = { (float)cos(-2 * M_PI * j / N), (float)sin(-2 * M_PI * j / N) };
   blockDim = { (unsigned)NT, 1, 1 };
@@ -90,22 +93,30 @@ static int run_case(const char* name, int dim, int transposed, int dct, int N0, 
   }
   cur = "philox";
   for (unsigned lin : blocks) run_block(lin, NT, [&] { kf_gen_last<float, N, L, NT, 1, false>(g, a, sq, nullptr, seed, first, B1, tw); });
+  cur = "philox, lower half of the last stage";
+  if (half)
+    for (unsigned lin : blocks) run_block(lin, NT, [&] { kf_gen_last<float, N, L, NT, 1, false, true>(g, a, sq, nullptr, seed, first, B3, tw); });
   cur = "noise";
   for (unsigned lin : blocks) run_block(lin, NT, [&] { kf_gen_last<float, N, L, NT, 0, true>(g, a, sq, noise, seed, first, B2, tw); });
-  double mx = 0, md = 0; size_t nan_mis = 0, cnt = 0;
-  for (size_t i = 0; i < outn; i++) {
-    const bool n1 = std::isnan(B1[i].x), n2 = std::isnan(B2[i].x);
-    if (n1 != n2) { nan_mis++; continue; }
-    if (n1) continue;
-    cnt++;
-    mx = std::max(mx, (double)std::max(std::fabs(B2[i].x), std::fabs(B2[i].y)));
-    md = std::max(md, (double)std::max(std::fabs(B1[i].x - B2[i].x), std::fabs(B1[i].y - B2[i].y)));
+  int bad = 0;
+  for (int v = 0; v < (half ? 2 : 1); v++) {
+    const cplx<float>* X = v ? B3 : B1;
+    double mx = 0, md = 0; size_t nan_mis = 0, cnt = 0;
+    for (size_t i = 0; i < outn; i++) {
+      const bool n1 = std::isnan(X[i].x), n2 = std::isnan(B2[i].x);
+      if (n1 != n2) { nan_mis++; continue; }
+      if (n1) continue;
+      cnt++;
+      mx = std::max(mx, (double)std::max(std::fabs(B2[i].x), std::fabs(B2[i].y)));
+      md = std::max(md, (double)std::max(std::fabs(X[i].x - B2[i].x), std::fabs(X[i].y - B2[i].y)));
+    }
+    const bool ok = nan_mis == 0 && cnt > 0 && md <= 2e-5 * mx;
+    printf("%-34s %-5s N=%d L=%d NT=%d blocks=%zu/%u written=%zu nan_mismatch=%zu maxdiff/max=%.3g %s\n", name,
+           v ? "half" : "full", N, L, NT, blocks.size(), tot, cnt, nan_mis, mx > 0 ? md / mx : -1.0, ok ? "OK" : "FAIL");
+    fflush(stdout);
+    bad += !ok;
   }
-  const bool ok = nan_mis == 0 && cnt > 0 && md <= 2e-5 * mx;
-  printf("%-34s N=%d L=%d NT=%d blocks=%zu/%u written=%zu nan_mismatch=%zu maxdiff/max=%.3g %s\n", name, N, L, NT,
-         blocks.size(), tot, cnt, nan_mis, mx > 0 ? md / mx : -1.0, ok ? "OK" : "FAIL");
-  fflush(stdout);
-  return ok ? 0 : 1;
+  return bad;
 }
 
 int main(int argc, char** argv) {
@@ -121,6 +132,8 @@ int main(int argc, char** argv) {
   bad += run_case<128>("2D dct", 2, 0, 1, 96, 1, 64, 2, 1 << 30);
   bad += run_case<768>("2D dct N=768", 2, 0, 1, 32, 1, 384, 1, 1 << 30);
   bad += run_case<16>("3D transposed r2c N=16", 3, 1, 0, 32, 16, 8, 2, 1 << 30);
+  bad += run_case<32>("3D x-fastest r2c nlast=N", 3, 0, 0, 32, 32, 32, 2, 1 << 30);
+  bad += run_case<64>("2D r2c nlast=N", 2, 0, 0, 64, 1, 64, 2, 1 << 30);
   printf(bad ? "FAILURES %d\n" : "all OK\n", bad);
   return bad;
 }
