@@ -151,33 +151,13 @@ size_t lds_budget()
 // through the generic kernels (used by the tests to check both).
 size_t fast_budget() { return getenv("PF_NO_FAST") ? 0 : lds_budget(); }
 
-// Device maximum of dynamic LDS per block (160 KB on gfx950).
-size_t device_lds_max()
-{
-  static size_t v = 0;
-  if (v == 0) {
-    int dev = 0, a = 65536;
-    if (hipGetDevice(&dev) == hipSuccess)
-      (void)hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    v = (size_t)a;
-  }
-  return v;
-}
-
-// The specialised DCT/DST passes (pf_dctdst_fast.hpp) are OFF unless
-// PF_DCT_FAST=1. Two 1024^3 runs of the DCT/DST generation with tiles above
-// 64 KB of LDS (the specialised kernels, then the generic ones with 80 KB
-// tiles), each after other workloads in the same process, ended in GPU
-// memory faults whose cause is not identified: host mirrors of every index
-// and a host emulation of both kernel sources stay in bounds. The generic
-// kernels with tiles of at most 64 KB carry the path meanwhile.
-size_t dct_fast_budget()
-{
-  const char* e = getenv("PF_DCT_FAST");
-  if (getenv("PF_NO_FAST") || !(e && e[0] == '1'))
-    return 0;
-  return device_lds_max();
-}
+// LDS budget of the specialised DCT/DST passes (pf_dctdst_fast.hpp), whose
+// tiles are at most 64 KB. Two 1024^3 runs of the DCT/DST generation with
+// tiles above 64 KB of LDS (an earlier form of the specialised kernels with a
+// pad line, 73.8 KB, then the generic ones with 80 KB tiles), each after
+// other workloads in the same process, ended in GPU memory faults whose cause
+// is not identified; no kernel here takes more than 64 KB of LDS since.
+size_t dct_fast_budget() { return fast_budget(); }
 
 // Tiles of the generic DCT/DST kernels: at most 64 KB (see above).
 size_t dct_generic_budget() { return lds_budget(); }
@@ -1009,26 +989,36 @@ void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t firs
     T* G = P.w1.as<T>();
     T* Hh = dim == 3 ? P.w2.as<T>() : nullptr;
     T* o = out + (int64_t)b0 * g.domain;
-    bool fast = false;
-    {
-      Scope sc(&P, "dct_gen_last");
-      fast = fast_dct_gen_last<T>(fc, Nlast / 2, dg, sq, nz, seed, first + b0, G,
-                                  P.table<T>(Nlast), nb);
-      check_launch();
-    }
-    if (fast && dim == 3) {
-      Scope sc(&P, "dct_mid");
-      fast = fast_dct_mid<T>(fc, g.N[1] / 2, dg, G, Hh, P.table<T>(g.N[1]), nb);
-      check_launch();
-    }
+    const int64_t xrows = (int64_t)g.n[1] * g.n[2];
+    // the specialised passes run when all of them apply (the G and H layouts
+    // of both kinds agree, but one kind throughout keeps the choice simple)
+    FastCtx q = fc;
+    q.query = true;
+    const bool fast =
+      fast_dct_gen_last<T>(q, Nlast / 2, dg, sq, nz, seed, first + b0, G, nullptr, nb) &&
+      (dim != 3 || fast_dct_mid<T>(q, g.N[1] / 2, dg, G, Hh, nullptr, nb)) &&
+      fast_dct_x<T>(q, g.N[0] / 2, dg, dim == 3 ? Hh : G, o, xrows, nullptr, transform, (T)param,
+                    nb);
     if (fast) {
-      Scope sc(&P, "dct_x");
-      fast = fast_dct_x<T>(fc, g.N[0] / 2, dg, dim == 3 ? Hh : G, o, (int64_t)g.n[1] * g.n[2],
-                           P.table<T>(g.N[0]), transform, (T)param, nb);
-      check_launch();
-    }
-    if (fast)
+      {
+        Scope sc(&P, "dct_gen_last_fast");
+        fast_dct_gen_last<T>(fc, Nlast / 2, dg, sq, nz, seed, first + b0, G, P.table<T>(Nlast),
+                             nb);
+        check_launch();
+      }
+      if (dim == 3) {
+        Scope sc(&P, "dct_mid_fast");
+        fast_dct_mid<T>(fc, g.N[1] / 2, dg, G, Hh, P.table<T>(g.N[1]), nb);
+        check_launch();
+      }
+      {
+        Scope sc(&P, "dct_x_fast");
+        fast_dct_x<T>(fc, g.N[0] / 2, dg, dim == 3 ? Hh : G, o, xrows, P.table<T>(g.N[0]),
+                      transform, (T)param, nb);
+        check_launch();
+      }
       continue;
+    }
     // generic kernels (any size with prime factors <= 7): all three passes
     {
       const int M = Nlast / 2;

@@ -9,18 +9,38 @@
 // 4-byte reals a tile row is only 4L bytes wide, so blocks run in an
 // XCD-aware order that keeps the neighbouring tiles of a row (which share its
 // cache lines) in one L2.
+//
+// LDS tile: L lines of M complex values, row pitch L with permuted rows
+// (tix<L, -1>), no pad line, so a tile of 8 x 1024 FP32 lines is exactly
+// 64 KB. The packed c2r reads only the real parts of X_0 and X_M (FFTW's c2r
+// ignores their imaginary parts), so Re X_M is kept in the imaginary slot of
+// row 0 and the tile needs no extra row for X_M.
 #pragma once
 
 #include "pf_dctdst.hpp"
 
 namespace pfamd {
 
-// Pack X (buf rows m < M, xM = X_M) into Z in place and run the inverse FFT.
-template <typename T, int M, int L, int NT>
-__device__ __forceinline__ void c2r_tile_ct(cplx<T>* buf, const cplx<T>* xM, const cplx<T>* tw,
-                                            int tid)
+constexpr int DCT_PAD = -1; // tile layout tix<L, -1>: pitch L, permuted rows
+
+// X_m of line l into the tile (m <= M): row m, except that X_0 and X_M
+// share row 0 as (Re X_0, Re X_M).
+template <typename T, int M, int L>
+__device__ __forceinline__ void put_x(cplx<T>* buf, int m, int l, const cplx<T>& x)
 {
-  constexpr int SE = L + 1;
+  if (m == 0)
+    buf[tix<L, DCT_PAD>(0, l)].x = x.x;
+  else if (m == M)
+    buf[tix<L, DCT_PAD>(0, l)].y = x.x;
+  else
+    buf[tix<L, DCT_PAD>(m, l)] = x;
+}
+
+// Pack X (layout of put_x) into Z in place and run the inverse FFT.
+// Afterwards output x of line l is (x odd ? Im : Re) of row x/2.
+template <typename T, int M, int L, int NT>
+__device__ __forceinline__ void c2r_tile_ct(cplx<T>* buf, const cplx<T>* tw, int tid)
+{
   using TP = TileLoop<L * (M / 2 + 1), NT>;
 #pragma unroll
   for (int it = 0; it < TP::IT; it++) {
@@ -28,41 +48,64 @@ __device__ __forceinline__ void c2r_tile_ct(cplx<T>* buf, const cplx<T>* xM, con
       continue;
     const int idx = TP::idx(it, tid);
     const int l = idx % L, k = idx / L;
-    cplx<T> xk = buf[k * SE + l];
-    cplx<T> xm = (k == 0) ? xM[l] : buf[(M - k) * SE + l];
+    cplx<T> xk, xm;
     if (k == 0) {
-      xk.y = 0;
-      xm.y = 0;
+      const cplx<T> z = buf[tix<L, DCT_PAD>(0, l)];
+      xk = { z.x, T(0) };
+      xm = { z.y, T(0) };
+    } else {
+      xk = buf[tix<L, DCT_PAD>(k, l)];
+      xm = buf[tix<L, DCT_PAD>(M - k, l)];
     }
     {
       const cplx<T> sm = { xk.x + xm.x, xk.y - xm.y };
       const cplx<T> d = { xk.x - xm.x, xk.y + xm.y };
       const cplx<T> td = cmul(cconj(tw[k]), d);
-      buf[k * SE + l] = { sm.x - td.y, sm.y + td.x };
+      buf[tix<L, DCT_PAD>(k, l)] = { sm.x - td.y, sm.y + td.x };
     }
     if (k != 0 && 2 * k != M) {
       const cplx<T> sm = { xm.x + xk.x, xm.y - xk.y };
       const cplx<T> d = { xm.x - xk.x, xm.y + xk.y };
       const cplx<T> td = cmul(cconj(tw[M - k]), d);
-      buf[(M - k) * SE + l] = { sm.x - td.y, sm.y + td.x };
+      buf[tix<L, DCT_PAD>(M - k, l)] = { sm.x - td.y, sm.y + td.x };
     }
   }
   __syncthreads();
-  tfft<T, M, L, NT, 1, true, 2>(buf, tw, tid);
+  tfft<T, M, L, NT, DCT_PAD, true, 2>(buf, tw, tid);
+}
+
+// dct_factor(g, t, k0, k1, k2, 1) without branches: 2^{-d/2} 2^{border/2},
+// or 0 when an odd axis of type t sits on its border.
+template <typename T>
+__device__ __forceinline__ T dct_border_factor(const DctGeom& g, int t, int k0, int k1, int k2)
+{
+  const int b0 = (k0 == 0) | (k0 == g.P0 - 1);
+  const int b1 = (k1 == 0) | (k1 == g.P1 - 1);
+  const int b2 = g.dim == 3 ? ((k2 == 0) | (k2 == g.P2 - 1)) : 0;
+  const int zero = (b0 & t) | (b1 & (t >> 1)) | (b2 & (t >> 2));
+  const int border = b0 + b1 + b2;
+  T v = g.dim == 3 ? (T)0.35355339059327376220 : (T)0.5; // 1/sqrt(2^d)
+  v = (border & 1) ? v * (T)1.41421356237309504880 : v;
+  v = (border & 2) ? v * (T)2 : v;
+  return (zero & 1) ? T(0) : v;
 }
 
 // Pass along the last axis (see kd_gen_last): noise of types (c, c + 2^{d-1})
 // times the DCT/DST factors, c2r along the last axis, first n_last outputs ->
 // G[c][j][k_mid][kx] (row pitch P0p).
+// The noise is drawn without branches, one tile entry (m < M, l) per item:
+// Re X_m from type c and Im X_m from type c + 2^{d-1}; the entry of row 0
+// takes Re X_0 and Re X_M instead (the c2r uses neither Im X_0 nor Im X_M,
+// and the tile keeps Re X_M in the imaginary slot of row 0). Each sample is
+// one counter-based normal (philox_normal), scaled by its border factor and
+// its root eigenvalue.
 template <typename T, int M, int L, int NT, int NOISE>
 __global__ void __launch_bounds__(NT, 4) kdf_gen_last(DctGeom g, const T* sq, const double* noise,
                                                    uint64_t seed, uint64_t first_real, T* G,
                                                    const cplx<T>* tw)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int SE = L + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
-  cplx<T>* xM = buf + SE * M;
   const int Pm = g.dim == 3 ? g.P1 : 1;
   const int nlast = g.dim == 3 ? g.n2 : g.n1;
   int bx, by, b;
@@ -84,92 +127,66 @@ __global__ void __launch_bounds__(NT, 4) kdf_gen_last(DctGeom g, const T* sq, co
   const int oddlast = 1 << (g.dim - 1);
   const int64_t plane = (int64_t)Pm * g.P0p;
   T* out = G + (int64_t)b * ncomb * plane * nlast;
+  // line l: (kx, km) of the tile; lines of an edge tile past the last row
+  // (km >= Pm) read row Pm - 1 and are zeroed
+  auto line = [&](int l, int& kx, int& km, bool& live) {
+    kx = edge ? kx0 : kx0 + l;
+    km = edge ? km0 + l : km0;
+    live = km < Pm;
+    km = live ? km : Pm - 1;
+  };
+  auto root = [&](int kx, int km, int m) -> T {
+    const int k1 = g.dim == 3 ? km : m, k2 = g.dim == 3 ? m : 0;
+    return sq[kx + (int64_t)g.P0 * (k1 + (int64_t)g.P1 * k2)];
+  };
+  // outputs: block-uniform base, 32-bit element offsets (the launcher checks
+  // nlast * plane < 2^32)
+  const uint32_t lstep = edge ? (uint32_t)g.P0p : 1u;
 #pragma unroll 1
   for (int c = 0; c < ncomb; c++) {
     const int t0 = c, t1 = c + oddlast;
     const int tid = opaque_tid();
-    if (NOISE == 1 && !edge && g.sx == 1) {
-      // consecutive kx are consecutive samples: one Philox block (4 normals)
-      // feeds up to 4 entries; NB blocks cover the L samples of a (type, m)
-      // row whatever its alignment
-      constexpr int NB = (L + 6) / 4;
-      using TG = TileLoop<2 * NB * (M + 1), NT>;
-#pragma unroll 2
-      for (int it = 0; it < TG::IT; it++) {
-        if (!TG::live(it, tid))
-          continue;
-        const int idx = TG::idx(it, tid);
-        const int j = idx % NB, rest = idx / NB;
-        const int half = rest & 1, m = rest >> 1;
-        const int t = half ? t1 : t0;
-        const int k1 = g.dim == 3 ? km0 : m, k2 = g.dim == 3 ? m : 0;
-        const int64_t q0 = t * g.Ptot + kx0 + k1 * g.s1 + k2 * g.s2;
-        const uint64_t blk = (uint64_t)(q0 >> 2) + j;
-        const int64_t qb = (int64_t)(blk << 2);
-        if (qb >= q0 + L)
-          continue;
-        const T* sqrow = sq + (int64_t)g.P0 * (k1 + (int64_t)g.P1 * k2);
-        uint32_t cc[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
-                           (uint32_t)(real >> 32) };
-        philox4x32_10(cc, (uint32_t)seed, (uint32_t)(seed >> 32));
-        T nrm[4];
-        BoxMuller<T>::pair(cc[0], cc[1], nrm[0], nrm[1]);
-        BoxMuller<T>::pair(cc[2], cc[3], nrm[2], nrm[3]);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int l = (int)(qb + i - q0);
-          if (l < 0 || l >= L)
-            continue;
-          const int kx = kx0 + l;
-          const T v = dct_factor<T>(g, t, kx, k1, k2, sqrow[kx]) * nrm[i];
-          cplx<T>* dst = (m < M) ? &buf[m * SE + l] : &xM[l];
-          if (half)
-            dst->y = -v;
-          else
-            dst->x = v;
-        }
-      }
-    } else {
-      for (int idx = tid; idx < L * (M + 1); idx += NT) {
-        const int l = idx % L, m = idx / L;
-        const int kx = edge ? kx0 : kx0 + l;
-        const int km = edge ? km0 + l : km0;
-        cplx<T> x = { 0, 0 };
-        if (km < Pm) {
-          const int k1 = g.dim == 3 ? km : m, k2 = g.dim == 3 ? m : 0;
-          const T sl = sq[kx + (int64_t)g.P0 * (k1 + (int64_t)g.P1 * k2)];
-          const int64_t st = kx * g.sx + k1 * g.s1 + k2 * g.s2;
-          const T f0 = dct_factor<T>(g, t0, kx, k1, k2, sl);
-          const T f1 = dct_factor<T>(g, t1, kx, k1, k2, sl);
-          T r0, r1;
-          if (NOISE == 0) {
-            r0 = (T)nz[t0 * g.Ptot + st];
-            r1 = (T)nz[t1 * g.Ptot + st];
-          } else {
-            r0 = philox_normal<T>(seed, real, (uint64_t)(t0 * g.Ptot + st));
-            r1 = philox_normal<T>(seed, real, (uint64_t)(t1 * g.Ptot + st));
-          }
-          x = { f0 * r0, -(f1 * r1) };
-        }
-        if (m < M)
-          buf[m * SE + l] = x;
-        else
-          xM[l] = x;
-      }
+    using TS = TileLoop<L * M, NT>;
+    auto draw = [&](int t, int kx, int km, int me) -> T {
+      const int k1 = g.dim == 3 ? km : me, k2 = g.dim == 3 ? me : 0;
+      const int64_t st = (int64_t)t * g.Ptot + kx * g.sx + k1 * g.s1 + k2 * g.s2;
+      T r;
+      if constexpr (NOISE == 0)
+        r = (T)nz[st];
+      else
+        r = philox_normal<T>(seed, real, (uint64_t)st);
+      return dct_border_factor<T>(g, t, kx, k1, k2) * r;
+    };
+#pragma unroll 1
+    for (int it = 0; it < TS::IT; it++) {
+      if (!TS::live(it, tid))
+        continue;
+      const int idx = TS::idx(it, tid);
+      const int l = idx % L, m = idx / L;
+      const bool row0 = m == 0;
+      const int m2 = row0 ? M : m;
+      int kx, km;
+      bool live;
+      line(l, kx, km, live);
+      const T s_re = root(kx, km, m), s_im = root(kx, km, m2);
+      // Re X_m (type c); Im X_m (type c + 2^{d-1}), or Re X_M in row 0
+      const T re = draw(t0, kx, km, m) * s_re;
+      const T im = draw(row0 ? t0 : t1, kx, km, m2) * s_im;
+      const cplx<T> v = { live ? re : T(0), live ? (row0 ? im : -im) : T(0) };
+      buf[tix<L, DCT_PAD>(m, l)] = v;
     }
     __syncthreads();
-    c2r_tile_ct<T, M, L, NT>(buf, xM, tw, tid);
+    c2r_tile_ct<T, M, L, NT>(buf, tw, tid);
+    T* oc = out + (int64_t)c * plane * nlast + (int64_t)km0 * g.P0p + kx0;
     using TO = TileLoop<L * M, NT>;
 #pragma unroll
     for (int it = 0; it < TO::IT; it++) {
       const int idx = TO::idx(it, tid);
       const int l = idx % L, j = idx / L;
-      const int kx = edge ? kx0 : kx0 + l;
       const int km = edge ? km0 + l : km0;
       if (TO::live(it, tid) && j < nlast && km < Pm) {
-        const cplx<T> v = buf[(j >> 1) * SE + l];
-        out[(int64_t)c * plane * nlast + (int64_t)j * plane + (int64_t)km * g.P0p + kx] =
-          (j & 1) ? v.y : v.x;
+        const cplx<T> v = buf[tix<L, DCT_PAD>(j >> 1, l)];
+        oc[(uint32_t)j * (uint32_t)plane + (uint32_t)l * lstep] = (j & 1) ? v.y : v.x;
       }
     }
     __syncthreads();
@@ -182,9 +199,7 @@ template <typename T, int M, int L, int NT>
 __global__ void __launch_bounds__(NT, 4) kdf_mid(DctGeom g, const T* G, T* Hh, const cplx<T>* tw)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int SE = L + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
-  cplx<T>* xM = buf + SE * M;
   int bx, by, b;
   xcd_block(bx, by, b);
   const bool edge = bx >= (g.P0 - 1) / L;
@@ -240,15 +255,11 @@ __global__ void __launch_bounds__(NT, 4) kdf_mid(DctGeom g, const T* G, T* Hh, c
           continue;
         const int idx = TL::idx(it, tid);
         const int l = idx % L, m = idx / L;
-        const cplx<T> x = { ve[i], -vo[i] };
-        if (m < M)
-          buf[m * SE + l] = x;
-        else
-          xM[l] = x;
+        put_x<T, M, L>(buf, m, l, { ve[i], -vo[i] });
       }
     }
     __syncthreads();
-    c2r_tile_ct<T, M, L, NT>(buf, xM, tw, tid);
+    c2r_tile_ct<T, M, L, NT>(buf, tw, tid);
     T* hb = Hb + (int64_t)tx * hsize + (int64_t)z0 * hplane + kx0;
     const uint32_t hstep = edge ? (uint32_t)hplane : 1u;
     using TO = TileLoop<L * M, NT>;
@@ -259,7 +270,7 @@ __global__ void __launch_bounds__(NT, 4) kdf_mid(DctGeom g, const T* G, T* Hh, c
       const int kx = edge ? kx0 : kx0 + l;
       const int z = edge ? z0 + l : z0;
       if (TO::live(it, tid) && y < g.n1 && z < g.n2) {
-        const cplx<T> v = buf[(y >> 1) * SE + l];
+        const cplx<T> v = buf[tix<L, DCT_PAD>(y >> 1, l)];
         hb[(uint32_t)y * (uint32_t)g.P0p + (uint32_t)l * hstep] = (y & 1) ? v.y : v.x;
       }
     }
@@ -274,9 +285,8 @@ __global__ void __launch_bounds__(NT) kdf_x(DctGeom g, const T* Hh, T* out, int6
                                             const cplx<T>* tw, int transform, T param)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int SE = L + 1, P0 = M + 1;
+  constexpr int P0 = M + 1;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
-  cplx<T>* xM = buf + SE * M;
   const int b = blockIdx.y;
   const int64_t hsize = rows * g.P0p;
   const T* h0 = Hh + (int64_t)b * 2 * hsize;
@@ -303,14 +313,10 @@ __global__ void __launch_bounds__(NT) kdf_x(DctGeom g, const T* Hh, T* out, int6
       continue;
     const int idx = TL::idx(it);
     const int l = idx / P0, m = idx % P0;
-    const cplx<T> x = { va[it], -vb[it] };
-    if (m < M)
-      buf[m * SE + l] = x;
-    else
-      xM[l] = x;
+    put_x<T, M, L>(buf, m, l, { va[it], -vb[it] });
   }
   __syncthreads();
-  c2r_tile_ct<T, M, L, NT>(buf, xM, tw, (int)threadIdx.x);
+  c2r_tile_ct<T, M, L, NT>(buf, tw, (int)threadIdx.x);
   using TO = TileLoop<L * M, NT>;
   // partial unroll: the value transform is a runtime switch
 #pragma unroll 4
@@ -319,7 +325,7 @@ __global__ void __launch_bounds__(NT) kdf_x(DctGeom g, const T* Hh, T* out, int6
     const int l = idx / M, x = idx % M;
     const int64_t r = row0 + l;
     if (TO::live(it) && x < g.n0 && r < rows) {
-      const cplx<T> v = buf[(x >> 1) * SE + l];
+      const cplx<T> v = buf[tix<L, DCT_PAD>(x >> 1, l)];
       out[r * g.n0 + x] = value_transform<T>(transform, (x & 1) ? v.y : v.x, param);
     }
   }

@@ -107,7 +107,24 @@ struct TileLds
 {
 };
 
-// One Stockham stage; tile layout buf[e*SE + l], SE = L + PAD; lane mapping
+// LDS tile index of (element e, line l). PAD >= 0: row pitch L + PAD.
+// PAD < 0: row pitch L with the rows permuted, e -> e ^ ((e >> 3) & 3). A
+// first Stockham stage of radix R writes rows j*R + r for eight consecutive j
+// at once; for 64-byte rows (L = 8 complex FP32) those rows all start on the
+// same bank, and the permutation spreads them over the four bank offsets of
+// a 64-byte row. It keeps each aligned group of 8 rows in place as a set, so
+// the accesses of the other stages (eight consecutive rows of one aligned
+// group per wave) stay contiguous.
+template <int L, int PAD>
+__device__ __forceinline__ int tix(int e, int l)
+{
+  if constexpr (PAD >= 0)
+    return e * (L + PAD) + l;
+  else
+    return (e ^ ((e >> 3) & 3)) * L + l;
+}
+
+// One Stockham stage; tile layout buf[tix<L, PAD>(e, l)]; lane mapping
 // l fastest. With In = TileLds it reads the tile (and waits at a barrier
 // before overwriting it), else it takes its inputs from in(e, l); with
 // Out = TileLds it writes the tile and ends with a barrier, else it hands
@@ -123,7 +140,6 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
   constexpr bool LDS_IN = std::is_same<In, TileLds>::value;
   constexpr bool LDS_OUT = std::is_same<Out, TileLds>::value;
   static_assert(RO == R || (!LDS_OUT && NS * R == N), "pruned outputs: last stage, global sink");
-  constexpr int SE = L + PAD;
   constexpr int NR = N / R;
   constexpr int TOTAL = L * NR;
   constexpr int IT = (TOTAL + NT - 1) / NT;
@@ -138,7 +154,7 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
 #pragma unroll
       for (int r = 0; r < R; r++) {
         if constexpr (LDS_IN)
-          v[it][r] = buf[(j + r * NR) * SE + l];
+          v[it][r] = buf[tix<L, PAD>(j + r * NR, l)];
         else
           v[it][r] = in(j + r * NR, l);
       }
@@ -166,7 +182,7 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
 #pragma unroll
       for (int r = 0; r < RO; r++) {
         if constexpr (LDS_OUT)
-          buf[(base + r * NS) * SE + l] = v[it][r];
+          buf[tix<L, PAD>(base + r * NS, l)] = v[it][r];
         else
           out(base + r * NS, l, v[it][r]);
       }

@@ -262,13 +262,14 @@ bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const Las
 }
 
 // --- DCT/DST field passes ---------------------------------------------------
-// Tiles of L lines of M+1 points: the largest power of two L <= 32 (FP32) /
-// 16 (FP64) whose tile fits in 80 KB, so two blocks share a CU's 160 KB.
+// Tiles of L lines of M complex values (no pad line, X_M folded into row 0):
+// the largest power of two L <= 32 (FP32) / 16 (FP64) whose tile fits in
+// 64 KB (FP32: 8 x 1024, 16 x 512, 32 x 256).
 template <typename T>
 constexpr int dct_L(int M)
 {
   int L = sizeof(T) == 4 ? 32 : 16;
-  while (L > 1 && (size_t)((L + 1) * M + L) * 2 * sizeof(T) > (size_t)80 * 1024)
+  while (L > 1 && (size_t)L * M * 2 * sizeof(T) > (size_t)64 * 1024)
     L /= 2;
   return L;
 }
@@ -276,7 +277,18 @@ constexpr int dct_L(int M)
 template <typename T, int M>
 constexpr size_t dct_lds()
 {
-  return (size_t)((dct_L<T>(M) + 1) * M + dct_L<T>(M)) * 2 * sizeof(T);
+  return (size_t)dct_L<T>(M) * M * 2 * sizeof(T);
+}
+
+// Threads of a DCT/DST pass: 16 values per thread for FP32, 8 for FP64 (its
+// values take twice the registers).
+template <typename T>
+constexpr int dct_NT(int M, int L)
+{
+  const int me = sizeof(T) == 4 ? 16 : 8;
+  int nt = (L * M + me - 1) / me;
+  nt = ((nt + 63) / 64) * 64;
+  return nt < 64 ? 64 : nt;
 }
 
 #define PF_DCT_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
@@ -285,11 +297,16 @@ template <typename T, int M>
 bool dct_gen_last_n(const FastCtx& c, const DctGeom& g, const T* sq, const double* noise,
                     uint64_t seed, uint64_t first, T* G, const cplx<T>* tw, int batch)
 {
-  constexpr int L = dct_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr int L = dct_L<T>(M), NT = dct_NT<T>(M, L);
   constexpr size_t lds = dct_lds<T, M>();
   const int Pm = g.dim == 3 ? g.P1 : 1;
-  if ((g.P0 - 1) % L != 0 || lds > c.lds_budget)
+  const int64_t nlast = g.dim == 3 ? g.n2 : g.n1;
+  // the pass stores at 32-bit element offsets within one output array
+  if ((g.P0 - 1) % L != 0 || lds > c.lds_budget ||
+      nlast * Pm * (int64_t)g.P0p >= ((int64_t)1 << 32))
     return false;
+  if (c.query)
+    return true;
   dim3 grid((g.P0 - 1) / L + 1, Pm, batch);
   if (noise)
     kdf_gen_last<T, M, L, NT, 0><<<grid, NT, lds, c.stream>>>(g, sq, noise, seed, first, G, tw);
@@ -318,10 +335,12 @@ template <typename T, int M>
 bool dct_mid_n(const FastCtx& c, const DctGeom& g, const T* G, T* Hh, const cplx<T>* tw,
                int batch)
 {
-  constexpr int L = dct_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr int L = dct_L<T>(M), NT = dct_NT<T>(M, L);
   constexpr size_t lds = dct_lds<T, M>();
   if ((g.P0 - 1) % L != 0 || lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid((g.P0 - 1) / L + 1, g.n2, batch);
   kdf_mid<T, M, L, NT><<<grid, NT, lds, c.stream>>>(g, G, Hh, tw);
   return true;
@@ -346,10 +365,12 @@ template <typename T, int M>
 bool dct_x_n(const FastCtx& c, const DctGeom& g, const T* Hh, T* out, int64_t rows,
              const cplx<T>* tw, int transform, T param, int batch)
 {
-  constexpr int L = dct_L<T>(M), NT = fast_NT<T>(M, L);
+  constexpr int L = dct_L<T>(M), NT = dct_NT<T>(M, L);
   constexpr size_t lds = dct_lds<T, M>();
   if (lds > c.lds_budget)
     return false;
+  if (c.query)
+    return true;
   dim3 grid((unsigned)((rows + L - 1) / L), batch);
   kdf_x<T, M, L, NT><<<grid, NT, lds, c.stream>>>(g, Hh, out, rows, tw, transform, param);
   return true;

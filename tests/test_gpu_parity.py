@@ -372,21 +372,25 @@ DCT_CASES = [
     (3, [32, 32, 16], dict(covariance="exponential", corr_length=0.1)),
 ]
 
+# Pass length M = 1024 (FP32 tiles of 8 x 1024 complex, exactly 64 KB of LDS;
+# FP64 4 x 1024) in the last-axis pass, the x pass and the middle pass; every
+# pass of these shapes has a specialised kernel.
+DCT_FAST_CASES = [
+    (2, [16, 1024], dict(covariance="exponential", corr_length=0.05, transposed=0,
+                         approximate=True)),
+    (2, [1024, 16], dict(covariance="exponential", corr_length=0.05, approximate=True)),
+    (3, [32, 1024, 16], dict(covariance="matern32", anisotropy="axiparallel",
+                             corr_length=[0.1, 0.02, 0.1], transposed=0, approximate=True)),
+]
 
-DCT_FAST_OFF = ("the specialised DCT/DST passes are switched off (PF_DCT_FAST unset): their first "
-                "1024^3 run ended in a GPU memory fault whose cause is not identified")
 
-
-@pytest.mark.parametrize("path", [pytest.param("fast", marks=pytest.mark.skip(reason=DCT_FAST_OFF)),
-                                  "generic"])
-@pytest.mark.parametrize("dim,cells,kw", DCT_CASES)
+@pytest.mark.parametrize("path", ["fast", "generic"])
+@pytest.mark.parametrize("dim,cells,kw", DCT_CASES + DCT_FAST_CASES)
 def test_dctdst_generate_fp64(dim, cells, kw, path, monkeypatch):
     """Host noise and Philox noise against the oracle, through the
-    compile-time specialised DCT/DST passes (where the pass lengths have one,
-    opt-in with PF_DCT_FAST=1) and through the generic ones."""
-    if path == "fast":
-        monkeypatch.setenv("PF_DCT_FAST", "1")
-    else:
+    compile-time specialised DCT/DST passes (the default where the pass
+    lengths have one) and through the generic ones (PF_NO_FAST=1)."""
+    if path == "generic":
         monkeypatch.setenv("PF_NO_FAST", "1")
     d = desc(dim, cells, field="dctdst", matrix="dct", **kw)
     p = pfb.Plan(d)
@@ -399,7 +403,22 @@ def test_dctdst_generate_fp64(dim, cells, kw, path, monkeypatch):
     assert rel_max(gp, orc.generate(d, orc.noise_philox(5, 2, 0, npf))) < FP64_TOL
 
 
-@pytest.mark.parametrize("dim,cells,kw", DCT_CASES[2:])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("dim,cells,kw", DCT_FAST_CASES)
+def test_dctdst_fast_passes_run(dim, cells, kw, precision):
+    """The shapes of DCT_FAST_CASES go through the specialised passes (their
+    profiling labels end in _fast) and the generic path is not taken."""
+    p = pfb.Plan(desc(dim, cells, field="dctdst", matrix="dct", precision=precision, **kw))
+    p.generate(1, seed=3)
+    p.set_profiling(True)
+    p.generate(2, seed=3)
+    labels = [name for name, _ in p.last_timings()]
+    p.set_profiling(False)
+    want = ["dct_gen_last_fast"] + (["dct_mid_fast"] if dim == 3 else []) + ["dct_x_fast"]
+    assert [x for x in labels if x.startswith("dct_")] == want, labels
+
+
+@pytest.mark.parametrize("dim,cells,kw", DCT_CASES[2:] + DCT_FAST_CASES)
 def test_dctdst_generate_fp32(dim, cells, kw):
     d = desc(dim, cells, field="dctdst", matrix="dct", **kw)
     d32 = desc(dim, cells, field="dctdst", matrix="dct", precision="fp32", **kw)

@@ -1,4 +1,4 @@
-"""Host build of the generation kernel (no GPU needed).
+"""Host builds of the generation kernels (no GPU needed).
 
 `kf_gen_last` (parafields-core_amd/csrc/pf_fast.hpp) is compiled with g++
 against a small HIP shim (tests/emu/hip/hip_runtime.h): every workgroup runs
@@ -9,6 +9,12 @@ main pass plus odd-row tail pass) against the same kernel fed the identical
 normals from a precomputed array, over both storage orders, 2D and 3D, R2C and
 DCT layouts and odd tile counts. An out-of-bounds access stops the run naming
 the buffer and offset; a mismatch or a missed entry fails the comparison.
+
+`tests/emu/dct_pass_check.cc` does the same for the three specialised
+DCT/DST passes (pf_dctdst_fast.hpp, 64 KB tiles): every block of small
+shapes, sampled blocks of the 1024^3 shape at its launch geometry (three
+realizations per launch), and the in-kernel Philox normals against the same
+passes fed those normals from an array.
 
 Three device-only constructs are replaced by plain C++ with the same result
 before compiling: the three-input XOR instruction (a ^ b ^ k), the empty asm
@@ -67,3 +73,17 @@ def test_gen_pass_host_build(tmp_path):
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "all OK" in r.stdout
     assert r.stdout.count(" OK") >= 20
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_dct_passes_host_build(tmp_path):
+    src = _host_sources(str(tmp_path))
+    exe = str(tmp_path / "dct_pass_check")
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-w", "-I", EMU, "-I", src,
+           os.path.join(EMU, "dct_pass_check.cc"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=900)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "all OK" in r.stdout
