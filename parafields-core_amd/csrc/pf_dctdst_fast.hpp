@@ -127,17 +127,34 @@ __global__ void __launch_bounds__(NT, 4) kdf_gen_last(DctGeom g, const T* sq, co
   const int oddlast = 1 << (g.dim - 1);
   const int64_t plane = (int64_t)Pm * g.P0p;
   T* out = G + (int64_t)b * ncomb * plane * nlast;
-  // line l: (kx, km) of the tile; lines of an edge tile past the last row
-  // (km >= Pm) read row Pm - 1 and are zeroed
-  auto line = [&](int l, int& kx, int& km, bool& live) {
-    kx = edge ? kx0 : kx0 + l;
-    km = edge ? km0 + l : km0;
-    live = km < Pm;
-    km = live ? km : Pm - 1;
+  // The thread's line l = tid % L is the same for all its items (NT is a
+  // multiple of L); its items are the rows m = m_t + it * (NT / L). Lines of
+  // an edge tile past the last row (km >= Pm) read row Pm - 1 and are zeroed.
+  using TS = TileLoop<L * M, NT>;
+  static_assert(NT % L == 0, "items of a thread share one line");
+  constexpr int MS = NT / L;
+  const int l = (int)threadIdx.x % L, m_t = (int)threadIdx.x / L;
+  const int kx = edge ? kx0 : kx0 + l;
+  const int km_raw = edge ? km0 + l : km0;
+  const bool live = km_raw < Pm;
+  const int km = live ? km_raw : Pm - 1;
+  // sample index (reference storage order) and root-table offset of row m:
+  // base + m * step along the last axis
+  const int64_t s_base = (int64_t)kx * g.sx + (g.dim == 3 ? (int64_t)km * g.s1 : 0);
+  const int64_t s_step = g.dim == 3 ? g.s2 : g.s1;
+  const int64_t r_base = kx + (g.dim == 3 ? (int64_t)g.P0 * km : 0);
+  const int64_t r_step = g.dim == 3 ? (int64_t)g.P0 * g.P1 : (int64_t)g.P0;
+  const T sM = sq[r_base + M * r_step]; // root of element M (row 0's imaginary slot)
+  // border factor of type t for a last-axis element inside (1) or on the
+  // border (0 or M) of the axis; the other axes are fixed by the line
+  auto fac = [&](int t, int mlast) {
+    return dct_border_factor<T>(g, t, kx, g.dim == 3 ? km : mlast, g.dim == 3 ? mlast : 0);
   };
-  auto root = [&](int kx, int km, int m) -> T {
-    const int k1 = g.dim == 3 ? km : m, k2 = g.dim == 3 ? m : 0;
-    return sq[kx + (int64_t)g.P0 * (k1 + (int64_t)g.P1 * k2)];
+  auto normal = [&](int64_t st) -> T {
+    if constexpr (NOISE == 0)
+      return (T)nz[st];
+    else
+      return philox_normal1<T>(seed, real, (uint64_t)st);
   };
   // outputs: block-uniform base, 32-bit element offsets (the launcher checks
   // nlast * plane < 2^32)
@@ -146,34 +163,28 @@ __global__ void __launch_bounds__(NT, 4) kdf_gen_last(DctGeom g, const T* sq, co
   for (int c = 0; c < ncomb; c++) {
     const int t0 = c, t1 = c + oddlast;
     const int tid = opaque_tid();
-    using TS = TileLoop<L * M, NT>;
-    auto draw = [&](int t, int kx, int km, int me) -> T {
-      const int k1 = g.dim == 3 ? km : me, k2 = g.dim == 3 ? me : 0;
-      const int64_t st = (int64_t)t * g.Ptot + kx * g.sx + k1 * g.s1 + k2 * g.s2;
-      T r;
-      if constexpr (NOISE == 0)
-        r = (T)nz[st];
-      else
-        r = philox_normal<T>(seed, real, (uint64_t)st);
-      return dct_border_factor<T>(g, t, kx, k1, k2) * r;
-    };
+    const T f0i = fac(t0, 1), f0b = fac(t0, 0), f1i = fac(t1, 1);
+    // Re X_m (type c) and Im X_m (type c + 2^{d-1}) of the thread's rows; the
+    // entry of row 0 takes Re X_0 and Re X_M
+    int64_t s_re = (int64_t)t0 * g.Ptot + s_base + (int64_t)m_t * s_step;
+    int64_t s_im = s_re + (int64_t)(t1 - t0) * g.Ptot;
+    const int64_t s_xm = (int64_t)t0 * g.Ptot + s_base + (int64_t)M * s_step;
+    const T* rp = sq + r_base + (int64_t)m_t * r_step;
+    const int64_t ds = (int64_t)MS * s_step, dr = (int64_t)MS * r_step;
 #pragma unroll 1
     for (int it = 0; it < TS::IT; it++) {
-      if (!TS::live(it, tid))
-        continue;
-      const int idx = TS::idx(it, tid);
-      const int l = idx % L, m = idx / L;
-      const bool row0 = m == 0;
-      const int m2 = row0 ? M : m;
-      int kx, km;
-      bool live;
-      line(l, kx, km, live);
-      const T s_re = root(kx, km, m), s_im = root(kx, km, m2);
-      // Re X_m (type c); Im X_m (type c + 2^{d-1}), or Re X_M in row 0
-      const T re = draw(t0, kx, km, m) * s_re;
-      const T im = draw(row0 ? t0 : t1, kx, km, m2) * s_im;
-      const cplx<T> v = { live ? re : T(0), live ? (row0 ? im : -im) : T(0) };
-      buf[tix<L, DCT_PAD>(m, l)] = v;
+      if (TS::live(it, tid)) {
+        const int m = m_t + it * MS;
+        const bool row0 = m == 0;
+        const T root_m = *rp;
+        const T re = normal(s_re) * (row0 ? f0b : f0i) * root_m;
+        const T im = normal(row0 ? s_xm : s_im) * (row0 ? f0b : f1i) * (row0 ? sM : root_m);
+        const cplx<T> v = { live ? re : T(0), live ? (row0 ? im : -im) : T(0) };
+        buf[tix<L, DCT_PAD>(m, l)] = v;
+      }
+      s_re += ds;
+      s_im += ds;
+      rp += dr;
     }
     __syncthreads();
     c2r_tile_ct<T, M, L, NT>(buf, tw, tid);

@@ -148,6 +148,28 @@ __device__ __forceinline__ T philox_normal(uint64_t seed, uint64_t real, uint64_
   return (s & 1) ? b : a;
 }
 
+// philox_normal for a sample drawn alone: the same value, computing only the
+// half of the Box-Muller pair that sample s takes. FP32: cos(2 pi u) is
+// taken as sin(2 pi (u + 1/4)), one hardware sine (the sum rounds the turn
+// by at most 6e-8).
+template <typename T>
+__device__ __forceinline__ T philox_normal1(uint64_t seed, uint64_t real, uint64_t s)
+{
+  return philox_normal<T>(seed, real, s);
+}
+
+template <>
+__device__ __forceinline__ float philox_normal1<float>(uint64_t seed, uint64_t real, uint64_t s)
+{
+  const uint64_t blk = s >> 2;
+  uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real, (uint32_t)(real >> 32) };
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const bool hi = (s >> 1) & 1;
+  const float r = BoxMuller<float>::radius(hi ? c[2] : c[0]);
+  const float u = BoxMuller<float>::turn(hi ? c[3] : c[1]);
+  return r * __builtin_amdgcn_sinf((s & 1) ? u : u + 0.25f);
+}
+
 // ---------------------------------------------------------------------------
 // Covariance model on the device (FP64), covariance.hh:18-823.
 // ---------------------------------------------------------------------------
