@@ -9,7 +9,8 @@ resident in HBM. value = fields per second over the timed steps.
 
 Also reported: per-kernel device times (HIP events around every launch of the
 timed steps, on the stream the launch runs on), the
-roofline of the dominant kernel against 8 TB/s, the FP64 per-realization GPU
+roofline of the dominant kernel against 8 TB/s and against the VALU issue
+peak (its launch time without the two-stream overlap), the FP64 per-realization GPU
 path on the same grid, and the FP64 CPU oracle on a bounded sample (one
 realization) as the CPU baseline.
 """
@@ -84,6 +85,24 @@ def measured_traffic(kernel, fields_per_launch):
     return int((rec["fetch"] + rec["write"]) * fields_per_launch)
 
 
+# VALU issue peak of MI355X without packed FP32 math (which this build keeps
+# off): one wave64 VALU instruction per CU per cycle, 256 CUs at 2.4 GHz
+# (= 78.6 TFLOP/s of FP32 FMA; the 157.3 TFLOP/s vector figure counts packed
+# FP32 instructions).
+VALU_PEAK = 256 * 2.4e9
+
+
+def valu_per_field(kernel):
+    """wave64 VALU instructions per field of `kernel` (SQ_INSTS_VALU of a
+    rocprofv3 --pmc pass over this workload, profiles/pmc_valu_config2.json);
+    None when that record is absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu_config2.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get("per_field", {}).get(kernel)
+    return None if rec is None else float(rec["SQ_INSTS_VALU"])
+
+
 def set_profiling(plan, on):
     lib = pfb.load()
     lib.pf_set_profiling.argtypes = [ctypes.c_void_p, ctypes.c_int32]
@@ -117,6 +136,18 @@ def run_gpu(steps, warmup, batch):
         print(f"bench: step {s + 1}/{steps} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # after the timed steps: one step with the kernels run one after another
+    # (no two-stream overlap), for the launch times of each kernel alone
+    prev = os.environ.get("PF_OVERLAP")
+    os.environ["PF_OVERLAP"] = "0"
+    alone = {}
+    plan.generate(batch, seed=SEED, first=(warmup + steps) * batch, transform="logNormal", out=out)
+    for name, ms in timings(plan):
+        alone.setdefault(name, []).append(ms)
+    if prev is None:
+        os.environ.pop("PF_OVERLAP")
+    else:
+        os.environ["PF_OVERLAP"] = prev
     set_profiling(plan, False)
     sample = out[0].float().log().cpu().numpy()
     stats = {"mean_log": float(sample.mean()), "var_log": float(sample.var())}
@@ -135,6 +166,7 @@ def run_gpu(steps, warmup, batch):
     del out
     torch.cuda.empty_cache()
     stats["pcie_inclusive_fields_per_s"] = round(pcie_fps, 1)
+    stats["alone"] = alone
     return elapsed, per_kernel, st, eig_s, stats, field0
 
 
@@ -313,11 +345,16 @@ def run_dctdst(batch=8, steps=1):
     out = torch.empty((batch, plan.domain), dtype=torch.float32, device="cuda")
     plan.generate(1, seed=13, transform="sign", param=0.5, out=out)
     torch.cuda.synchronize()
+    plan.set_profiling(True)
+    per_kernel = {}
     t0 = time.perf_counter()
     for s in range(steps):
         plan.generate(batch, seed=13, first=s * batch, transform="sign", param=0.5, out=out)
+        for name, ms in plan.last_timings():
+            per_kernel[name] = per_kernel.get(name, 0.0) + ms
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / steps
+    plan.set_profiling(False)
     P = 1025 ** 3
     alg = 4 * 12 * P + 4 * 1024 ** 3 + 8 * P / batch
     plan.close()
@@ -327,6 +364,7 @@ def run_dctdst(batch=8, steps=1):
             "alg_GB_per_field": round(alg / 1e9, 2),
             "alg_GBps": round(alg * batch / el / 1e9, 1),
             "frac_of_hbm": round(alg * batch / el / HBM_PEAK, 4), "eig_setup_s": round(eig_s, 2),
+            "kernel_ms_per_field": {k: round(v / (steps * batch), 3) for k, v in per_kernel.items()},
             "eig_negative": st["negative"]}
 
 
@@ -344,8 +382,8 @@ def main():
     ap.add_argument("--extra", action="store_true",
                     help="also run the uneven-batch and CG legs")
     ap.add_argument("--config5", action="store_true",
-                    help="also run the 1024^3 DCT/DST leg (kept out of --extra: two runs of it after "
-                         "the other legs ended in GPU memory faults of unidentified cause)")
+                    help="also run the 1024^3 DCT/DST leg (about a minute: the 2048^3 embedding's "
+                         "eigenvalues and 8 realizations)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine holds the whole extended grid on one "
@@ -377,6 +415,22 @@ def main():
                    "alg_GBps": round(kb[k] * fields_per_launch[k] / (avg[k] * 1e-3) / 1e9, 1)}
                for k in avg if k in kb}
     traffic = measured_traffic(dom, fields_per_launch[dom])
+    # VALU roofline of the same kernel: its instructions per launch over its
+    # launch time without the overlap (one launch of all batch fields)
+    alone = stats.pop("alone")
+    vpf = valu_per_field(dom)
+    valu_roof = None
+    if vpf is not None and dom in alone:
+        t_alone = float(np.mean(alone[dom])) * 1e-3 / a.batch  # seconds per field
+        valu_roof = {"bound": "valu", "kernel": dom, "achieved": round(vpf / t_alone / 1e9, 1),
+                     "peak": VALU_PEAK / 1e9, "unit": "G wave64 VALU instructions/s",
+                     "frac": round(vpf / t_alone / VALU_PEAK, 4),
+                     "valu_insts_per_field": int(vpf), "ms_per_field_alone": round(t_alone * 1e3, 4),
+                     "source": "SQ_INSTS_VALU per field from profiles/pmc_valu_config2.json; "
+                               "launch time from one non-overlapped step after the timed steps"}
+    for k in kernels:
+        if k in alone:
+            kernels[k]["ms_per_field_alone"] = round(float(np.mean(alone[k])) / a.batch, 4)
     fp64_fps, field64 = (None, None) if a.no_fp64 else run_gpu_fp64_single(3)
     extra = {}
     legs = []
@@ -424,6 +478,7 @@ def main():
             "frac": round(achieved / HBM_PEAK, 4),
             "traffic": traffic,
         },
+        "valu_roofline": valu_roof,
         "pipeline_alg_GBps": round(total_b * fps / 1e9, 1),
         "pipeline_frac_of_hbm": round(total_b * fps / HBM_PEAK, 4),
         "alg_bytes_per_field": int(total_b),
