@@ -68,6 +68,20 @@ enum pf_field_backend { PF_FIELD_R2C = 0, PF_FIELD_DCTDST = 1 };
 /* arithmetic of the per-realization transforms (eigenvalues are always FP64) */
 enum pf_precision { PF_FP64 = 0, PF_FP32 = 1 };
 
+/* embedding.periodization, matrix.hh:515-580: classical mirroring
+   (computeMatrixEntriesWithMirroring, :615-638) or the smooth merge embedding
+   (computeMatrixEntriesWithMerge, :652-775). fold / cofold (:791-1013) are not
+   built and are rejected with PF_EINVAL. */
+enum pf_periodization { PF_PERIOD_CLASSICAL = 0, PF_PERIOD_MERGE = 1 };
+/* embedding.sigmoid: SmoothstepSigmoid (default) or SmoothSigmoid, covariance.hh:147-290 */
+enum pf_sigmoid { PF_SIGMOID_SMOOTHSTEP = 0, PF_SIGMOID_SMOOTH = 1 };
+/* stochastic.sigmoidCombine, matrix.hh:660-671 */
+enum pf_sigmoid_combine {
+  PF_COMBINE_RADIAL_PLUS = 0, /* default */
+  PF_COMBINE_RADIAL = 1,
+  PF_COMBINE_TENSOR = 2
+};
+
 /* randomField.transform, mutators.hh:126-163 */
 enum pf_transform {
   PF_TRANSFORM_NONE = 0,
@@ -114,6 +128,15 @@ typedef struct pf_desc {
                                   buffer (matrix.hh:270-291 reading r2cmatrixbackend.hh:378-400, which
                                   covers Re and Im of the first half of the entries only); 1 = check
                                   and clamp the real part of every entry (not reference behaviour) */
+  /* Embedding (matrix.hh:515-775). A zeroed tail gives the reference defaults. */
+  int32_t periodization;       /* enum pf_periodization */
+  int32_t sigmoid;             /* enum pf_sigmoid (merge only) */
+  int32_t sigmoid_combine;     /* enum pf_sigmoid_combine (merge only) */
+  int32_t merge_recursions;    /* embedding.mergeRecursions: 0 = the reference default (its value 99:
+                                  a count per covariance family, matrix.hh:679-705), k > 0, or -1 for
+                                  zero recursions */
+  double sigmoid_start;        /* embedding.sigmoidStart; 0 = default 1 */
+  double sigmoid_end;          /* embedding.sigmoidEnd; 0 = default factor - 1 */
 } pf_desc;
 
 /* eigenvalue classification, matrix.hh:266-305 */

@@ -131,6 +131,37 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   d.cg_iterations = c.get<int>("randomField.cgIterations", 100);
   d.nugget = 0.;
   d.r2c_check_all = 0;
+  // embedding.periodization and the merge taper (matrix.hh:515-580, :652-775)
+  const std::string per = c.get<std::string>("embedding.periodization", "classical");
+  if (per == "classical")
+    d.periodization = PF_PERIOD_CLASSICAL;
+  else if (per == "merge")
+    d.periodization = PF_PERIOD_MERGE;
+  else if (per == "fold" || per == "cofold")
+    throw std::runtime_error("embedding.periodization \"" + per + "\" is not built");
+  else
+    throw std::runtime_error(
+      "stochastic.periodization must be \"classical\", \"merge\", \"fold\", or \"cofold\"");
+  const std::string sg = c.get<std::string>("embedding.sigmoid", "smoothstep");
+  if (sg == "smoothstep")
+    d.sigmoid = PF_SIGMOID_SMOOTHSTEP;
+  else if (sg == "smooth")
+    d.sigmoid = PF_SIGMOID_SMOOTH;
+  else
+    throw std::runtime_error("embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
+  const std::string sc = c.get<std::string>("stochastic.sigmoidCombine", "radialPlus");
+  if (sc == "radialPlus")
+    d.sigmoid_combine = PF_COMBINE_RADIAL_PLUS;
+  else if (sc == "radial")
+    d.sigmoid_combine = PF_COMBINE_RADIAL;
+  else if (sc == "tensor")
+    d.sigmoid_combine = PF_COMBINE_TENSOR;
+  else
+    throw std::runtime_error("sigmoidCombine must be tensor, radial or radialPlus");
+  const int rec = c.get<int>("embedding.mergeRecursions", 99);
+  d.merge_recursions = rec == 99 ? 0 : (rec == 0 ? -1 : rec);
+  d.sigmoid_start = c.get<double>("embedding.sigmoidStart", 1.);
+  d.sigmoid_end = c.hasKey("embedding.sigmoidEnd") ? c.get<double>("embedding.sigmoidEnd") : 0.;
   return d;
 }
 
@@ -206,6 +237,8 @@ public:
   template <typename Covariance>
   pf_eig_stats fillTransformedMatrix(Covariance&& covariance) const
   {
+    if (desc_.periodization != PF_PERIOD_CLASSICAL)
+      throw std::runtime_error("a custom covariance is evaluated with the classical embedding only");
     const int f = desc_.periodic ? 1 : desc_.embedding_factor;
     std::array<long, 3> N{ 1, 1, 1 }, cnt{ 1, 1, 1 };
     for (int i = 0; i < dim; i++)

@@ -283,6 +283,13 @@ Geo make_geo(const pf_desc& d)
   g.factor = d.periodic ? 1 : d.embedding_factor;
   if (g.factor < 1)
     throw std::runtime_error("embedding.factor must be >= 1");
+  if (d.periodization != PF_PERIOD_CLASSICAL && d.periodization != PF_PERIOD_MERGE)
+    throw std::runtime_error(
+      "stochastic.periodization must be \"classical\" or \"merge\" (fold and cofold are not built)");
+  if (d.sigmoid != PF_SIGMOID_SMOOTHSTEP && d.sigmoid != PF_SIGMOID_SMOOTH)
+    throw std::runtime_error("embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
+  if (d.sigmoid_combine < PF_COMBINE_RADIAL_PLUS || d.sigmoid_combine > PF_COMBINE_TENSOR)
+    throw std::runtime_error("sigmoidCombine must be tensor, radial or radialPlus");
   g.Next = 1;
   g.domain = 1;
   for (int i = 0; i < 3; i++) {
@@ -562,17 +569,12 @@ struct Covariance
   }
 };
 
-// Entry value of the embedding grid at integer position idx:
-// Matrix::computeMatrixEntriesWithMirroring, matrix.hh:615-638.
-double embedded_cov(const Geo& g, const Geometry& geo, const Covariance& cov, const pf_desc& d,
-                    const Index idx[3])
+// Covariance at the physical offset coord (geometry transform, family,
+// nugget at the origin).
+double cov_at(const Geo& g, const Geometry& geo, const Covariance& cov, const pf_desc& d,
+              const double coord[3])
 {
-  double coord[3] = { 0, 0, 0 }, tc[3] = { 0, 0, 0 };
-  for (int i = 0; i < g.dim; i++) {
-    coord[i] = (double)idx[i] * g.h[i];
-    if (coord[i] > 0.5 * g.ext[i] * (double)g.factor)
-      coord[i] -= g.ext[i] * (double)g.factor;
-  }
+  double tc[3] = { 0, 0, 0 };
   geo.transform(coord, tc);
   double v = cov(tc);
   if (d.nugget != 0.) {
@@ -584,6 +586,155 @@ double embedded_cov(const Geo& g, const Geometry& geo, const Covariance& cov, co
       v += d.nugget;
   }
   return v;
+}
+
+// Sigmoids of the merge embedding, covariance.hh:147-290: value 1 for |x|
+// below oneEdge, 0 beyond zeroEdge.
+// ClampingSigmoid (covariance.hh:204-234): linear in between.
+double sigmoid_clamp(double one, double zero, double x)
+{
+  if (std::fabs(x) < one)
+    return 1.;
+  if (std::fabs(x) > zero)
+    return 0.;
+  return (zero - std::fabs(x)) / (zero - one);
+}
+
+// SmoothSigmoid (covariance.hh:147-196): theta(x1) / (theta(x1) + theta(x2)),
+// theta(y) = exp(-1/y) for y > 0.
+double sigmoid_smooth(double one, double zero, double x)
+{
+  if (std::fabs(x) < one)
+    return 1.;
+  if (std::fabs(x) > zero)
+    return 0.;
+  const double x1 = (zero - std::fabs(x)) / (zero - one);
+  const double x2 = (std::fabs(x) - one) / (zero - one);
+  auto theta = [](double y) { return y > 0. ? std::exp(-1. / y) : 0.; };
+  return theta(x1) / (theta(x1) + theta(x2));
+}
+
+// SmoothstepSigmoid (covariance.hh:246-290): Hermite interpolation matching
+// `rec` derivatives, built by the reference's recursion over
+// (1 - x2^2)^i with x2 = 2 clamp - 1.
+double sigmoid_smoothstep(double one, double zero, double x, unsigned rec)
+{
+  const double x2 = 2. * sigmoid_clamp(one, zero, x) - 1.;
+  double norm = 1., add = 1., out = 1.;
+  for (unsigned i = 1; i <= rec; i++) {
+    out *= 2. * i;
+    add *= 1 - x2 * x2;
+    out += add;
+    out /= 2. * i + 1.;
+    norm *= 2. * i / (2. * i + 1.);
+  }
+  out *= x2 / norm;
+  out += 1.;
+  out /= 2.;
+  return out;
+}
+
+// embedding.mergeRecursions: the reference's value 99 (the default) picks a
+// count per covariance family, matrix.hh:679-705.
+unsigned merge_recursions(const pf_desc& d)
+{
+  if (d.merge_recursions > 0)
+    return (unsigned)d.merge_recursions;
+  if (d.merge_recursions < 0)
+    return 0;
+  const double half_dim = (double)d.dim / 2.;
+  switch (d.covariance) {
+    case PF_COV_MATERN:
+      return (unsigned)std::floor(d.matern_nu * 2. + half_dim);
+    case PF_COV_GAMMA_EXPONENTIAL:
+      return (unsigned)std::floor(d.exp_gamma + half_dim);
+    case PF_COV_GENERALIZED_CAUCHY:
+      return (unsigned)std::floor(d.cauchy_alpha + half_dim);
+    case PF_COV_EXPONENTIAL:
+      return 2;
+    case PF_COV_MATERN32:
+      return 4;
+    case PF_COV_MATERN52:
+      return 6;
+    case PF_COV_CAUCHY:
+      return 3;
+    default:
+      throw std::runtime_error(
+        "magic recursions value 99 not implemented for choice of covariance");
+  }
+}
+
+// Entry of the merge embedding at integer position idx (no wrap):
+// Matrix::computeMatrixEntriesWithMerge, matrix.hh:652-775. The 2^dim mirror
+// images x_i -> ext_i * factor - x_i of the position each contribute the
+// covariance times a sigmoid taper; what the tapers leave (remainder) takes
+// the covariance at the centre of the extended domain (radialPlus) or 0.
+double merged_cov(const Geo& g, const Geometry& geo, const Covariance& cov, const pf_desc& d,
+                  const Index idx[3])
+{
+  const int dim = g.dim;
+  const unsigned factor = (unsigned)g.factor;
+  const unsigned rec = merge_recursions(d);
+  const int combine = d.sigmoid_combine;
+  const bool radial = combine == PF_COMBINE_RADIAL || combine == PF_COMBINE_RADIAL_PLUS;
+  const double start = d.sigmoid_start != 0. ? d.sigmoid_start : 1.;
+  const double end = d.sigmoid_end != 0. ? d.sigmoid_end : (double)factor - 1.;
+  auto sig = [&](double one, double zero, double x) {
+    return d.sigmoid == PF_SIGMOID_SMOOTH ? sigmoid_smooth(one, zero, x)
+                                          : sigmoid_smoothstep(one, zero, x, rec);
+  };
+  double constValue = 0.;
+  if (combine == PF_COMBINE_RADIAL_PLUS) {
+    double c[3] = { 0, 0, 0 }, tc[3] = { 0, 0, 0 };
+    for (int i = 0; i < dim; i++)
+      c[i] = g.ext[i] * (double)factor / 2.;
+    geo.transform(c, tc);
+    constValue = cov(tc);
+  }
+  double trueCoord[3] = { 0, 0, 0 };
+  for (int i = 0; i < dim; i++)
+    trueCoord[i] = (double)idx[i] * g.h[i];
+  double value = 0., remainder = 1.;
+  const double eps = 1e-10;
+  const double sqrtDim = std::sqrt((double)dim);
+  for (unsigned j = 0; j < (1u << dim); j++) {
+    double mirror[3] = { 0, 0, 0 };
+    for (int i = 0; i < dim; i++)
+      mirror[i] = (j & (1u << i)) ? g.ext[i] * (double)factor - trueCoord[i] : trueCoord[i];
+    double damp = 1.;
+    if (radial && sqrtDim < factor / 2.) {
+      double radius = 0.;
+      for (int i = 0; i < dim; i++)
+        radius += (mirror[i] * mirror[i]) / (g.ext[i] * g.ext[i]);
+      radius = std::sqrt(radius);
+      damp = sig(start * sqrtDim - eps, factor - (factor - end) * sqrtDim + eps, radius);
+    } else {
+      for (int i = 0; i < dim; i++)
+        damp *= sig(g.ext[i] * start - eps, g.ext[i] * end + eps, mirror[i]);
+    }
+    if (damp < eps)
+      continue;
+    remainder -= damp;
+    value += cov_at(g, geo, cov, d, mirror) * damp;
+  }
+  return value + constValue * remainder;
+}
+
+// Entry value of the embedding grid at integer position idx:
+// Matrix::computeMatrixEntriesWithMirroring, matrix.hh:615-638 (classical),
+// or the merge embedding.
+double embedded_cov(const Geo& g, const Geometry& geo, const Covariance& cov, const pf_desc& d,
+                    const Index idx[3])
+{
+  if (d.periodization == PF_PERIOD_MERGE)
+    return merged_cov(g, geo, cov, d, idx);
+  double coord[3] = { 0, 0, 0 };
+  for (int i = 0; i < g.dim; i++) {
+    coord[i] = (double)idx[i] * g.h[i];
+    if (coord[i] > 0.5 * g.ext[i] * (double)g.factor)
+      coord[i] -= g.ext[i] * (double)g.factor;
+  }
+  return cov_at(g, geo, cov, d, coord);
 }
 
 // ----------------------------------------------------------------------------
@@ -1486,6 +1637,40 @@ int pfo_cov_eval(const pf_desc* d, const double* x, long long n, double* out)
 }
 
 double pfo_bessel_k(double nu, double x) { return bessel_k(nu, x); }
+
+// Embedding grid before the forward transform (classical or merge), natural
+// x-fastest layout: the DCT half grid (N_i/2+1 per axis) or the full
+// extended grid (R2C matrix).
+int pfo_cov_grid(const pf_desc* d, double* out)
+{
+  PFO_TRY({
+    Geo g = make_geo(*d);
+    Geometry geo(*d);
+    Covariance cov(*d);
+    Index C[3];
+    for (int i = 0; i < 3; i++)
+      C[i] = i >= g.dim ? 1 : (d->matrix_backend == PF_MATRIX_DCT ? g.N[i] / 2 + 1 : g.N[i]);
+    const Index total = C[0] * C[1] * C[2];
+    parallel_for(total, [&](Index b, Index en) {
+      for (Index f = b; f < en; f++) {
+        Index idx[3] = { f % C[0], (f / C[0]) % C[1], f / (C[0] * C[1]) };
+        out[f] = embedded_cov(g, geo, cov, *d, idx);
+      }
+    });
+  });
+  return PF_OK;
+}
+
+// Sigmoids of the merge embedding, for the tests: kind 0 clamping, 1 smooth,
+// 2 smoothstep with `rec` recursions.
+double pfo_sigmoid(int kind, double one, double zero, double x, int rec)
+{
+  if (kind == 0)
+    return sigmoid_clamp(one, zero, x);
+  if (kind == 1)
+    return sigmoid_smooth(one, zero, x);
+  return sigmoid_smoothstep(one, zero, x, (unsigned)rec);
+}
 // Learned-PCG: preconditioner R F^{-1} diag(exp(L_bin)) F E.
 int pfo_pcg_learned(const pf_desc* d, const double* log_mult, int nbins, int maxit, double rel_tol,
                     const double* b, double* x, int* iterations, double* relres)

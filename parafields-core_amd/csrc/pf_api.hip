@@ -325,6 +325,37 @@ void check_launch()
     throw PfError(PF_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
 }
 
+// embedding.mergeRecursions (smoothstep taper of the merge embedding): the
+// reference's value 99, its default, picks a count per covariance family
+// (matrix.hh:679-705) and throws for the others.
+int merge_recursions(const pf_desc& d)
+{
+  if (d.merge_recursions > 0)
+    return d.merge_recursions;
+  if (d.merge_recursions < 0)
+    return 0;
+  const double half_dim = (double)d.dim / 2.;
+  switch (d.covariance) {
+    case PF_COV_MATERN:
+      return (int)std::floor(d.matern_nu * 2. + half_dim);
+    case PF_COV_GAMMA_EXPONENTIAL:
+      return (int)std::floor(d.exp_gamma + half_dim);
+    case PF_COV_GENERALIZED_CAUCHY:
+      return (int)std::floor(d.cauchy_alpha + half_dim);
+    case PF_COV_EXPONENTIAL:
+      return 2;
+    case PF_COV_MATERN32:
+      return 4;
+    case PF_COV_MATERN52:
+      return 6;
+    case PF_COV_CAUCHY:
+      return 3;
+    default:
+      throw PfError(PF_EINVAL,
+                    "magic recursions value 99 not implemented for choice of covariance");
+  }
+}
+
 Geo make_geo(const pf_desc& d)
 {
   Geo g{};
@@ -389,6 +420,15 @@ Geo make_geo(const pf_desc& d)
     throw PfError(PF_EINVAL, "stochastic.anisotropy must be none, axiparallel or geometric");
   if (d.precision != PF_FP64 && d.precision != PF_FP32)
     throw PfError(PF_EINVAL, "precision must be PF_FP64 or PF_FP32");
+  if (d.periodization != PF_PERIOD_CLASSICAL && d.periodization != PF_PERIOD_MERGE)
+    throw PfError(PF_EINVAL, "stochastic.periodization must be \"classical\" or \"merge\" (fold "
+                             "and cofold are not built)");
+  if (d.sigmoid != PF_SIGMOID_SMOOTHSTEP && d.sigmoid != PF_SIGMOID_SMOOTH)
+    throw PfError(PF_EINVAL, "embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
+  if (d.sigmoid_combine < PF_COMBINE_RADIAL_PLUS || d.sigmoid_combine > PF_COMBINE_TENSOR)
+    throw PfError(PF_EINVAL, "sigmoidCombine must be tensor, radial or radialPlus");
+  if (d.periodization == PF_PERIOD_MERGE)
+    (void)merge_recursions(d); // rejects families without a default count
   const int64_t half = (int64_t)g.H * g.N[1] * g.N[2];
   const int64_t P = (int64_t)g.P[0] * g.P[1] * g.P[2];
   g.half = half;
@@ -454,6 +494,18 @@ CovParams make_cov(const pf_desc& d, const Geo& g)
     p.wrap_len[i] = g.ext[i] * (double)g.factor;
   }
   p.factor = g.factor;
+  p.merge = d.periodization == PF_PERIOD_MERGE;
+  if (p.merge) {
+    p.sigmoid = d.sigmoid;
+    p.radial = d.sigmoid_combine != PF_COMBINE_TENSOR &&
+               std::sqrt((double)d.dim) < (double)g.factor / 2.;
+    p.radial_plus = d.sigmoid_combine == PF_COMBINE_RADIAL_PLUS;
+    p.recursions = merge_recursions(d);
+    p.sig_start = d.sigmoid_start != 0. ? d.sigmoid_start : 1.;
+    p.sig_end = d.sigmoid_end != 0. ? d.sigmoid_end : (double)g.factor - 1.;
+    for (int i = 0; i < 3; i++)
+      p.ext[i] = g.ext[i];
+  }
   return p;
 }
 

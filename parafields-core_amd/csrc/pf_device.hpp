@@ -186,6 +186,14 @@ struct CovParams
   // embedding grid geometry (matrix.hh:615-638)
   double h[3], wrap_half[3], wrap_len[3];
   int factor;
+  // merge embedding (matrix.hh:652-775): taper kind, combination, recursion
+  // count, taper interval in units of the extensions, extensions
+  int merge;          // 0: classical mirroring
+  int sigmoid;        // enum pf_sigmoid
+  int radial;         // radial taper (combine radial/radialPlus and sqrt(dim) < factor / 2)
+  int radial_plus;    // the remainder takes the covariance at the centre of the extended domain
+  int recursions;
+  double sig_start, sig_end, ext[3];
 };
 
 // Modified Bessel K_nu(x) (Temme series for x < 2, Steed's CF2 otherwise,
@@ -260,20 +268,14 @@ __device__ inline double bessel_k(double nu, double x)
   return rkmu;
 }
 
-// Covariance at the embedding grid position idx (integer cell offsets):
-// coordinates idx*h wrapped by the classical rule, geometry transform,
-// covariance functor (+ nugget at the origin).
+// Covariance at the physical offset coord: geometry transform, covariance
+// functor (+ nugget at the origin).
 // COV >= 0: the family is fixed at compile time (p.covariance is not read).
 template <int COV = -1>
-__device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
+__device__ inline double cov_at(const CovParams& p, const double coord[3])
 {
 #pragma clang fp contract(off)
-  double coord[3] = { 0., 0., 0. }, t[3] = { 0., 0., 0. };
-  for (int i = 0; i < p.dim; i++) {
-    coord[i] = (double)idx[i] * p.h[i];
-    if (coord[i] > p.wrap_half[i])
-      coord[i] -= p.wrap_len[i];
-  }
+  double t[3] = { 0., 0., 0. };
   if (p.aniso == PF_ANISO_NONE) {
     for (int i = 0; i < p.dim; i++)
       t[i] = p.m[0] * coord[i];
@@ -363,6 +365,112 @@ __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
       v += p.nugget;
   }
   return v;
+}
+
+// Sigmoid tapers of the merge embedding (covariance.hh:147-290): 1 for |x|
+// below `one`, 0 beyond `zero`. Clamping: linear in between; smooth:
+// theta(x1) / (theta(x1) + theta(x2)) with theta(y) = exp(-1/y); smoothstep:
+// the Hermite interpolation matching `rec` derivatives (the regularized
+// incomplete beta function I_t(rec+1, rec+1) of the clamped value t), by the
+// reference's recursion.
+__device__ inline double sigmoid_clamp(double one, double zero, double x)
+{
+#pragma clang fp contract(off)
+  const double ax = fabs(x);
+  return ax < one ? 1. : (ax > zero ? 0. : (zero - ax) / (zero - one));
+}
+
+__device__ inline double sigmoid_taper(const CovParams& p, double one, double zero, double x)
+{
+#pragma clang fp contract(off)
+  if (p.sigmoid == PF_SIGMOID_SMOOTH) {
+    const double ax = fabs(x);
+    if (ax < one)
+      return 1.;
+    if (ax > zero)
+      return 0.;
+    const double x1 = (zero - ax) / (zero - one), x2 = (ax - one) / (zero - one);
+    const double t1 = x1 > 0. ? exp(-1. / x1) : 0., t2 = x2 > 0. ? exp(-1. / x2) : 0.;
+    return t1 / (t1 + t2);
+  }
+  const double x2 = 2. * sigmoid_clamp(one, zero, x) - 1.;
+  double norm = 1., add = 1., out = 1.;
+  for (int i = 1; i <= p.recursions; i++) {
+    out *= 2. * i;
+    add *= 1 - x2 * x2;
+    out += add;
+    out /= 2. * i + 1.;
+    norm *= 2. * i / (2. * i + 1.);
+  }
+  out *= x2 / norm;
+  out += 1.;
+  out /= 2.;
+  return out;
+}
+
+// Entry of the merge embedding at grid position idx (no wrap),
+// matrix.hh:652-775: the 2^dim mirror images x_i -> ext_i * factor - x_i each
+// contribute the covariance times a taper (radial in the extension-scaled
+// norm, or the product of per-axis tapers); what the tapers leave takes the
+// covariance at the centre of the extended domain (radialPlus) or nothing.
+template <int COV = -1>
+__device__ inline double merged_cov(const CovParams& p, const int64_t idx[3])
+{
+#pragma clang fp contract(off)
+  const double eps = 1e-10;
+  const double sqrt_dim = sqrt((double)p.dim);
+  const double factor = (double)p.factor;
+  double value = 0., remainder = 1.;
+  for (int j = 0; j < (1 << p.dim); j++) {
+    double mirror[3] = { 0., 0., 0. };
+    for (int i = 0; i < p.dim; i++) {
+      const double x = (double)idx[i] * p.h[i];
+      mirror[i] = ((j >> i) & 1) ? p.ext[i] * factor - x : x;
+    }
+    double damp = 1.;
+    if (p.radial) {
+      double r = 0.;
+      for (int i = 0; i < p.dim; i++)
+        r += (mirror[i] * mirror[i]) / (p.ext[i] * p.ext[i]);
+      r = sqrt(r);
+      damp = sigmoid_taper(p, p.sig_start * sqrt_dim - eps,
+                           factor - (factor - p.sig_end) * sqrt_dim + eps, r);
+    } else {
+      for (int i = 0; i < p.dim; i++)
+        damp *= sigmoid_taper(p, p.ext[i] * p.sig_start - eps, p.ext[i] * p.sig_end + eps,
+                              mirror[i]);
+    }
+    if (damp < eps)
+      continue;
+    remainder -= damp;
+    value += cov_at<COV>(p, mirror) * damp;
+  }
+  if (p.radial_plus) {
+    double c[3] = { 0., 0., 0. };
+    for (int i = 0; i < p.dim; i++)
+      c[i] = p.ext[i] * factor / 2.;
+    // the centre value without a nugget (the offset is never the origin)
+    value += cov_at<COV>(p, c) * remainder;
+  }
+  return value;
+}
+
+// Covariance at the embedding grid position idx (integer cell offsets):
+// coordinates idx*h wrapped by the classical rule (matrix.hh:615-638), or the
+// merge embedding.
+template <int COV = -1>
+__device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
+{
+#pragma clang fp contract(off)
+  if (p.merge)
+    return merged_cov<COV>(p, idx);
+  double coord[3] = { 0., 0., 0. };
+  for (int i = 0; i < p.dim; i++) {
+    coord[i] = (double)idx[i] * p.h[i];
+    if (coord[i] > p.wrap_half[i])
+      coord[i] -= p.wrap_len[i];
+  }
+  return cov_at<COV>(p, coord);
 }
 
 // Value transforms of mutators.hh:35-163 (sign threshold is a parameter,

@@ -33,6 +33,9 @@ ANISOTROPY = {"none": 0, "axiparallel": 1, "geometric": 2}
 MATRIX_BACKENDS = {"dct": 0, "r2c": 1}
 FIELD_BACKENDS = {"r2c": 0, "dctdst": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
+PERIODIZATION = {"classical": 0, "merge": 1}
+SIGMOID = {"smoothstep": 0, "smooth": 1}
+SIGMOID_COMBINE = {"radialPlus": 0, "radial": 1, "tensor": 2}
 TRANSFORMS = {"none": 0, "logNormal": 1, "foldedNormal": 2, "sign": 3, "boxCox": 4}
 OPS = {"multiply": 0, "root": 1, "inverse_embed": 2}
 CG_MODES = {"reference": 0, "plain": 1, "embed_pcg": 2, "learned_pcg": 3}
@@ -67,6 +70,12 @@ class PfDesc(ctypes.Structure):
         ("cg_iterations", ctypes.c_int32),
         ("nugget", ctypes.c_double),
         ("r2c_check_all", ctypes.c_int32),
+        ("periodization", ctypes.c_int32),
+        ("sigmoid", ctypes.c_int32),
+        ("sigmoid_combine", ctypes.c_int32),
+        ("merge_recursions", ctypes.c_int32),
+        ("sigmoid_start", ctypes.c_double),
+        ("sigmoid_end", ctypes.c_double),
     ]
 
 
@@ -122,10 +131,17 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
               threshold: float = 1e-14, precision: str = "fp64", cg_iterations: int = 100,
               nugget: float = 0.0, matern_nu: float = 1.0, exp_gamma: float = 1.0,
               cauchy_alpha: float = 1.0, cauchy_beta: float = 1.0,
-              r2c_check_all: bool = False) -> PfDesc:
+              r2c_check_all: bool = False, periodization: str = "classical",
+              sigmoid: str = "smoothstep", sigmoid_combine: str = "radialPlus",
+              merge_recursions: int = 0, sigmoid_start: float = 0.0,
+              sigmoid_end: float = 0.0) -> PfDesc:
     """Descriptor with the reference's defaults. `matrix=None` picks what
     RandomField would (randomfield.hh:153-164): DCT (DefaultIsoMatrix) for
-    none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy."""
+    none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy.
+    periodization / sigmoid / sigmoid_combine take the reference's names
+    (embedding.periodization, embedding.sigmoid, stochastic.sigmoidCombine);
+    merge_recursions 0 and sigmoid_start / sigmoid_end 0 mean the reference
+    defaults, merge_recursions -1 means zero recursions."""
     d = PfDesc()
     d.dim = dim
     for i in range(3):
@@ -152,6 +168,11 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
     d.cg_iterations = cg_iterations
     d.nugget = nugget
     d.r2c_check_all = int(r2c_check_all)
+    d.periodization = PERIODIZATION[periodization]
+    d.sigmoid = SIGMOID[sigmoid]
+    d.sigmoid_combine = SIGMOID_COMBINE[sigmoid_combine]
+    d.merge_recursions = merge_recursions
+    d.sigmoid_start, d.sigmoid_end = sigmoid_start, sigmoid_end
     return d
 
 

@@ -46,6 +46,8 @@ def load():
         "pfo_philox_raw": (i32, [vp, vp, ctypes.c_longlong, vp]),
         "pfo_cov_eval": (i32, [D, vp, ctypes.c_longlong, vp]),
         "pfo_bessel_k": (dp, [dp, dp]),
+        "pfo_cov_grid": (i32, [D, vp]),
+        "pfo_sigmoid": (dp, [i32, dp, dp, dp, i32]),
         "pfo_pcg_learned": (i32, [D, vp, i32, i32, dp, vp, vp, ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_double)]),
         "pfo_spectral_targets": (i32, [D, i32, dp, vp]),
@@ -157,6 +159,24 @@ def cov_eval(desc, pts):
     out = np.empty(p.shape[0])
     _check(load().pfo_cov_eval(ctypes.byref(desc), p.ctypes.data, p.shape[0], out.ctypes.data))
     return out
+
+
+def cov_grid(desc):
+    """Embedding grid before the forward transform (classical or merge),
+    natural layout [C2][C1][C0]: the DCT half grid or the full extended
+    grid (R2C matrix)."""
+    dim = desc.dim
+    N = [desc.cells[i] * (1 if desc.periodic else desc.embedding_factor) for i in range(dim)]
+    C = [n // 2 + 1 for n in N] if desc.matrix_backend == pfb.MATRIX_BACKENDS["dct"] else N
+    out = np.empty(int(np.prod(C)))
+    _check(load().pfo_cov_grid(ctypes.byref(desc), out.ctypes.data))
+    return out.reshape(C[::-1])
+
+
+def sigmoid(kind, one, zero, x, rec=0):
+    """kind: 'clamp', 'smooth' or 'smoothstep' (covariance.hh:147-290)."""
+    k = {"clamp": 0, "smooth": 1, "smoothstep": 2}[kind]
+    return load().pfo_sigmoid(k, one, zero, x, rec)
 
 
 def bessel_k(nu, x):

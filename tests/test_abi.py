@@ -5,6 +5,8 @@ import glob
 import os
 import re
 
+import pytest
+
 import pf_binding as pfb
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -37,8 +39,30 @@ def test_binding_names_are_exported():
         assert hasattr(lib, n), n
 
 
-def test_desc_struct_layout_matches_header():
-    # offsets of the C struct as compiled by the C compiler
-    assert ctypes.sizeof(pfb.PfDesc) == 224
-    assert ctypes.sizeof(pfb.PfEigStats) == 40
-    assert ctypes.sizeof(pfb.PfSizes) == 64
+def test_desc_struct_layout_matches_header(tmp_path):
+    """The ctypes mirrors of the C structs have the size and field offsets
+    the C compiler gives the header's declarations."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"pf_desc": pfb.PfDesc, "pf_eig_stats": pfb.PfEigStats, "pf_sizes": pfb.PfSizes}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "parafields_amd.h"',
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)], text=True).splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)

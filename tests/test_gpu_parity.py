@@ -84,6 +84,62 @@ def test_eigs_match_oracle(dim, cells, kw):
     assert st["smallest"] == pytest.approx(wst["smallest"], rel=1e-9, abs=1e-15)
 
 
+# merge embedding (matrix.hh:652-775): DCT and R2C matrices, tensor / radial /
+# radialPlus combinations (radial tapers need sqrt(d) < factor / 2), both
+# sigmoids, default and explicit recursion counts
+MERGE_CASES = [
+    (2, [64, 48], dict(covariance="matern52", corr_length=0.25, factor=3,
+                       sigmoid_combine="tensor")),
+    (3, [16, 12, 8], dict(covariance="matern32", anisotropy="axiparallel",
+                          corr_length=[0.3, 0.2, 0.25], factor=3)),
+    (2, [32, 24], dict(covariance="exponential", corr_length=0.3, factor=4, sigmoid="smooth")),
+    (2, [24, 20], dict(covariance="gaussian", anisotropy="geometric",
+                       corr_length=[0.2, 0.1, 0.05, 0.3], factor=3, merge_recursions=3,
+                       sigmoid_combine="radial")),
+    (3, [8, 8, 6], dict(covariance="matern", matern_nu=1.2, corr_length=0.3, factor=3,
+                        matrix="r2c")),
+]
+
+
+@pytest.mark.parametrize("dim,cells,kw", MERGE_CASES)
+def test_merge_eigs_match_oracle(dim, cells, kw):
+    d = desc(dim, cells, periodization="merge", approximate=True, **kw)
+    p = pfb.Plan(d)
+    status, st = p.eigs_stats_nothrow()
+    want, wst, wstatus = orc.eigs(d, allow_negative=True)
+    assert status == wstatus
+    assert rel_max(p.eig_values(), want) < FP64_TOL
+    for k in ("small", "zero", "negative", "entries"):
+        assert st[k] == wst[k], (k, st, wst)
+
+
+def test_merge_generate_matches_oracle():
+    """A field on the merge embedding (factor 3, no negative eigenvalues left)
+    from host noise and from device Philox noise, FP64 and FP32."""
+    kw = dict(covariance="matern52", corr_length=0.25, factor=3, periodization="merge",
+              sigmoid_combine="tensor")
+    d = desc(2, [64, 48], **kw)
+    p = pfb.Plan(d)
+    st = p.eigs()
+    assert st["negative"] == 0
+    npf = int(p.sizes.noise_per_field)
+    noise = orc.noise_cpp(23, npf)
+    assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < FP64_TOL
+    p32 = pfb.Plan(desc(2, [64, 48], precision="fp32", **kw))
+    got = p32.generate(1, seed=4)[0]
+    want = orc.generate(d, orc.noise_philox(4, 0, 0, npf))
+    assert np.abs(got - want).max() / want.std() < FP32_TOL
+
+
+def test_merge_rejects_unbuilt_periodizations():
+    with pytest.raises(RuntimeError):
+        pfb.Plan(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="merge"))
+    d = desc(2, [8, 8], factor=3)
+    d.periodization = 2  # fold is not built
+    with pytest.raises(RuntimeError):
+        pfb.Plan(d)
+
+
 R2C_CHECK_ALL_CASES = [
     (2, [48, 96], dict(covariance="gaussian", anisotropy="geometric",
                        corr_length=[0.08, 0.02, -0.01, 0.03], approximate=True)),

@@ -443,3 +443,116 @@ def test_learned_pcg_with_method_targets_solves_dense_system():
     assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-7
     _, it_plain, _ = orc.pcg(d, b, mode="plain", maxit=500, rel_tol=1e-10)
     assert it < it_plain
+
+
+# --- merge embedding (matrix.hh:652-775, covariance.hh:147-290) -------------
+
+@pytest.mark.parametrize("k", range(7))
+def test_smoothstep_sigmoid_is_regularized_incomplete_beta(k):
+    """The smoothstep taper with k recursions is I_t(k+1, k+1) of the clamped
+    value t (the Hermite interpolation matching k derivatives: k = 1 gives
+    3t^2 - 2t^3, k = 2 gives 6t^5 - 15t^4 + 10t^3)."""
+    one, zero = 0.3, 1.7
+    for x in np.linspace(0.0, 2.0, 41):
+        t = 1.0 if abs(x) < one else 0.0 if abs(x) > zero else (zero - abs(x)) / (zero - one)
+        assert orc.sigmoid("smoothstep", one, zero, x, k) == pytest.approx(
+            special.betainc(k + 1, k + 1, t), abs=1e-14)
+        assert orc.sigmoid("smoothstep", one, zero, -x, k) == orc.sigmoid(
+            "smoothstep", one, zero, x, k)
+    assert orc.sigmoid("clamp", one, zero, 1.0) == pytest.approx(0.5)
+
+
+def test_smooth_sigmoid_shape():
+    """C-infinity taper: 1 below the one edge, 0 beyond the zero edge,
+    s(x) + s(one + zero - x) = 1 in between (so 1/2 at the midpoint),
+    decreasing."""
+    one, zero = 0.5, 2.0
+    assert orc.sigmoid("smooth", one, zero, 0.2) == 1.0
+    assert orc.sigmoid("smooth", one, zero, 2.5) == 0.0
+    xs = np.linspace(one, zero, 31)
+    v = np.array([orc.sigmoid("smooth", one, zero, x) for x in xs])
+    w = np.array([orc.sigmoid("smooth", one, zero, one + zero - x) for x in xs])
+    np.testing.assert_allclose(v + w, 1.0, atol=1e-15)
+    assert orc.sigmoid("smooth", one, zero, 1.25) == pytest.approx(0.5, abs=1e-15)
+    assert np.all(np.diff(v) <= 0)
+
+
+MERGE_CASES = [
+    (2, [12, 10], dict(covariance="matern52", corr_length=0.25, factor=3, sigmoid_combine="tensor")),
+    (2, [12, 10], dict(covariance="exponential", corr_length=0.3, factor=4, sigmoid="smooth")),
+    (3, [6, 5, 4], dict(covariance="matern32", corr_length=[0.3, 0.2, 0.25],
+                        anisotropy="axiparallel", factor=3)),
+    (2, [10, 8], dict(covariance="gaussian", corr_length=[0.2, 0.1, 0.05, 0.3],
+                      anisotropy="geometric", factor=3, merge_recursions=3,
+                      sigmoid_combine="radial")),
+]
+
+
+@pytest.mark.parametrize("dim,cells,kw", MERGE_CASES)
+@pytest.mark.parametrize("matrix", ["dct", "r2c"])
+def test_merge_keeps_the_covariance_on_the_domain(dim, cells, kw, matrix):
+    """Inside the original domain (every x_i < ext_i; for a radial taper also
+    |x / ext| < sigmoidStart sqrt(d)) the merged entry is the covariance
+    itself: the unmirrored copy has weight 1 and every mirror image weight 0
+    (matrix.hh:652-775)."""
+    d = desc(dim, cells, matrix=matrix, approximate=True, periodization="merge", **kw)
+    c = desc(dim, cells, matrix=matrix, approximate=True, **kw)
+    gm, gc = orc.cov_grid(d), orc.cov_grid(c)
+    idx = np.stack(np.meshgrid(*[np.arange(s) for s in gm.shape], indexing="ij"), -1)  # [z][y][x]
+    pos = idx[..., ::-1].astype(float) / np.array(cells, float)  # x / ext, x first
+    inside = np.all(pos < 1.0, axis=-1)
+    if kw.get("sigmoid_combine", "radialPlus") != "tensor":
+        inside &= np.sqrt((pos ** 2).sum(-1)) < np.sqrt(dim) - 1e-9
+    assert inside.sum() > 0
+    np.testing.assert_allclose(gm[inside], gc[inside], rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("dim,cells,kw", MERGE_CASES)
+def test_merge_grid_is_mirror_symmetric(dim, cells, kw):
+    """The merged R2C grid is even about the centre of every axis
+    (G[.., i] = G[.., N - i]): the mirror images of x and of N h - x are the
+    same set."""
+    d = desc(dim, cells, matrix="r2c", approximate=True, periodization="merge", **kw)
+    g = orc.cov_grid(d)
+    for ax in range(dim):
+        a = g.shape[dim - 1 - ax]  # axis ax of x-fastest data sits at numpy axis dim-1-ax
+        flipped = np.take(g, [(-i) % a for i in range(a)], axis=dim - 1 - ax)
+        np.testing.assert_allclose(g, flipped, rtol=1e-13, atol=1e-15)
+
+
+def test_merge_with_factor_two_is_classical_on_the_dct_grid():
+    """With embedding factor 2 the taper interval (sigmoidStart 1,
+    sigmoidEnd factor - 1 = 1) has no width: on the DCT half grid the merge
+    entries are the classical ones."""
+    kw = dict(covariance="matern52", corr_length=0.3, approximate=True)
+    gm = orc.cov_grid(desc(2, [16, 12], periodization="merge", sigmoid_combine="tensor", **kw))
+    gc = orc.cov_grid(desc(2, [16, 12], **kw))
+    np.testing.assert_allclose(gm, gc, rtol=1e-13, atol=1e-15)
+
+
+def test_merge_removes_the_negative_eigenvalues_of_mirroring():
+    """The purpose of the smooth embedding: Matern-5/2 with l = 0.25 on a
+    factor-3 embedding has negative eigenvalues with classical mirroring and
+    none with the merge embedding."""
+    kw = dict(covariance="matern52", corr_length=0.25, factor=3, approximate=True)
+    _, st_c, _ = orc.eigs(desc(2, [64, 64], **kw), allow_negative=True)
+    _, st_m, status = orc.eigs(desc(2, [64, 64], periodization="merge",
+                                    sigmoid_combine="tensor", **kw), allow_negative=True)
+    assert st_c["negative"] > 0
+    assert st_m["negative"] == 0 and status == 0 and st_m["smallest"] > 0
+
+
+def test_merge_default_recursions_rule():
+    """embedding.mergeRecursions 99 (the default) has a count for the Matern
+    families, exponential, cauchy and their generalizations only; others are
+    rejected like the reference does (matrix.hh:679-705)."""
+    with pytest.raises(RuntimeError):
+        orc.cov_grid(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="merge"))
+    orc.cov_grid(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="merge",
+                      merge_recursions=2))
+    # explicit counts 1 and 2 through the taper change the entries beyond the domain
+    g1 = orc.cov_grid(desc(2, [8, 8], covariance="exponential", corr_length=0.5, factor=3,
+                           matrix="r2c", periodization="merge", merge_recursions=1))
+    g2 = orc.cov_grid(desc(2, [8, 8], covariance="exponential", corr_length=0.5, factor=3,
+                           matrix="r2c", periodization="merge", merge_recursions=2))
+    assert np.abs(g1 - g2).max() > 1e-6
