@@ -24,6 +24,7 @@
 
 #include "../parafields_amd.h"
 #include "parametertree.hh"
+#include "trend.hh"
 
 namespace parafields_amd {
 
@@ -428,6 +429,7 @@ private:
   StochasticPart<RF> sp_;
   std::array<int, dim> cells_;
   std::array<double, dim> ext_;
+  TrendPart<RF, dim> trend_;
 
 public:
   explicit RandomField(const ParameterTree& config)
@@ -436,6 +438,7 @@ public:
   {
     cells_ = config.get<std::array<int, dim>>("grid.cells");
     ext_ = config.get<std::array<double, dim>>("grid.extensions");
+    trend_ = TrendPart<RF, dim>(config, ext_);
     const std::string an = config.get<std::string>("stochastic.anisotropy", "none");
     const std::string cov = config.get<std::string>("stochastic.covariance");
     if (cov != "custom-aniso" && (an == "none" || an == "axiparallel"))
@@ -460,6 +463,8 @@ public:
     generateWithRNG(rng, allowNonWorldComm);
   }
 
+  // randomfield.hh:541-558: the field, then the trend coefficients from the
+  // same RNG
   template <typename RNG>
   void generateWithRNG(RNG& rng, bool = false)
   {
@@ -467,14 +472,17 @@ public:
       aniso_->generateField(rng, sp_);
     else
       iso_->generateField(rng, sp_);
+    trend_.generate(rng);
   }
 
+  // randomfield.hh:589-603
   void generateUncorrelated(unsigned int seed, bool = false)
   {
     if (aniso_)
       aniso_->generateUncorrelatedField(seed, sp_);
     else
       iso_->generateUncorrelatedField(seed, sp_);
+    trend_.generateUncorrelated(seed);
   }
 
   template <typename Covariance>
@@ -486,16 +494,32 @@ public:
       iso_->fillTransformedMatrix(std::forward<Covariance>(cov));
   }
 
-  void timesMatrix() { sp_ = aniso_ ? (*aniso_) * sp_ : (*iso_) * sp_; }
-  void timesInverseMatrix() { sp_ = aniso_ ? aniso_->multiplyInverse(sp_) : iso_->multiplyInverse(sp_); }
-  void timesMatrixRoot() { sp_ = aniso_ ? aniso_->multiplyRoot(sp_) : iso_->multiplyRoot(sp_); }
+  // randomfield.hh:1181-1309: the stochastic part through the matrix, the
+  // trend coefficients scaled by their variances
+  void timesMatrix()
+  {
+    sp_ = aniso_ ? (*aniso_) * sp_ : (*iso_) * sp_;
+    trend_.timesMatrix();
+  }
+  void timesInverseMatrix()
+  {
+    sp_ = aniso_ ? aniso_->multiplyInverse(sp_) : iso_->multiplyInverse(sp_);
+    trend_.timesInverseMatrix();
+  }
+  void timesMatrixRoot()
+  {
+    sp_ = aniso_ ? aniso_->multiplyRoot(sp_) : iso_->multiplyRoot(sp_);
+    trend_.timesMatrixRoot();
+  }
   void timesInvMatRoot()
   {
-    timesInverseMatrix();
-    timesMatrixRoot();
+    sp_ = aniso_ ? aniso_->multiplyInverse(sp_) : iso_->multiplyInverse(sp_);
+    sp_ = aniso_ ? aniso_->multiplyRoot(sp_) : iso_->multiplyRoot(sp_);
+    trend_.timesInvMatRoot();
   }
 
-  // randomfield.hh:641-651: value at a location (cell lookup + transform)
+  // randomfield.hh:641-651: stochastic value (cell lookup) plus trend, then
+  // the value transform
   void evaluate(const std::array<double, dim>& location, RF& output) const
   {
     size_t idx = 0, stride = 1;
@@ -505,19 +529,32 @@ public:
       idx += (size_t)c * stride;
       stride *= cells_[i];
     }
-    output = sp_.dataVector.at(idx);
+    output = sp_.dataVector.at(idx) + trend_.evaluate(location);
     transform_.apply(output);
   }
 
-  // randomfield.hh:664-695: all cells, x fastest
+  // randomfield.hh:664-695: all cells, x fastest, each at its centre
   void bulkEvaluate(std::vector<RF>& output, std::array<unsigned int, dim>& sizes) const
   {
     for (int i = 0; i < dim; i++)
       sizes[i] = cells_[i];
     output = sp_.dataVector;
-    for (auto& v : output)
-      transform_.apply(v);
+    const bool trend = trend_.size() > 0;
+    std::array<double, dim> x{};
+    for (size_t i = 0; i < output.size(); i++) {
+      if (trend) {
+        size_t r = i;
+        for (int a = 0; a < dim; a++) {
+          x[a] = ((double)(r % cells_[a]) + 0.5) * (ext_[a] / cells_[a]);
+          r /= cells_[a];
+        }
+        output[i] += trend_.evaluate(x);
+      }
+      transform_.apply(output[i]);
+    }
   }
+
+  const TrendPart<RF, dim>& trendPart() const { return trend_; }
 
   const std::vector<RF>& dataVector() const { return sp_.dataVector; }
   std::vector<RF>& dataVector() { return sp_.dataVector; }

@@ -1,0 +1,155 @@
+// trend.hh -- deterministic trend components of a random field (host side).
+//
+// Counterpart of TrendComponent / TrendPart (trend.hh:29-640, :851-1505 of the
+// reference): mean, slope, disk and block components read from the field's
+// configuration, each with coefficients mean + shift. `generate` draws the
+// shifts from the field's RNG after the stochastic part (trend.hh:324-342,
+// :1148-1155, called from randomfield.hh:550-554), `generateUncorrelated`
+// from a fresh std::default_random_engine(seed) per component
+// (trend.hh:355-385), and the matrix products scale the shifts by the
+// variances (trend.hh:394-441). The image component (PNG input) is not built.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "parametertree.hh"
+
+namespace parafields_amd {
+
+template <typename RF, int dim>
+class TrendPart
+{
+public:
+  enum class Kind { Mean, Slope, Disk, Block };
+
+  struct Component
+  {
+    Kind kind;
+    std::vector<RF> mean, variance, shift;
+
+    // value at location x (trend.hh:585-620)
+    RF evaluate(const std::array<double, dim>& x, const std::array<double, dim>& ext) const
+    {
+      auto c = [&](size_t i) { return mean[i] + shift[i]; };
+      switch (kind) {
+        case Kind::Mean:
+          return c(0);
+        case Kind::Slope: {
+          RF out = 0.;
+          for (int i = 0; i < dim; i++)
+            out += c(i) * (RF)(x[i] - ext[i] / 2.);
+          return out;
+        }
+        case Kind::Disk: {
+          RF d2 = 0.;
+          for (int i = 0; i < dim; i++)
+            d2 += std::pow((RF)x[i] - c(i), 2);
+          return std::exp(-d2 / std::pow(c(dim), 2) * c(dim + 1));
+        }
+        case Kind::Block:
+          for (int i = 0; i < dim; i++)
+            if (std::abs((RF)x[i] - c(i)) > 0.5 * c(dim + i))
+              return 0.;
+          return c(2 * dim);
+      }
+      return 0.;
+    }
+  };
+
+  TrendPart() = default;
+
+  // Components in the reference's order: mean, slope, disk0, disk1, ...,
+  // block0, block1, ... (trend.hh:905-1035); keys <name>.mean and
+  // <name>.variance.
+  TrendPart(const ParameterTree& config, const std::array<double, dim>& ext)
+    : ext_(ext)
+  {
+    add(config, "mean", Kind::Mean, 1);
+    add(config, "slope", Kind::Slope, dim);
+    for (int k = 0; add(config, "disk" + std::to_string(k), Kind::Disk, dim + 2); k++) {
+    }
+    for (int k = 0; add(config, "block" + std::to_string(k), Kind::Block, 2 * dim + 1); k++) {
+    }
+    if (config.hasKey("image.mean"))
+      throw std::runtime_error("the image trend component (PNG input) is not built");
+  }
+
+  size_t size() const { return comps_.size(); }
+  const Component& component(size_t i) const { return comps_.at(i); }
+  Component& component(size_t i) { return comps_.at(i); }
+
+  // shift_i = sample() * sqrt(variance_i), component by component
+  // (trend.hh:324-342, :1148-1155)
+  template <typename RNG>
+  void generate(RNG& rng)
+  {
+    for (auto& c : comps_)
+      for (size_t i = 0; i < c.shift.size(); i++)
+        c.shift[i] = rng.sample() * std::sqrt(c.variance[i]);
+  }
+
+  // white-noise coefficients: each component draws from its own
+  // std::default_random_engine(seed) with std::normal_distribution
+  // (trend.hh:355-385)
+  void generateUncorrelated(unsigned int seed)
+  {
+    for (auto& c : comps_) {
+      std::default_random_engine generator(seed);
+      std::normal_distribution<RF> normal(0., 1.);
+      for (auto& s : c.shift)
+        s = normal(generator);
+    }
+  }
+
+  // trend.hh:394-441
+  void timesMatrix() { scale([](RF v) { return v; }); }
+  void timesInverseMatrix() { scale([](RF v) { return 1. / v; }); }
+  void timesMatrixRoot() { scale([](RF v) { return std::sqrt(v); }); }
+  void timesInvMatRoot() { scale([](RF v) { return 1. / std::sqrt(v); }); }
+
+  // sum of the component values at x (trend.hh:1489-1505)
+  RF evaluate(const std::array<double, dim>& x) const
+  {
+    RF out = 0.;
+    for (const auto& c : comps_)
+      out += c.evaluate(x, ext_);
+    return out;
+  }
+
+private:
+  std::array<double, dim> ext_{};
+  std::vector<Component> comps_;
+
+  bool add(const ParameterTree& config, const std::string& name, Kind kind, size_t n)
+  {
+    if (!config.hasKey(name + ".mean"))
+      return false;
+    Component c;
+    c.kind = kind;
+    c.mean = config.get<std::vector<RF>>(name + ".mean");
+    c.variance = config.get<std::vector<RF>>(name + ".variance");
+    if (c.mean.size() != c.variance.size())
+      throw std::runtime_error("trend component size does not match");
+    if (c.mean.size() != n)
+      throw std::runtime_error("trend component " + name + " must contain " + std::to_string(n) +
+                               " parameters");
+    c.shift.assign(n, RF(0));
+    comps_.push_back(c);
+    return true;
+  }
+
+  template <typename F>
+  void scale(F f)
+  {
+    for (auto& c : comps_)
+      for (size_t i = 0; i < c.shift.size(); i++)
+        c.shift[i] *= f(c.variance[i]);
+  }
+};
+
+} // namespace parafields_amd

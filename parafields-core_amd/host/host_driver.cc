@@ -4,6 +4,10 @@
 //   pf_host_driver field <prefix>  2D 256x256 exponential l=0.05, generate(42) in
 //                                  double, then timesMatrix, timesMatrixRoot and
 //                                  timesInverseMatrix; raw doubles to <prefix>.*.bin
+//   pf_host_driver trend <prefix>  2D 64x48 exponential l=0.1 with mean, slope, disk
+//                                  and block trend components: generate(42), the
+//                                  stochastic part, the trend coefficients and
+//                                  bulkEvaluate to <prefix>.*.bin
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -77,6 +81,43 @@ int main(int argc, char** argv)
       field.bulkEvaluate(vals, sizes);
       std::cout << "field: ok " << sizes[0] << "x" << sizes[1] << std::endl;
       return 0;
+    }
+    if (!std::strcmp(argv[1], "trend") && argc > 2) {
+      const std::string pre = argv[2];
+      ParameterTree config;
+      std::istringstream ini("[grid]\ncells = 64 48\nextensions = 1 1\n"
+                             "[stochastic]\nvariance = 1.0\ncovariance = exponential\n"
+                             "corrLength = 0.1\n"
+                             "[mean]\nmean = 0.5\nvariance = 0.25\n"
+                             "[slope]\nmean = 1 -2\nvariance = 0.5 2\n"
+                             "[disk0]\nmean = 0.3 0.6 0.1 1\nvariance = 0.01 0.01 0.001 0.5\n"
+                             "[block0]\nmean = 0.5 0.5 0.2 0.3 2\nvariance = 0.01 0.01 0 0 1\n");
+      config.readINI(ini);
+      RandomField<GridTraits<double, double, 2>> field(config);
+      field.generate(42u);
+      write(pre + ".stoch.bin", field.dataVector());
+      std::vector<double> shifts, means;
+      for (size_t k = 0; k < field.trendPart().size(); k++)
+        for (size_t i = 0; i < field.trendPart().component(k).shift.size(); i++) {
+          shifts.push_back(field.trendPart().component(k).shift[i]);
+          means.push_back(field.trendPart().component(k).mean[i]);
+        }
+      write(pre + ".shift.bin", shifts);
+      std::vector<double> vals;
+      std::array<unsigned, 2> sizes;
+      field.bulkEvaluate(vals, sizes);
+      write(pre + ".bulk.bin", vals);
+      // evaluate() at every cell centre agrees with bulkEvaluate
+      double worst = 0.;
+      for (unsigned j = 0; j < sizes[1]; j++)
+        for (unsigned i = 0; i < sizes[0]; i++) {
+          double v;
+          field.evaluate({ (i + 0.5) / sizes[0], (j + 0.5) / sizes[1] }, v);
+          worst = std::max(worst, std::fabs(v - vals[i + sizes[0] * j]));
+        }
+      std::cout << "trend: ok " << shifts.size() << " coefficients, evaluate/bulk " << worst
+                << std::endl;
+      return worst < 1e-12 ? 0 : 1;
     }
   } catch (const NegativeEigenvalueError& e) {
     std::cerr << "NegativeEigenvalueError: " << e.what() << std::endl;

@@ -521,6 +521,42 @@ def test_cpp_host_api_end_to_end(tmp_path):
     assert np.linalg.norm(load("inv") - inv) / np.linalg.norm(inv) < 1e-7
 
 
+def test_cpp_trend_components_end_to_end(tmp_path):
+    """RandomField with mean, slope, disk and block trend components
+    (pf_host_driver trend): the stochastic part is the oracle's field of the
+    first samples of the libstdc++ seed-42 stream; the 12 trend coefficients
+    are the next 12 samples times the square roots of their variances, in
+    component order (randomfield.hh:550-554, trend.hh:324-342); bulkEvaluate
+    adds the trend with coefficients mean + shift at the cell centres (the
+    component shapes themselves are pinned by tests/host/trend_check.cc)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(pfb.LIB_PATH), "pf_host_driver")
+    pre = str(tmp_path / "t")
+    r = subprocess.run([exe, "trend", pre], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "trend: ok 12" in r.stdout, r.stdout + r.stderr
+    d = desc(2, [64, 48], covariance="exponential", corr_length=0.1)
+    npf = int(orc.sizes(d).noise_per_field)
+    stream = orc.noise_cpp(42, npf + 12)
+    load = lambda k: np.fromfile(f"{pre}.{k}.bin", dtype=np.float64)
+    stoch = load("stoch")
+    assert rel_max(stoch, orc.generate(d, stream[:npf])) < FP64_TOL
+    # mean | slope x, y | disk centre x, y, radius, value | block centre x, y, extent x, y, value
+    mean = np.array([0.5, 1, -2, 0.3, 0.6, 0.1, 1, 0.5, 0.5, 0.2, 0.3, 2])
+    var = np.array([0.25, 0.5, 2, 0.01, 0.01, 0.001, 0.5, 0.01, 0.01, 0, 0, 1])
+    shift = load("shift")
+    np.testing.assert_allclose(shift, stream[npf:] * np.sqrt(var), rtol=1e-15, atol=0)
+    c = mean + shift
+    x = (np.arange(64) + 0.5) / 64
+    y = ((np.arange(48) + 0.5) / 48)[:, None]
+    trend = (c[0] + c[1] * (x - 0.5) + c[2] * (y - 0.5)
+             + np.exp(-((x - c[3]) ** 2 + (y - c[4]) ** 2) / c[5] ** 2 * c[6])
+             + np.where((np.abs(x - c[7]) <= 0.5 * c[9]) & (np.abs(y - c[8]) <= 0.5 * c[10]),
+                        c[11], 0.0))
+    np.testing.assert_allclose(load("bulk").reshape(48, 64) - stoch.reshape(48, 64), trend,
+                               rtol=1e-12, atol=1e-12)
+
+
 # ------------------------------------------------ learned preconditioner --
 
 def test_mlp_gradients_match_fp64_oracle():
