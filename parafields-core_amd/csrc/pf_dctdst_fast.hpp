@@ -156,6 +156,26 @@ __global__ void __launch_bounds__(NT, 4) kdf_gen_last(DctGeom g, const T* sq, co
     else
       return philox_normal1<T>(seed, real, (uint64_t)st);
   };
+  // Philox blocks shared along a tile row: in an interior tile with
+  // consecutive kx stored consecutively (sx = 1), the L lanes of one row
+  // hold consecutive samples, four to a block. Lanes l < L/2 draw the blocks
+  // of the row's real parts (b_re0 + l), lanes l >= L/2 those of its
+  // imaginary parts, and every lane takes the words of its sample's block
+  // from the lane that drew it (ds_bpermute). A row of L samples spans at
+  // most L/2 blocks for L >= 4. FP32 only: FP64's Box-Muller (software
+  // log, sincos) twice per item next to the shared words spills registers.
+  static_assert(L >= 4, "a tile row spans at most L/2 Philox blocks");
+  const bool share = NOISE == 1 && sizeof(T) == 4 && !edge && g.sx == 1;
+  const int lane0 = ((int)threadIdx.x & 63) - l; // first lane of the thread's row in its wave
+  auto from_lane = [&](uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int src,
+                       int64_t st) -> T {
+    const int a = src << 2;
+    return box_muller_of_sample<T>((uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)c0),
+                                   (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)c1),
+                                   (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)c2),
+                                   (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)c3),
+                                   (uint64_t)st);
+  };
   // outputs: block-uniform base, 32-bit element offsets (the launcher checks
   // nlast * plane < 2^32)
   const uint32_t lstep = edge ? (uint32_t)g.P0p : 1u;
@@ -177,8 +197,23 @@ __global__ void __launch_bounds__(NT, 4) kdf_gen_last(DctGeom g, const T* sq, co
         const int m = m_t + it * MS;
         const bool row0 = m == 0;
         const T root_m = *rp;
-        const T re = normal(s_re) * (row0 ? f0b : f0i) * root_m;
-        const T im = normal(row0 ? s_xm : s_im) * (row0 ? f0b : f1i) * (row0 ? sM : root_m);
+        const int64_t s_i = row0 ? s_xm : s_im;
+        T n_re, n_im;
+        if (share) {
+          const int64_t b_re0 = (s_re - l) >> 2, b_im0 = (s_i - l) >> 2;
+          const int64_t blk = l < L / 2 ? b_re0 + l : b_im0 + (l - L / 2);
+          uint32_t cw[4] = { (uint32_t)blk, (uint32_t)((uint64_t)blk >> 32), (uint32_t)real,
+                             (uint32_t)(real >> 32) };
+          philox4x32_10(cw, (uint32_t)seed, (uint32_t)(seed >> 32));
+          n_re = from_lane(cw[0], cw[1], cw[2], cw[3], lane0 + (int)((s_re >> 2) - b_re0), s_re);
+          n_im = from_lane(cw[0], cw[1], cw[2], cw[3], lane0 + L / 2 + (int)((s_i >> 2) - b_im0),
+                           s_i);
+        } else {
+          n_re = normal(s_re);
+          n_im = normal(s_i);
+        }
+        const T re = n_re * (row0 ? f0b : f0i) * root_m;
+        const T im = n_im * (row0 ? f0b : f1i) * (row0 ? sM : root_m);
         const cplx<T> v = { live ? re : T(0), live ? (row0 ? im : -im) : T(0) };
         buf[tix<L, DCT_PAD>(m, l)] = v;
       }

@@ -158,16 +158,38 @@ __device__ __forceinline__ T philox_normal1(uint64_t seed, uint64_t real, uint64
   return philox_normal<T>(seed, real, s);
 }
 
+// The normal of sample s from the four words c of its Philox block (block
+// s / 4): Box-Muller of the word pair (s / 2) mod 2, its cosine (s even) or
+// sine (s odd) part.
+// (The words are passed as values: a select between elements of an array
+// would keep the array in private memory.)
+template <typename T>
+__device__ __forceinline__ T box_muller_of_sample(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                  uint32_t c3, uint64_t s)
+{
+  const bool hi = (s >> 1) & 1;
+  T a, b;
+  BoxMuller<T>::pair(hi ? c2 : c0, hi ? c3 : c1, a, b);
+  return (s & 1) ? b : a;
+}
+
+template <>
+__device__ __forceinline__ float box_muller_of_sample<float>(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                             uint32_t c3, uint64_t s)
+{
+  const bool hi = (s >> 1) & 1;
+  const float r = BoxMuller<float>::radius(hi ? c2 : c0);
+  const float u = BoxMuller<float>::turn(hi ? c3 : c1);
+  return r * __builtin_amdgcn_sinf((s & 1) ? u : u + 0.25f);
+}
+
 template <>
 __device__ __forceinline__ float philox_normal1<float>(uint64_t seed, uint64_t real, uint64_t s)
 {
   const uint64_t blk = s >> 2;
   uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real, (uint32_t)(real >> 32) };
   philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const bool hi = (s >> 1) & 1;
-  const float r = BoxMuller<float>::radius(hi ? c[2] : c[0]);
-  const float u = BoxMuller<float>::turn(hi ? c[3] : c[1]);
-  return r * __builtin_amdgcn_sinf((s & 1) ? u : u + 0.25f);
+  return box_muller_of_sample<float>(c[0], c[1], c[2], c[3], s);
 }
 
 // ---------------------------------------------------------------------------

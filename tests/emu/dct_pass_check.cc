@@ -22,6 +22,7 @@
 thread_local emu_dim3 threadIdx;
 emu_dim3 blockIdx, blockDim, gridDim;
 std::barrier<>* emu_bar;
+int emu_perm[1024];
 namespace pfamd { alignas(16) unsigned char smem[160 * 1024]; }
 #include "pf_dctdst_fast.hpp"
 using namespace pfamd;

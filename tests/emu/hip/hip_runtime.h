@@ -21,6 +21,18 @@ extern thread_local emu_dim3 threadIdx;
 extern emu_dim3 blockIdx, blockDim, gridDim;
 extern std::barrier<>* emu_bar;
 inline void __syncthreads() { emu_bar->arrive_and_wait(); }
+// ds_bpermute: lane i of a wave reads `val` of lane (addr / 4) mod 64 of the
+// same wave; every thread of the block takes part, as in the kernels that
+// call it (block-uniform control flow)
+extern int emu_perm[1024];
+inline int __builtin_amdgcn_ds_bpermute(int addr, int val)
+{
+  emu_perm[threadIdx.x] = val;
+  __syncthreads();
+  const int r = emu_perm[(threadIdx.x & ~63u) + ((unsigned)(addr >> 2) & 63u)];
+  __syncthreads();
+  return r;
+}
 // the hardware transcendentals (log2, sin/cos of revolutions, sqrt)
 inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
 inline float __builtin_amdgcn_sinf(float x) { return std::sin(6.283185307179586f * x); }
