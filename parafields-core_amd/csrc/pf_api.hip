@@ -616,8 +616,12 @@ bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial
   } else {
     Scope sc(&P, "eig_cov");
     with_cov(cp.covariance, [&](auto c) {
-      k_eval_cov_full<decltype(c)::value><<<row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]), 256,
-                                            0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      constexpr int C = decltype(c)::value;
+      const dim3 grid = row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]);
+      if (cp.merge)
+        k_eval_cov_full<C, 1><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      else
+        k_eval_cov_full<C, 0><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
     });
     check_launch();
   }
@@ -686,8 +690,12 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     else {
       Scope sc(&P, "eig_cov");
       with_cov(cp.covariance, [&](auto c) {
-        k_eval_cov_dct<decltype(c)::value><<<row_blocks((int64_t)g.P[1] * g.P[2], g.P[0]), 256,
-                                             0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+        constexpr int C = decltype(c)::value;
+        const dim3 grid = row_blocks((int64_t)g.P[1] * g.P[2], g.P[0]);
+        if (cp.merge)
+          k_eval_cov_dct<C, 1><<<grid, 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+        else
+          k_eval_cov_dct<C, 0><<<grid, 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
       });
       check_launch();
     }
@@ -750,8 +758,12 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     } else {
       Scope sc(&P, "eig_cov");
       with_cov(cp.covariance, [&](auto c) {
-      k_eval_cov_full<decltype(c)::value><<<row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]), 256,
-                                            0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      constexpr int C = decltype(c)::value;
+      const dim3 grid = row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]);
+      if (cp.merge)
+        k_eval_cov_full<C, 1><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      else
+        k_eval_cov_full<C, 0><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
     });
       check_launch();
     }
@@ -1605,9 +1617,15 @@ struct Batch
     double* real = G.scratch.as<double>();
     cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + nf * N0 * rows);
     with_cov(P.d.covariance, [&](auto c) {
-      k_eval_cov_full_group<decltype(c)::value><<<dim3(row_blocks(rows, (int)N0), nf), 256, 0,
-                                                  s>>>(G.cov.as<CovParams>(), (int)N0, (int)N1, 1,
-                                                       real, N0 * rows);
+      // the fields of a group share one periodization (part of the group key)
+      constexpr int C = decltype(c)::value;
+      const dim3 grid(row_blocks(rows, (int)N0), nf);
+      if (P.d.periodization == PF_PERIOD_MERGE)
+        k_eval_cov_full_group<C, 1><<<grid, 256, 0, s>>>(G.cov.as<CovParams>(), (int)N0, (int)N1,
+                                                         1, real, N0 * rows);
+      else
+        k_eval_cov_full_group<C, 0><<<grid, 256, 0, s>>>(G.cov.as<CovParams>(), (int)N0, (int)N1,
+                                                         1, real, N0 * rows);
     });
     check_launch();
     const FastCtx fc{ s, fast_budget() };
@@ -2310,7 +2328,8 @@ int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
         continue;
       }
       const std::array<int64_t, 6> key = { P.g.n[0], P.g.n[1], P.g.N[0], P.g.N[1],
-                                           (int64_t)P.g.transposed, (int64_t)P.d.covariance };
+                                           (int64_t)P.g.transposed,
+                                           (int64_t)P.d.covariance * 2 + P.d.periodization };
       auto it = key_group.find(key);
       if (it == key_group.end()) {
         it = key_group.emplace(key, (int)B->groups.size()).first;

@@ -479,13 +479,18 @@ __device__ inline double merged_cov(const CovParams& p, const int64_t idx[3])
 
 // Covariance at the embedding grid position idx (integer cell offsets):
 // coordinates idx*h wrapped by the classical rule (matrix.hh:615-638), or the
-// merge embedding.
-template <int COV = -1>
+// merge embedding. PER: 0 classical, 1 merge, -1 as p.merge says (a kernel
+// compiled for one periodization keeps the other's code and registers out).
+template <int COV = -1, int PER = -1>
 __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
 {
 #pragma clang fp contract(off)
-  if (p.merge)
+  if constexpr (PER == 1)
     return merged_cov<COV>(p, idx);
+  if constexpr (PER == -1) {
+    if (p.merge)
+      return merged_cov<COV>(p, idx);
+  }
   double coord[3] = { 0., 0., 0. };
   for (int i = 0; i < p.dim; i++) {
     coord[i] = (double)idx[i] * p.h[i];

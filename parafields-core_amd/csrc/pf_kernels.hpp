@@ -54,26 +54,26 @@ inline unsigned row_blocks(int64_t nrows, int n)
 // COV: the covariance family at compile time (-1: p.covariance at run time);
 // one family per instantiation keeps the other families' code (and
 // registers) out of the kernel.
-template <int COV>
+template <int COV, int PER>
 __global__ void k_eval_cov_dct(CovParams p, int P0, int P1, int P2, double* out)
 {
   for_rows((int64_t)P1 * P2, P0, [&](int64_t row, int i) {
     const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)P1),
                              (int64_t)((uint32_t)row / (uint32_t)P1) };
-    out[row * P0 + i] = embedded_cov<COV>(p, idx);
+    out[row * P0 + i] = embedded_cov<COV, PER>(p, idx);
   });
 }
 
 // Covariance on the full real grid [N2][N1][N0] (r2cmatrixbackend.hh:150-170;
 // the r2c padding entries are never read by the transform, so they are not
 // stored).
-template <int COV>
+template <int COV, int PER>
 __global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out)
 {
   for_rows((int64_t)N1 * N2, N0, [&](int64_t row, int i) {
     const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)N1),
                              (int64_t)((uint32_t)row / (uint32_t)N1) };
-    out[row * N0 + i] = embedded_cov<COV>(p, idx);
+    out[row * N0 + i] = embedded_cov<COV, PER>(p, idx);
   });
 }
 
@@ -338,7 +338,7 @@ __global__ void k_finish_r2c_eigs(const cplx<double>* spec, int Hp, int H, int64
 // Field f = blockIdx.y of a group: its own covariance parameters, threshold
 // and arrays at stride `stride` (entries) from the group's base.
 
-template <int COV>
+template <int COV, int PER>
 __global__ void k_eval_cov_full_group(const CovParams* cps, int N0, int N1, int N2, double* out,
                                       int64_t stride)
 {
@@ -347,7 +347,7 @@ __global__ void k_eval_cov_full_group(const CovParams* cps, int N0, int N1, int 
   for_rows((int64_t)N1 * N2, N0, [&](int64_t row, int i) {
     const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)N1),
                              (int64_t)((uint32_t)row / (uint32_t)N1) };
-    out[row * N0 + i] = embedded_cov<COV>(p, idx);
+    out[row * N0 + i] = embedded_cov<COV, PER>(p, idx);
   });
 }
 
