@@ -132,14 +132,17 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   d.cg_iterations = c.get<int>("randomField.cgIterations", 100);
   d.nugget = 0.;
   d.r2c_check_all = 0;
-  // embedding.periodization and the merge taper (matrix.hh:515-580, :652-775)
+  // embedding.periodization and the merge / fold taper (matrix.hh:515-580,
+  // :652-775, :790-888)
   const std::string per = c.get<std::string>("embedding.periodization", "classical");
   if (per == "classical")
     d.periodization = PF_PERIOD_CLASSICAL;
   else if (per == "merge")
     d.periodization = PF_PERIOD_MERGE;
-  else if (per == "fold" || per == "cofold")
-    throw std::runtime_error("embedding.periodization \"" + per + "\" is not built");
+  else if (per == "fold")
+    d.periodization = PF_PERIOD_FOLD;
+  else if (per == "cofold")
+    throw std::runtime_error("embedding.periodization \"cofold\" is not built");
   else
     throw std::runtime_error(
       "stochastic.periodization must be \"classical\", \"merge\", \"fold\", or \"cofold\"");
@@ -159,10 +162,11 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
     d.sigmoid_combine = PF_COMBINE_TENSOR;
   else
     throw std::runtime_error("sigmoidCombine must be tensor, radial or radialPlus");
-  const int rec = c.get<int>("embedding.mergeRecursions", 99);
+  const int rec = c.get<int>(per == "fold" ? "embedding.foldRecursions" : "embedding.mergeRecursions", 99);
   d.merge_recursions = rec == 99 ? 0 : (rec == 0 ? -1 : rec);
   d.sigmoid_start = c.get<double>("embedding.sigmoidStart", 1.);
   d.sigmoid_end = c.hasKey("embedding.sigmoidEnd") ? c.get<double>("embedding.sigmoidEnd") : 0.;
+  d.fold_max_factor = c.get<double>("embedding.foldMaxFactor", 1.);
   return d;
 }
 

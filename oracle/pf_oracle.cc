@@ -283,9 +283,9 @@ Geo make_geo(const pf_desc& d)
   g.factor = d.periodic ? 1 : d.embedding_factor;
   if (g.factor < 1)
     throw std::runtime_error("embedding.factor must be >= 1");
-  if (d.periodization != PF_PERIOD_CLASSICAL && d.periodization != PF_PERIOD_MERGE)
+  if (d.periodization < PF_PERIOD_CLASSICAL || d.periodization > PF_PERIOD_FOLD)
     throw std::runtime_error(
-      "stochastic.periodization must be \"classical\" or \"merge\" (fold and cofold are not built)");
+      "stochastic.periodization must be \"classical\", \"merge\" or \"fold\" (cofold is not built)");
   if (d.sigmoid != PF_SIGMOID_SMOOTHSTEP && d.sigmoid != PF_SIGMOID_SMOOTH)
     throw std::runtime_error("embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
   if (d.sigmoid_combine < PF_COMBINE_RADIAL_PLUS || d.sigmoid_combine > PF_COMBINE_TENSOR)
@@ -720,9 +720,55 @@ double merged_cov(const Geo& g, const Geometry& geo, const Covariance& cov, cons
   return value + constValue * remainder;
 }
 
+// Adaptive Simpson quadrature of f on [a, b] to absolute tolerance eps.
+template <typename F>
+double simpson_adapt(F& f, double a, double b, double fa, double fm, double fb, double whole,
+                     double eps, int depth)
+{
+  const double m = 0.5 * (a + b), lm = 0.5 * (a + m), rm = 0.5 * (m + b);
+  const double flm = f(lm), frm = f(rm);
+  const double left = (m - a) / 6. * (fa + 4. * flm + fm);
+  const double right = (b - m) / 6. * (fm + 4. * frm + fb);
+  if (depth <= 0 || std::fabs(left + right - whole) <= 15. * eps)
+    return left + right + (left + right - whole) / 15.;
+  return simpson_adapt(f, a, m, fa, flm, fm, left, 0.5 * eps, depth - 1) +
+         simpson_adapt(f, m, b, fm, frm, fb, right, 0.5 * eps, depth - 1);
+}
+
+// Integral of the taper sigmoid(1, beta, x) over x in [1, norm], the quantity
+// the fold embedding computes with gsl_integration_qng (absolute tolerance
+// 1e-6, relative 1e-3; matrix.hh:866-868). The taper is zero from beta on.
+// Smoothstep: exact antiderivative. With u = 1 - 2 (x - 1) / (beta - 1) and
+// w = 1 - u^2 the taper is (1 + u S(u) / n) / 2, S = sum_j a_j w^j from the
+// recursion out_i = (2i out_{i-1} + w^i) / (2i + 1); Q(u) = sum_j a_j
+// (-w^(j+1) / (2 (j + 1))) has Q' = u S and follows the same recursion, and
+// the integral is (beta - 1) / 2 * (G(1) - G(u)), G(u) = u / 2 + Q(u) / (2 n),
+// G(1) = 1/2. Smooth: adaptive Simpson to 1e-14.
+double fold_integral(int sigmoid, double beta, double norm, unsigned rec)
+{
+  const double hi = std::min(norm, beta);
+  if (!(hi > 1.))
+    return 0.;
+  if (sigmoid == PF_SIGMOID_SMOOTHSTEP) {
+    const double u = 1. - 2. * (hi - 1.) / (beta - 1.), w = 1. - u * u;
+    double n = 1., q = -0.5 * w, wp = w;
+    for (unsigned i = 1; i <= rec; i++) {
+      wp *= w;
+      q = (2. * i * q - wp / (2. * (i + 1.))) / (2. * i + 1.);
+      n *= 2. * i / (2. * i + 1.);
+    }
+    return 0.5 * (beta - 1.) * (0.5 - (0.5 * u + 0.5 * q / n));
+  }
+  auto f = [&](double x) { return sigmoid_smooth(1., beta, x); };
+  const double fa = f(1.), fm = f(0.5 * (1. + hi)), fb = f(hi);
+  return simpson_adapt(f, 1., hi, fa, fm, fb, (hi - 1.) / 6. * (fa + 4. * fm + fb), 1e-14, 40);
+}
+
 // Entry value of the embedding grid at integer position idx:
 // Matrix::computeMatrixEntriesWithMirroring, matrix.hh:615-638 (classical),
-// or the merge embedding.
+// computeMatrixEntriesWithFold, matrix.hh:790-888 (fold: coordinates whose
+// extension-scaled norm exceeds 1 move towards the origin, to norm
+// 1 + the taper's integral), or the merge embedding.
 double embedded_cov(const Geo& g, const Geometry& geo, const Covariance& cov, const pf_desc& d,
                     const Index idx[3])
 {
@@ -733,6 +779,19 @@ double embedded_cov(const Geo& g, const Geometry& geo, const Covariance& cov, co
     coord[i] = (double)idx[i] * g.h[i];
     if (coord[i] > 0.5 * g.ext[i] * (double)g.factor)
       coord[i] -= g.ext[i] * (double)g.factor;
+  }
+  if (d.periodization == PF_PERIOD_FOLD) {
+    const double max_factor = d.fold_max_factor != 0. ? d.fold_max_factor : 1.;
+    const double beta = max_factor * 0.5 * (double)g.factor / std::sqrt((double)g.dim);
+    double norm = 0.;
+    for (int i = 0; i < g.dim; i++)
+      norm += (coord[i] * coord[i]) / (g.ext[i] * g.ext[i]);
+    norm = std::sqrt(norm / (double)g.dim);
+    if (norm > 1.) {
+      const double val = fold_integral(d.sigmoid, beta, norm, merge_recursions(d)) + 1.;
+      for (int i = 0; i < g.dim; i++)
+        coord[i] *= val / norm;
+    }
   }
   return cov_at(g, geo, cov, d, coord);
 }
@@ -1670,6 +1729,11 @@ double pfo_sigmoid(int kind, double one, double zero, double x, int rec)
   if (kind == 1)
     return sigmoid_smooth(one, zero, x);
   return sigmoid_smoothstep(one, zero, x, (unsigned)rec);
+}
+// Fold taper integral, for the tests (sigmoid: enum pf_sigmoid).
+double pfo_fold_integral(int sigmoid, double beta, double norm, int rec)
+{
+  return fold_integral(sigmoid, beta, norm, (unsigned)rec);
 }
 // Learned-PCG: preconditioner R F^{-1} diag(exp(L_bin)) F E.
 int pfo_pcg_learned(const pf_desc* d, const double* log_mult, int nbins, int maxit, double rel_tol,

@@ -325,9 +325,10 @@ void check_launch()
     throw PfError(PF_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
 }
 
-// embedding.mergeRecursions (smoothstep taper of the merge embedding): the
-// reference's value 99, its default, picks a count per covariance family
-// (matrix.hh:679-705) and throws for the others.
+// embedding.mergeRecursions / embedding.foldRecursions (smoothstep taper of
+// the merge and fold embeddings): the reference's value 99, its default,
+// picks a count per covariance family (matrix.hh:679-705, :805-829, the same
+// rule) and throws for the others.
 int merge_recursions(const pf_desc& d)
 {
   if (d.merge_recursions > 0)
@@ -420,15 +421,19 @@ Geo make_geo(const pf_desc& d)
     throw PfError(PF_EINVAL, "stochastic.anisotropy must be none, axiparallel or geometric");
   if (d.precision != PF_FP64 && d.precision != PF_FP32)
     throw PfError(PF_EINVAL, "precision must be PF_FP64 or PF_FP32");
-  if (d.periodization != PF_PERIOD_CLASSICAL && d.periodization != PF_PERIOD_MERGE)
-    throw PfError(PF_EINVAL, "stochastic.periodization must be \"classical\" or \"merge\" (fold "
-                             "and cofold are not built)");
+  if (d.periodization == PF_PERIOD_COFOLD)
+    throw PfError(PF_EINVAL, "cofold periodization is not built");
+  if (d.periodization < PF_PERIOD_CLASSICAL || d.periodization > PF_PERIOD_FOLD)
+    throw PfError(PF_EINVAL, "stochastic.periodization must be \"classical\", \"merge\", \"fold\", "
+                             "or \"cofold\"");
   if (d.sigmoid != PF_SIGMOID_SMOOTHSTEP && d.sigmoid != PF_SIGMOID_SMOOTH)
     throw PfError(PF_EINVAL, "embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
   if (d.sigmoid_combine < PF_COMBINE_RADIAL_PLUS || d.sigmoid_combine > PF_COMBINE_TENSOR)
     throw PfError(PF_EINVAL, "sigmoidCombine must be tensor, radial or radialPlus");
-  if (d.periodization == PF_PERIOD_MERGE)
+  if (d.periodization == PF_PERIOD_MERGE || d.periodization == PF_PERIOD_FOLD)
     (void)merge_recursions(d); // rejects families without a default count
+  if (!(d.fold_max_factor >= 0.))
+    throw PfError(PF_EINVAL, "embedding.foldMaxFactor must be positive");
   const int64_t half = (int64_t)g.H * g.N[1] * g.N[2];
   const int64_t P = (int64_t)g.P[0] * g.P[1] * g.P[2];
   g.half = half;
@@ -494,8 +499,18 @@ CovParams make_cov(const pf_desc& d, const Geo& g)
     p.wrap_len[i] = g.ext[i] * (double)g.factor;
   }
   p.factor = g.factor;
-  p.merge = d.periodization == PF_PERIOD_MERGE;
-  if (p.merge) {
+  p.period = d.periodization;
+  if (p.period == PF_PERIOD_FOLD) {
+    // matrix.hh:800-834: embedding.foldMaxFactor (default 1) and
+    // embedding.foldRecursions (same family rule as mergeRecursions)
+    p.sigmoid = d.sigmoid;
+    p.recursions = merge_recursions(d);
+    const double max_factor = d.fold_max_factor != 0. ? d.fold_max_factor : 1.;
+    p.fold_beta = max_factor * 0.5 * (double)g.factor / std::sqrt((double)d.dim);
+    for (int i = 0; i < 3; i++)
+      p.ext[i] = g.ext[i];
+  }
+  if (p.period == PF_PERIOD_MERGE) {
     p.sigmoid = d.sigmoid;
     p.radial = d.sigmoid_combine != PF_COMBINE_TENSOR &&
                std::sqrt((double)d.dim) < (double)g.factor / 2.;
@@ -533,6 +548,18 @@ GenGeom gen_geom(const Plan& P)
 
 // f(std::integral_constant<int, family>) for the covariance family `cov`
 // (-1 for families without an instantiation of their own).
+// the covariance kernels' periodization as a compile-time constant
+template <typename F>
+void with_period(int per, F&& f)
+{
+  if (per == PF_PERIOD_MERGE)
+    f(std::integral_constant<int, 1>{});
+  else if (per == PF_PERIOD_FOLD)
+    f(std::integral_constant<int, 2>{});
+  else
+    f(std::integral_constant<int, 0>{});
+}
+
 template <typename F>
 void with_cov(int cov, F&& f)
 {
@@ -618,10 +645,9 @@ bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial
     with_cov(cp.covariance, [&](auto c) {
       constexpr int C = decltype(c)::value;
       const dim3 grid = row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]);
-      if (cp.merge)
-        k_eval_cov_full<C, 1><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
-      else
-        k_eval_cov_full<C, 0><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      with_period(cp.period, [&](auto q) {
+        k_eval_cov_full<C, decltype(q)::value><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      });
     });
     check_launch();
   }
@@ -692,10 +718,9 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
       with_cov(cp.covariance, [&](auto c) {
         constexpr int C = decltype(c)::value;
         const dim3 grid = row_blocks((int64_t)g.P[1] * g.P[2], g.P[0]);
-        if (cp.merge)
-          k_eval_cov_dct<C, 1><<<grid, 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
-        else
-          k_eval_cov_dct<C, 0><<<grid, 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+        with_period(cp.period, [&](auto q) {
+          k_eval_cov_dct<C, decltype(q)::value><<<grid, 256, 0, P.stream>>>(cp, g.P[0], g.P[1], g.P[2], a);
+        });
       });
       check_launch();
     }
@@ -760,10 +785,9 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
       with_cov(cp.covariance, [&](auto c) {
       constexpr int C = decltype(c)::value;
       const dim3 grid = row_blocks((int64_t)g.N[1] * g.N[2], g.N[0]);
-      if (cp.merge)
-        k_eval_cov_full<C, 1><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
-      else
-        k_eval_cov_full<C, 0><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      with_period(cp.period, [&](auto q) {
+        k_eval_cov_full<C, decltype(q)::value><<<grid, 256, 0, P.stream>>>(cp, g.N[0], g.N[1], g.N[2], real);
+      });
     });
       check_launch();
     }
@@ -1620,12 +1644,10 @@ struct Batch
       // the fields of a group share one periodization (part of the group key)
       constexpr int C = decltype(c)::value;
       const dim3 grid(row_blocks(rows, (int)N0), nf);
-      if (P.d.periodization == PF_PERIOD_MERGE)
-        k_eval_cov_full_group<C, 1><<<grid, 256, 0, s>>>(G.cov.as<CovParams>(), (int)N0, (int)N1,
-                                                         1, real, N0 * rows);
-      else
-        k_eval_cov_full_group<C, 0><<<grid, 256, 0, s>>>(G.cov.as<CovParams>(), (int)N0, (int)N1,
-                                                         1, real, N0 * rows);
+      with_period(P.d.periodization, [&](auto q) {
+        k_eval_cov_full_group<C, decltype(q)::value><<<grid, 256, 0, s>>>(
+          G.cov.as<CovParams>(), (int)N0, (int)N1, 1, real, N0 * rows);
+      });
     });
     check_launch();
     const FastCtx fc{ s, fast_budget() };
@@ -2329,7 +2351,7 @@ int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
       }
       const std::array<int64_t, 6> key = { P.g.n[0], P.g.n[1], P.g.N[0], P.g.N[1],
                                            (int64_t)P.g.transposed,
-                                           (int64_t)P.d.covariance * 2 + P.d.periodization };
+                                           (int64_t)P.d.covariance * 4 + P.d.periodization };
       auto it = key_group.find(key);
       if (it == key_group.end()) {
         it = key_group.emplace(key, (int)B->groups.size()).first;

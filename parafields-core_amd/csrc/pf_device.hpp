@@ -210,12 +210,13 @@ struct CovParams
   int factor;
   // merge embedding (matrix.hh:652-775): taper kind, combination, recursion
   // count, taper interval in units of the extensions, extensions
-  int merge;          // 0: classical mirroring
+  int period;         // enum pf_periodization: 0 classical, 1 merge, 2 fold
   int sigmoid;        // enum pf_sigmoid
   int radial;         // radial taper (combine radial/radialPlus and sqrt(dim) < factor / 2)
   int radial_plus;    // the remainder takes the covariance at the centre of the extended domain
   int recursions;
   double sig_start, sig_end, ext[3];
+  double fold_beta;   // fold: the taper reaches zero at this norm (matrix.hh:831-833)
 };
 
 // Modified Bessel K_nu(x) (Temme series for x < 2, Steed's CF2 otherwise,
@@ -477,10 +478,39 @@ __device__ inline double merged_cov(const CovParams& p, const int64_t idx[3])
   return value;
 }
 
+// Integral of the taper sigmoid(1, fold_beta, x) over x in [1, norm]. The
+// taper is zero from fold_beta on, so the integral stops there; below that it
+// is smooth, and a composite 5-point Gauss-Legendre rule over 16 panels
+// (nodes 0, +-sqrt(5 -+ 2 sqrt(10/7)) / 3, weights 128/225,
+// (322 +- 13 sqrt(70)) / 900) integrates it to round-off for the recursion
+// counts in use. The reference calls gsl_integration_qng with absolute
+// tolerance 1e-6 and relative tolerance 1e-3 on [1, norm] (matrix.hh:866-868).
+__device__ inline double fold_integral(const CovParams& p, double norm)
+{
+  const double b = p.fold_beta;
+  const double hi = norm < b ? norm : b;
+  if (!(hi > 1.))
+    return 0.;
+  const double r10 = sqrt(10. / 7.), r70 = sqrt(70.);
+  const double x1 = sqrt(5. - 2. * r10) / 3., x2 = sqrt(5. + 2. * r10) / 3.;
+  const double w0 = 128. / 225., w1 = (322. + 13. * r70) / 900., w2 = (322. - 13. * r70) / 900.;
+  const int panels = 16;
+  const double half = 0.5 * (hi - 1.) / panels;
+  double acc = 0.;
+  for (int k = 0; k < panels; k++) {
+    const double c = 1. + (2 * k + 1) * half;
+    acc += w0 * sigmoid_taper(p, 1., b, c);
+    acc += w1 * (sigmoid_taper(p, 1., b, c - half * x1) + sigmoid_taper(p, 1., b, c + half * x1));
+    acc += w2 * (sigmoid_taper(p, 1., b, c - half * x2) + sigmoid_taper(p, 1., b, c + half * x2));
+  }
+  return acc * half;
+}
+
 // Covariance at the embedding grid position idx (integer cell offsets):
-// coordinates idx*h wrapped by the classical rule (matrix.hh:615-638), or the
-// merge embedding. PER: 0 classical, 1 merge, -1 as p.merge says (a kernel
-// compiled for one periodization keeps the other's code and registers out).
+// coordinates idx*h wrapped by the classical rule (matrix.hh:615-638), then
+// folded, or the merge embedding. PER: 0 classical, 1 merge, 2 fold, -1 as
+// p.period says (a kernel compiled for one periodization keeps the others'
+// code and registers out).
 template <int COV = -1, int PER = -1>
 __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
 {
@@ -488,7 +518,7 @@ __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
   if constexpr (PER == 1)
     return merged_cov<COV>(p, idx);
   if constexpr (PER == -1) {
-    if (p.merge)
+    if (p.period == 1)
       return merged_cov<COV>(p, idx);
   }
   double coord[3] = { 0., 0., 0. };
@@ -496,6 +526,19 @@ __device__ inline double embedded_cov(const CovParams& p, const int64_t idx[3])
     coord[i] = (double)idx[i] * p.h[i];
     if (coord[i] > p.wrap_half[i])
       coord[i] -= p.wrap_len[i];
+  }
+  // fold (matrix.hh:850-881): coordinates whose extension-scaled norm
+  // exceeds 1 move towards the origin, to norm 1 + the taper's integral
+  if (PER == 2 || (PER == -1 && p.period == 2)) {
+    double norm = 0.;
+    for (int i = 0; i < p.dim; i++)
+      norm += (coord[i] * coord[i]) / (p.ext[i] * p.ext[i]);
+    norm = sqrt(norm / (double)p.dim);
+    if (norm > 1.) {
+      const double val = fold_integral(p, norm) + 1.;
+      for (int i = 0; i < p.dim; i++)
+        coord[i] *= val / norm;
+    }
   }
   return cov_at<COV>(p, coord);
 }

@@ -33,7 +33,7 @@ ANISOTROPY = {"none": 0, "axiparallel": 1, "geometric": 2}
 MATRIX_BACKENDS = {"dct": 0, "r2c": 1}
 FIELD_BACKENDS = {"r2c": 0, "dctdst": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
-PERIODIZATION = {"classical": 0, "merge": 1}
+PERIODIZATION = {"classical": 0, "merge": 1, "fold": 2, "cofold": 3}
 SIGMOID = {"smoothstep": 0, "smooth": 1}
 SIGMOID_COMBINE = {"radialPlus": 0, "radial": 1, "tensor": 2}
 TRANSFORMS = {"none": 0, "logNormal": 1, "foldedNormal": 2, "sign": 3, "boxCox": 4}
@@ -76,6 +76,7 @@ class PfDesc(ctypes.Structure):
         ("merge_recursions", ctypes.c_int32),
         ("sigmoid_start", ctypes.c_double),
         ("sigmoid_end", ctypes.c_double),
+        ("fold_max_factor", ctypes.c_double),
     ]
 
 
@@ -134,13 +135,14 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
               r2c_check_all: bool = False, periodization: str = "classical",
               sigmoid: str = "smoothstep", sigmoid_combine: str = "radialPlus",
               merge_recursions: int = 0, sigmoid_start: float = 0.0,
-              sigmoid_end: float = 0.0) -> PfDesc:
+              sigmoid_end: float = 0.0, fold_max_factor: float = 0.0) -> PfDesc:
     """Descriptor with the reference's defaults. `matrix=None` picks what
     RandomField would (randomfield.hh:153-164): DCT (DefaultIsoMatrix) for
     none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy.
     periodization / sigmoid / sigmoid_combine take the reference's names
     (embedding.periodization, embedding.sigmoid, stochastic.sigmoidCombine);
-    merge_recursions 0 and sigmoid_start / sigmoid_end 0 mean the reference
+    merge_recursions (embedding.mergeRecursions, or embedding.foldRecursions for
+    fold), sigmoid_start / sigmoid_end and fold_max_factor 0 mean the reference
     defaults, merge_recursions -1 means zero recursions."""
     d = PfDesc()
     d.dim = dim
@@ -173,6 +175,7 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
     d.sigmoid_combine = SIGMOID_COMBINE[sigmoid_combine]
     d.merge_recursions = merge_recursions
     d.sigmoid_start, d.sigmoid_end = sigmoid_start, sigmoid_end
+    d.fold_max_factor = fold_max_factor
     return d
 
 

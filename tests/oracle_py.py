@@ -48,6 +48,7 @@ def load():
         "pfo_bessel_k": (dp, [dp, dp]),
         "pfo_cov_grid": (i32, [D, vp]),
         "pfo_sigmoid": (dp, [i32, dp, dp, dp, i32]),
+        "pfo_fold_integral": (dp, [i32, dp, dp, i32]),
         "pfo_pcg_learned": (i32, [D, vp, i32, i32, dp, vp, vp, ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_double)]),
         "pfo_spectral_targets": (i32, [D, i32, dp, vp]),
@@ -177,6 +178,13 @@ def sigmoid(kind, one, zero, x, rec=0):
     """kind: 'clamp', 'smooth' or 'smoothstep' (covariance.hh:147-290)."""
     k = {"clamp": 0, "smooth": 1, "smoothstep": 2}[kind]
     return load().pfo_sigmoid(k, one, zero, x, rec)
+
+
+def fold_integral(sigmoid, beta, norm, rec=0):
+    """Integral of sigmoid(1, beta, x) over [1, norm] (fold embedding);
+    sigmoid: 'smoothstep' or 'smooth'."""
+    k = {"smoothstep": 0, "smooth": 1}[sigmoid]
+    return load().pfo_fold_integral(k, beta, norm, rec)
 
 
 def bessel_k(nu, x):

@@ -113,6 +113,44 @@ def test_merge_eigs_match_oracle(dim, cells, kw):
         assert st[k] == wst[k], (k, st, wst)
 
 
+FOLD_CASES = [
+    (2, [64, 48], dict(covariance="exponential", corr_length=0.2, factor=4)),
+    (2, [32, 24], dict(covariance="matern52", corr_length=0.25, factor=3, sigmoid="smooth")),
+    (3, [16, 12, 8], dict(covariance="matern32", anisotropy="axiparallel",
+                          corr_length=[0.3, 0.2, 0.25], factor=4, fold_max_factor=1.5)),
+    (2, [24, 20], dict(covariance="gaussian", anisotropy="geometric",
+                       corr_length=[0.2, 0.1, 0.05, 0.3], factor=4, merge_recursions=3)),
+    (3, [8, 8, 6], dict(covariance="matern", matern_nu=1.2, corr_length=0.3, factor=4,
+                        matrix="r2c")),
+]
+
+
+@pytest.mark.parametrize("dim,cells,kw", FOLD_CASES)
+def test_fold_eigs_match_oracle(dim, cells, kw):
+    """Fold embedding: the device integrates the taper with composite
+    Gauss-Legendre, the oracle in closed form (smoothstep) or by adaptive
+    Simpson (smooth)."""
+    d = desc(dim, cells, periodization="fold", approximate=True, **kw)
+    p = pfb.Plan(d)
+    status, st = p.eigs_stats_nothrow()
+    want, wst, wstatus = orc.eigs(d, allow_negative=True)
+    assert status == wstatus
+    assert rel_max(p.eig_values(), want) < FP64_TOL
+    for k in ("small", "zero", "negative", "entries"):
+        assert st[k] == wst[k], (k, st, wst)
+
+
+def test_fold_generate_matches_oracle():
+    kw = dict(covariance="exponential", corr_length=0.2, factor=4, periodization="fold",
+              approximate=True)
+    d = desc(2, [64, 48], **kw)
+    p = pfb.Plan(d)
+    p.eigs()
+    npf = int(p.sizes.noise_per_field)
+    noise = orc.noise_cpp(31, npf)
+    assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < FP64_TOL
+
+
 def test_merge_generate_matches_oracle():
     """A field on the merge embedding (factor 3, no negative eigenvalues left)
     from host noise and from device Philox noise, FP64 and FP32."""
@@ -135,7 +173,10 @@ def test_merge_rejects_unbuilt_periodizations():
     with pytest.raises(RuntimeError):
         pfb.Plan(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="merge"))
     d = desc(2, [8, 8], factor=3)
-    d.periodization = 2  # fold is not built
+    d.periodization = 3  # cofold is not built
+    with pytest.raises(RuntimeError):
+        pfb.Plan(d)
+    d.periodization = 4
     with pytest.raises(RuntimeError):
         pfb.Plan(d)
 
@@ -669,6 +710,28 @@ def test_uneven_batch_graph_matches_single_plans():
         want = orc.generate(d64, orc.noise_philox(21, 100 + i, 0,
                                                   int(orc.sizes(d64).noise_per_field)))
         assert np.abs(b.field(out_g, i) - want).max() / want.std() < FP32_TOL
+
+
+def test_uneven_batch_mixes_periodizations():
+    """Fields of one covariance family with classical, merge and fold
+    periodization in one batch: the covariance launches are grouped per family
+    and periodization, so each field keeps the embedding of its own plan."""
+    descs = []
+    for i, (n0, n1) in enumerate([(32, 24), (48, 32), (24, 24), (64, 48), (16, 32), (40, 24)]):
+        per = ("classical", "merge", "fold")[i % 3]
+        descs.append(desc(2, [n0, n1], covariance="matern52", corr_length=0.2, factor=3,
+                          periodization=per, approximate=True, precision="fp32"))
+    b = pfb.Batch(descs)
+    out, _ = b.generate(seed=5, first=40, use_graph=True)
+    for i, d in enumerate(descs):
+        single = pfb.Plan(d).generate(1, seed=5, first=40 + i)[0]
+        assert np.array_equal(b.field(out, i), single), i
+    for i in (0, 1, 2):
+        d64 = descs[i]
+        d64.precision = 0
+        want = orc.generate(d64, orc.noise_philox(5, 40 + i, 0,
+                                                  int(orc.sizes(d64).noise_per_field)))
+        assert np.abs(b.field(out, i) - want).max() / want.std() < FP32_TOL
 
 
 def test_uneven_batch_branches_and_autotune():

@@ -556,3 +556,83 @@ def test_merge_default_recursions_rule():
     g2 = orc.cov_grid(desc(2, [8, 8], covariance="exponential", corr_length=0.5, factor=3,
                            matrix="r2c", periodization="merge", merge_recursions=2))
     assert np.abs(g1 - g2).max() > 1e-6
+
+
+# Fold embedding (matrix.hh:790-888): beyond extension-scaled norm 1 the
+# coordinates move towards the origin, to norm 1 + the integral of the taper
+# sigmoid(1, beta, x) over [1, norm], beta = foldMaxFactor * factor / (2 sqrt(dim)).
+
+@pytest.mark.parametrize("sigmoid,rec", [("smoothstep", 0), ("smoothstep", 1), ("smoothstep", 4),
+                                         ("smoothstep", 9), ("smooth", 0)])
+@pytest.mark.parametrize("beta", [1.3, 2.0, 3.5])
+def test_fold_integral_matches_quadrature(sigmoid, rec, beta):
+    """The oracle's integral (closed-form antiderivative for smoothstep,
+    adaptive Simpson for smooth) against scipy's QUADPACK on the sigmoid."""
+    from scipy import integrate
+    kind = "smoothstep" if sigmoid == "smoothstep" else "smooth"
+    for norm in (1.0 + 1e-3, 1.0 + 0.37 * (beta - 1.0), 1.0 + 0.8 * (beta - 1.0)):
+        want, _ = integrate.quad(lambda x: orc.sigmoid(kind, 1.0, beta, x, rec), 1.0, norm,
+                                 epsabs=1e-15, epsrel=1e-13, limit=200)
+        got = orc.fold_integral(sigmoid, beta, norm, rec)
+        assert abs(got - want) <= 1e-12 * max(1.0, abs(want)), (norm, got, want)
+
+
+@pytest.mark.parametrize("sigmoid,rec", [("smoothstep", 0), ("smoothstep", 3), ("smoothstep", 6),
+                                         ("smooth", 0)])
+def test_fold_integral_closed_forms(sigmoid, rec):
+    """Both tapers satisfy s(x) + s(1 + beta - x) = 1, so the full integral is
+    (beta - 1) / 2, and it stays there beyond beta; no recursion: the taper is
+    linear, integral (beta - 1)(t - t^2 / 2) at t = (norm - 1) / (beta - 1);
+    beta <= 1 or norm <= 1: zero."""
+    for beta in (1.25, 2.0, 4.0):
+        for norm in (beta, beta + 0.5, 10.0):
+            assert abs(orc.fold_integral(sigmoid, beta, norm, rec) - 0.5 * (beta - 1.0)) < 1e-12
+        if sigmoid == "smoothstep" and rec == 0:
+            for t in (0.1, 0.5, 0.9):
+                want = (beta - 1.0) * (t - 0.5 * t * t)
+                got = orc.fold_integral(sigmoid, beta, 1.0 + t * (beta - 1.0), 0)
+                assert abs(got - want) < 1e-13
+    assert orc.fold_integral(sigmoid, 0.7, 3.0, rec) == 0.0
+    assert orc.fold_integral(sigmoid, 1.0, 3.0, rec) == 0.0
+    assert orc.fold_integral(sigmoid, 2.0, 1.0, rec) == 0.0
+
+
+@pytest.mark.parametrize("sigmoid", ["smoothstep", "smooth"])
+def test_fold_grid_invariants(sigmoid):
+    """Exponential covariance, 2D, equal extensions, factor 4 (beta = sqrt 2),
+    DCT half grid: entries with norm <= 1 are the classical ones; entries with
+    norm >= beta all take C at radius sqrt(dim) (1 + (beta - 1) / 2);
+    in between, C at the coordinates scaled to 1 + integral."""
+    cells, ext, l = [24, 20], [1.0, 1.0], 0.3
+    kw = dict(covariance="exponential", corr_length=l, factor=4, matrix="dct", extensions=ext)
+    fold = orc.cov_grid(desc(2, cells, periodization="fold", sigmoid=sigmoid, **kw))
+    classic = orc.cov_grid(desc(2, cells, **kw))
+    beta = 4 / (2 * np.sqrt(2.0))
+    rec = 2  # exponential: default recursion count
+    h = [ext[i] / cells[i] for i in range(2)]
+    inner = beyond = between = 0
+    for j in range(fold.shape[0]):
+        for i in range(fold.shape[1]):
+            x, y = i * h[0], j * h[1]
+            norm = np.sqrt((x * x / ext[0] ** 2 + y * y / ext[1] ** 2) / 2)
+            if norm <= 1:
+                assert fold[j, i] == classic[j, i]
+                inner += 1
+                continue
+            if norm >= beta:
+                r = np.sqrt(2.0) * (1 + 0.5 * (beta - 1))
+                beyond += 1
+            else:
+                r = np.hypot(x, y) * (1 + orc.fold_integral(sigmoid, beta, norm, rec)) / norm
+                between += 1
+            assert abs(fold[j, i] - np.exp(-r / l)) < 1e-14, (i, j)
+    assert inner and beyond and between
+
+
+def test_fold_rejects_cofold_and_family_without_default_count():
+    with pytest.raises(RuntimeError):
+        orc.cov_grid(desc(2, [8, 8], covariance="exponential", periodization="cofold"))
+    with pytest.raises(RuntimeError):
+        orc.cov_grid(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="fold"))
+    orc.cov_grid(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="fold",
+                      merge_recursions=3))
