@@ -70,8 +70,9 @@ enum pf_precision { PF_FP64 = 0, PF_FP32 = 1 };
 
 /* embedding.periodization, matrix.hh:515-580: classical mirroring
    (computeMatrixEntriesWithMirroring, :615-638) or the smooth merge embedding
-   (computeMatrixEntriesWithMerge, :652-775) or fold (computeMatrixEntriesWithFold,
-   :790-888). cofold (:890-1013) is not built and is rejected with PF_EINVAL. */
+   (computeMatrixEntriesWithMerge, :652-775), fold (computeMatrixEntriesWithFold,
+   :790-888) or cofold (modifyMatrixEntriesWithCofold on the classical entries,
+   :890-1013; not in batches: its offset is reduced on the host). */
 enum pf_periodization {
   PF_PERIOD_CLASSICAL = 0,
   PF_PERIOD_MERGE = 1,
@@ -135,15 +136,17 @@ typedef struct pf_desc {
                                   and clamp the real part of every entry (not reference behaviour) */
   /* Embedding (matrix.hh:515-775). A zeroed tail gives the reference defaults. */
   int32_t periodization;       /* enum pf_periodization */
-  int32_t sigmoid;             /* enum pf_sigmoid (merge and fold) */
+  int32_t sigmoid;             /* enum pf_sigmoid (merge, fold, cofold) */
   int32_t sigmoid_combine;     /* enum pf_sigmoid_combine (merge only) */
-  int32_t merge_recursions;    /* embedding.mergeRecursions (merge) or embedding.foldRecursions
-                                  (fold): 0 = the reference default (its value 99:
+  int32_t merge_recursions;    /* embedding.mergeRecursions (merge), embedding.foldRecursions (fold)
+                                  or embedding.cofoldRecursions (cofold, default 1): 0 = the reference
+                                  default (its value 99:
                                   a count per covariance family, matrix.hh:679-705), k > 0, or -1 for
                                   zero recursions */
   double sigmoid_start;        /* embedding.sigmoidStart; 0 = default 1 */
   double sigmoid_end;          /* embedding.sigmoidEnd; 0 = default factor - 1 */
-  double fold_max_factor;      /* embedding.foldMaxFactor; 0 = default 1 */
+  double fold_max_factor;      /* embedding.foldMaxFactor (fold, default 1) or
+                                  embedding.cofoldMaxFactor (cofold, default 5); 0 = the default */
 } pf_desc;
 
 /* eigenvalue classification, matrix.hh:266-305 */

@@ -132,8 +132,8 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   d.cg_iterations = c.get<int>("randomField.cgIterations", 100);
   d.nugget = 0.;
   d.r2c_check_all = 0;
-  // embedding.periodization and the merge / fold taper (matrix.hh:515-580,
-  // :652-775, :790-888)
+  // embedding.periodization and the merge / fold / cofold taper
+  // (matrix.hh:515-580, :652-1013)
   const std::string per = c.get<std::string>("embedding.periodization", "classical");
   if (per == "classical")
     d.periodization = PF_PERIOD_CLASSICAL;
@@ -142,7 +142,7 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   else if (per == "fold")
     d.periodization = PF_PERIOD_FOLD;
   else if (per == "cofold")
-    throw std::runtime_error("embedding.periodization \"cofold\" is not built");
+    d.periodization = PF_PERIOD_COFOLD;
   else
     throw std::runtime_error(
       "stochastic.periodization must be \"classical\", \"merge\", \"fold\", or \"cofold\"");
@@ -162,11 +162,18 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
     d.sigmoid_combine = PF_COMBINE_TENSOR;
   else
     throw std::runtime_error("sigmoidCombine must be tensor, radial or radialPlus");
-  const int rec = c.get<int>(per == "fold" ? "embedding.foldRecursions" : "embedding.mergeRecursions", 99);
-  d.merge_recursions = rec == 99 ? 0 : (rec == 0 ? -1 : rec);
+  if (per == "cofold") {
+    const int rec = (int)c.get<double>("embedding.cofoldRecursions", 1.);
+    d.merge_recursions = rec == 0 ? -1 : rec;
+  } else {
+    const int rec =
+      c.get<int>(per == "fold" ? "embedding.foldRecursions" : "embedding.mergeRecursions", 99);
+    d.merge_recursions = rec == 99 ? 0 : (rec == 0 ? -1 : rec);
+  }
   d.sigmoid_start = c.get<double>("embedding.sigmoidStart", 1.);
   d.sigmoid_end = c.hasKey("embedding.sigmoidEnd") ? c.get<double>("embedding.sigmoidEnd") : 0.;
-  d.fold_max_factor = c.get<double>("embedding.foldMaxFactor", 1.);
+  d.fold_max_factor = per == "cofold" ? c.get<double>("embedding.cofoldMaxFactor", 5.)
+                                      : c.get<double>("embedding.foldMaxFactor", 1.);
   return d;
 }
 

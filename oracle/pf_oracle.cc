@@ -283,9 +283,9 @@ Geo make_geo(const pf_desc& d)
   g.factor = d.periodic ? 1 : d.embedding_factor;
   if (g.factor < 1)
     throw std::runtime_error("embedding.factor must be >= 1");
-  if (d.periodization < PF_PERIOD_CLASSICAL || d.periodization > PF_PERIOD_FOLD)
+  if (d.periodization < PF_PERIOD_CLASSICAL || d.periodization > PF_PERIOD_COFOLD)
     throw std::runtime_error(
-      "stochastic.periodization must be \"classical\", \"merge\" or \"fold\" (cofold is not built)");
+      "stochastic.periodization must be \"classical\", \"merge\", \"fold\", or \"cofold\"");
   if (d.sigmoid != PF_SIGMOID_SMOOTHSTEP && d.sigmoid != PF_SIGMOID_SMOOTH)
     throw std::runtime_error("embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
   if (d.sigmoid_combine < PF_COMBINE_RADIAL_PLUS || d.sigmoid_combine > PF_COMBINE_TENSOR)
@@ -735,33 +735,95 @@ double simpson_adapt(F& f, double a, double b, double fa, double fm, double fb, 
          simpson_adapt(f, m, b, fm, frm, fb, right, 0.5 * eps, depth - 1);
 }
 
-// Integral of the taper sigmoid(1, beta, x) over x in [1, norm], the quantity
-// the fold embedding computes with gsl_integration_qng (absolute tolerance
-// 1e-6, relative 1e-3; matrix.hh:866-868). The taper is zero from beta on.
-// Smoothstep: exact antiderivative. With u = 1 - 2 (x - 1) / (beta - 1) and
-// w = 1 - u^2 the taper is (1 + u S(u) / n) / 2, S = sum_j a_j w^j from the
-// recursion out_i = (2i out_{i-1} + w^i) / (2i + 1); Q(u) = sum_j a_j
+// Integral of the taper sigmoid(one, zero, x) over x in [one, hi]; the taper
+// is zero from `zero` on. Smoothstep: exact antiderivative. With
+// u = 1 - 2 (x - one) / (zero - one) and w = 1 - u^2 the taper is
+// (1 + u S(u) / n) / 2, S = sum_j a_j w^j from the recursion
+// out_i = (2i out_{i-1} + w^i) / (2i + 1); Q(u) = sum_j a_j
 // (-w^(j+1) / (2 (j + 1))) has Q' = u S and follows the same recursion, and
-// the integral is (beta - 1) / 2 * (G(1) - G(u)), G(u) = u / 2 + Q(u) / (2 n),
+// the integral is (zero - one) / 2 * (G(1) - G(u)), G(u) = u / 2 + Q(u) / (2 n),
 // G(1) = 1/2. Smooth: adaptive Simpson to 1e-14.
-double fold_integral(int sigmoid, double beta, double norm, unsigned rec)
+double taper_integral(int sigmoid, double one, double zero, double hi, unsigned rec)
 {
-  const double hi = std::min(norm, beta);
-  if (!(hi > 1.))
+  hi = std::min(hi, zero);
+  if (!(hi > one))
     return 0.;
   if (sigmoid == PF_SIGMOID_SMOOTHSTEP) {
-    const double u = 1. - 2. * (hi - 1.) / (beta - 1.), w = 1. - u * u;
+    const double u = 1. - 2. * (hi - one) / (zero - one), w = 1. - u * u;
     double n = 1., q = -0.5 * w, wp = w;
     for (unsigned i = 1; i <= rec; i++) {
       wp *= w;
       q = (2. * i * q - wp / (2. * (i + 1.))) / (2. * i + 1.);
       n *= 2. * i / (2. * i + 1.);
     }
-    return 0.5 * (beta - 1.) * (0.5 - (0.5 * u + 0.5 * q / n));
+    return 0.5 * (zero - one) * (0.5 - (0.5 * u + 0.5 * q / n));
   }
-  auto f = [&](double x) { return sigmoid_smooth(1., beta, x); };
-  const double fa = f(1.), fm = f(0.5 * (1. + hi)), fb = f(hi);
-  return simpson_adapt(f, 1., hi, fa, fm, fb, (hi - 1.) / 6. * (fa + 4. * fm + fb), 1e-14, 40);
+  auto f = [&](double x) { return sigmoid_smooth(one, zero, x); };
+  const double fa = f(one), fm = f(0.5 * (one + hi)), fb = f(hi);
+  return simpson_adapt(f, one, hi, fa, fm, fb, (hi - one) / 6. * (fa + 4. * fm + fb), 1e-14, 40);
+}
+
+// The fold embedding's integral of sigmoid(1, beta, x) over [1, norm]
+// (gsl_integration_qng in the reference, absolute tolerance 1e-6, relative
+// 1e-3; matrix.hh:866-868).
+double fold_integral(int sigmoid, double beta, double norm, unsigned rec)
+{
+  return taper_integral(sigmoid, 1., beta, norm, rec);
+}
+
+// Matrix::modifyMatrixEntriesWithCofold, matrix.hh:905-1013, on the classical
+// entries of a dense grid with C[i] points per axis, reached through at(f):
+// alpha = largest entry of norm > 1 on the border band of the extended
+// domain, beta = min(cofoldMaxFactor * alpha, smallest entry of max-norm
+// <= 1); entries of norm > 1 below alpha take offset = beta - I(beta), those
+// in [alpha, beta) I(x) + offset, with I(x) the integral of
+// sigmoid(0, beta - alpha, beta - t) over t in [alpha, x], i.e.
+// F(beta - alpha) - F(beta - x), F(y) the taper's integral over [0, y].
+template <typename At>
+void cofold(const pf_desc& d, const Geo& g, const Index C[3], At&& at)
+{
+  const Index total = C[0] * C[1] * C[2];
+  auto norm_of = [&](Index f, bool& border) {
+    const Index idx[3] = { f % C[0], (f / C[0]) % C[1], f / (C[0] * C[1]) };
+    double norm = 0.;
+    border = false;
+    for (int i = 0; i < g.dim; i++) {
+      double c = (double)idx[i] * g.h[i];
+      if (c > 0.5 * g.ext[i] * (double)g.factor)
+        c -= g.ext[i] * (double)g.factor;
+      norm = std::max(norm, std::fabs(c) / g.ext[i]);
+      if (0.5 * g.ext[i] * (double)g.factor - std::fabs(c) < g.ext[i] / (double)g.N[i])
+        border = true;
+    }
+    return norm;
+  };
+  double min_constrained = std::numeric_limits<double>::max();
+  double max_border = -std::numeric_limits<double>::max();
+  for (Index f = 0; f < total; f++) {
+    bool border;
+    const double norm = norm_of(f, border);
+    if (norm <= 1.)
+      min_constrained = std::min(min_constrained, at(f));
+    else if (border)
+      max_border = std::max(max_border, at(f));
+  }
+  const double max_factor = d.fold_max_factor != 0. ? d.fold_max_factor : 5.;
+  const unsigned rec =
+    d.merge_recursions > 0 ? (unsigned)d.merge_recursions : (d.merge_recursions < 0 ? 0u : 1u);
+  const double alpha = max_border, beta = std::min(max_factor * max_border, min_constrained);
+  const double w = beta - alpha;
+  auto F = [&](double y) { return taper_integral(d.sigmoid, 0., w, y, rec); };
+  const double offset = w > 0. ? beta - F(w) : beta;
+  for (Index f = 0; f < total; f++) {
+    bool border;
+    if (!(norm_of(f, border) > 1.))
+      continue;
+    const double x = at(f);
+    if (x < alpha)
+      at(f) = offset;
+    else if (x < beta)
+      at(f) = (F(w) - F(beta - x)) + offset;
+  }
 }
 
 // Entry value of the embedding grid at integer position idx:
@@ -852,6 +914,8 @@ Eigs eigs_dct(const pf_desc& d, const Geo& g, const double* custom)
       e.v[f] = custom ? custom[f] : embedded_cov(g, geo, cov, d, idx);
     }
   });
+  if (!custom && d.periodization == PF_PERIOD_COFOLD)
+    cofold(d, g, P, [&](Index f) -> double& { return e.v[f]; });
   for (int ax = 0; ax < g.dim; ax++)
     redft00_axis(e.v.data(), P, ax);
   for (auto& x : e.v)
@@ -883,6 +947,10 @@ Eigs eigs_r2c(const pf_desc& d, const Geo& g, const double* custom)
       }
     }
   });
+  if (!custom && d.periodization == PF_PERIOD_COFOLD) {
+    const Index C[3] = { N0, N1, N2 };
+    cofold(d, g, C, [&](Index f) -> double& { return real[(f / N0) * 2 * H + f % N0]; });
+  }
   std::vector<cd> spec(H * rows);
   r2c_x(real.data(), 2 * H, spec.data(), (int)N0, rows);
   Index dims[3] = { H, N1, N2 };
@@ -1716,6 +1784,8 @@ int pfo_cov_grid(const pf_desc* d, double* out)
         out[f] = embedded_cov(g, geo, cov, *d, idx);
       }
     });
+    if (d->periodization == PF_PERIOD_COFOLD)
+      cofold(*d, g, C, [&](Index f) -> double& { return out[f]; });
   });
   return PF_OK;
 }
@@ -1734,6 +1804,12 @@ double pfo_sigmoid(int kind, double one, double zero, double x, int rec)
 double pfo_fold_integral(int sigmoid, double beta, double norm, int rec)
 {
   return fold_integral(sigmoid, beta, norm, (unsigned)rec);
+}
+
+// Integral of sigmoid(one, zero, x) over [one, hi], for the tests.
+double pfo_taper_integral(int sigmoid, double one, double zero, double hi, int rec)
+{
+  return taper_integral(sigmoid, one, zero, hi, (unsigned)rec);
 }
 // Learned-PCG: preconditioner R F^{-1} diag(exp(L_bin)) F E.
 int pfo_pcg_learned(const pf_desc* d, const double* log_mult, int nbins, int maxit, double rel_tol,

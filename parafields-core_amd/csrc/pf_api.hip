@@ -421,9 +421,7 @@ Geo make_geo(const pf_desc& d)
     throw PfError(PF_EINVAL, "stochastic.anisotropy must be none, axiparallel or geometric");
   if (d.precision != PF_FP64 && d.precision != PF_FP32)
     throw PfError(PF_EINVAL, "precision must be PF_FP64 or PF_FP32");
-  if (d.periodization == PF_PERIOD_COFOLD)
-    throw PfError(PF_EINVAL, "cofold periodization is not built");
-  if (d.periodization < PF_PERIOD_CLASSICAL || d.periodization > PF_PERIOD_FOLD)
+  if (d.periodization < PF_PERIOD_CLASSICAL || d.periodization > PF_PERIOD_COFOLD)
     throw PfError(PF_EINVAL, "stochastic.periodization must be \"classical\", \"merge\", \"fold\", "
                              "or \"cofold\"");
   if (d.sigmoid != PF_SIGMOID_SMOOTHSTEP && d.sigmoid != PF_SIGMOID_SMOOTH)
@@ -509,6 +507,16 @@ CovParams make_cov(const pf_desc& d, const Geo& g)
     p.fold_beta = max_factor * 0.5 * (double)g.factor / std::sqrt((double)d.dim);
     for (int i = 0; i < 3; i++)
       p.ext[i] = g.ext[i];
+  }
+  if (p.period == PF_PERIOD_COFOLD) {
+    // matrix.hh:945-951: embedding.cofoldRecursions (default 1); the border
+    // band is one cell of the extended grid (matrix.hh:936-941)
+    p.sigmoid = d.sigmoid;
+    p.recursions = d.merge_recursions > 0 ? d.merge_recursions : (d.merge_recursions < 0 ? 0 : 1);
+    for (int i = 0; i < 3; i++) {
+      p.ext[i] = g.ext[i];
+      p.border[i] = g.ext[i] / (double)g.N[i];
+    }
   }
   if (p.period == PF_PERIOD_MERGE) {
     p.sigmoid = d.sigmoid;
@@ -597,6 +605,37 @@ unsigned grid1d(int64_t n, int threads = 256)
   return (unsigned)b;
 }
 
+// Cofold (Matrix::modifyMatrixEntriesWithCofold, matrix.hh:905-1013) on the
+// classical entries of the dense grid a[n2][n1][n0]: block minima / maxima on
+// the device, their reduction and the offset on the host, then the remap of
+// the entries beyond the domain on the device. alpha is the largest border
+// entry, beta = min(cofoldMaxFactor * alpha, smallest entry on the domain).
+void apply_cofold(Plan& P, const CovParams& cp, double* a, int n0, int n1, int n2)
+{
+  const int blocks = 512;
+  DevBuf part;
+  part.ensure(2 * blocks * sizeof(double));
+  k_cofold_bounds<<<blocks, 256, 0, P.stream>>>(cp, a, n0, n1, n2, part.as<double>());
+  check_launch();
+  std::vector<double> h(2 * blocks);
+  HIPCHECK(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          P.stream));
+  HIPCHECK(hipStreamSynchronize(P.stream));
+  part.release();
+  double min_constrained = 1.7976931348623157e308, max_border = -1.7976931348623157e308;
+  for (int b = 0; b < blocks; b++) {
+    if (h[2 * b] < min_constrained)
+      min_constrained = h[2 * b];
+    if (h[2 * b + 1] > max_border)
+      max_border = h[2 * b + 1];
+  }
+  const double max_factor = P.d.fold_max_factor != 0. ? P.d.fold_max_factor : 5.;
+  const double alpha = max_border, beta = std::min(max_factor * max_border, min_constrained);
+  const double offset = beta - cofold_integral(cp, alpha, beta, beta);
+  k_cofold_apply<<<blocks, 256, 0, P.stream>>>(cp, a, n0, n1, n2, alpha, beta, offset);
+  check_launch();
+}
+
 // ---------------------------------------------------------------------------
 // Eigenvalues: Matrix::fillTransformedMatrix, matrix.hh:223-318.
 // ---------------------------------------------------------------------------
@@ -650,6 +689,8 @@ bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial
       });
     });
     check_launch();
+    if (P.d.periodization == PF_PERIOD_COFOLD)
+      apply_cofold(P, cp, real, g.N[0], g.N[1], g.N[2]);
   }
   const FastCtx fc{ P.stream, fast_budget() };
   {
@@ -723,6 +764,8 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
         });
       });
       check_launch();
+      if (P.d.periodization == PF_PERIOD_COFOLD)
+        apply_cofold(P, cp, a, g.P[0], g.P[1], g.P[2]);
     }
     int64_t stride = 1;
     for (int ax = 0; ax < g.dim; ax++) {
@@ -790,6 +833,8 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
       });
     });
       check_launch();
+      if (P.d.periodization == PF_PERIOD_COFOLD)
+        apply_cofold(P, cp, real, g.N[0], g.N[1], g.N[2]);
     }
     {
       Sched s = make_sched((int)N0);
@@ -2322,6 +2367,9 @@ int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
     for (int i = 0; i < n; i++) {
       if (descs[i].precision != descs[0].precision)
         throw PfError(PF_EINVAL, "all fields of a batch must use one precision");
+      if (descs[i].periodization == PF_PERIOD_COFOLD)
+        throw PfError(PF_EINVAL, "cofold fields are not supported in a batch (the cofold offset "
+                                 "is reduced on the host between its two passes)");
       auto P = std::make_unique<Plan>();
       P->d = descs[i];
       P->g = make_geo(descs[i]);

@@ -217,6 +217,7 @@ struct CovParams
   int recursions;
   double sig_start, sig_end, ext[3];
   double fold_beta;   // fold: the taper reaches zero at this norm (matrix.hh:831-833)
+  double border[3];   // cofold: ext_i / extended cells_i, the border band (matrix.hh:936-941)
 };
 
 // Modified Bessel K_nu(x) (Temme series for x < 2, Steed's CF2 otherwise,
@@ -396,14 +397,14 @@ __device__ inline double cov_at(const CovParams& p, const double coord[3])
 // the Hermite interpolation matching `rec` derivatives (the regularized
 // incomplete beta function I_t(rec+1, rec+1) of the clamped value t), by the
 // reference's recursion.
-__device__ inline double sigmoid_clamp(double one, double zero, double x)
+__host__ __device__ inline double sigmoid_clamp(double one, double zero, double x)
 {
 #pragma clang fp contract(off)
   const double ax = fabs(x);
   return ax < one ? 1. : (ax > zero ? 0. : (zero - ax) / (zero - one));
 }
 
-__device__ inline double sigmoid_taper(const CovParams& p, double one, double zero, double x)
+__host__ __device__ inline double sigmoid_taper(const CovParams& p, double one, double zero, double x)
 {
 #pragma clang fp contract(off)
   if (p.sigmoid == PF_SIGMOID_SMOOTH) {
@@ -478,32 +479,52 @@ __device__ inline double merged_cov(const CovParams& p, const int64_t idx[3])
   return value;
 }
 
+// Integral of f over [lo, hi]: composite 5-point Gauss-Legendre over 16
+// panels (nodes 0, +-sqrt(5 -+ 2 sqrt(10/7)) / 3, weights 128/225,
+// (322 +- 13 sqrt(70)) / 900). The tapers below are smooth on the intervals
+// it is used on, and the rule integrates them to round-off for the recursion
+// counts in use.
+template <typename F>
+__host__ __device__ inline double gauss_legendre_16x5(F&& f, double lo, double hi)
+{
+  const double r10 = sqrt(10. / 7.), r70 = sqrt(70.);
+  const double x1 = sqrt(5. - 2. * r10) / 3., x2 = sqrt(5. + 2. * r10) / 3.;
+  const double w0 = 128. / 225., w1 = (322. + 13. * r70) / 900., w2 = (322. - 13. * r70) / 900.;
+  const int panels = 16;
+  const double half = 0.5 * (hi - lo) / panels;
+  double acc = 0.;
+  for (int k = 0; k < panels; k++) {
+    const double c = lo + (2 * k + 1) * half;
+    acc += w0 * f(c);
+    acc += w1 * (f(c - half * x1) + f(c + half * x1));
+    acc += w2 * (f(c - half * x2) + f(c + half * x2));
+  }
+  return acc * half;
+}
+
 // Integral of the taper sigmoid(1, fold_beta, x) over x in [1, norm]. The
-// taper is zero from fold_beta on, so the integral stops there; below that it
-// is smooth, and a composite 5-point Gauss-Legendre rule over 16 panels
-// (nodes 0, +-sqrt(5 -+ 2 sqrt(10/7)) / 3, weights 128/225,
-// (322 +- 13 sqrt(70)) / 900) integrates it to round-off for the recursion
-// counts in use. The reference calls gsl_integration_qng with absolute
-// tolerance 1e-6 and relative tolerance 1e-3 on [1, norm] (matrix.hh:866-868).
+// taper is zero from fold_beta on, so the integral stops there. The reference
+// calls gsl_integration_qng with absolute tolerance 1e-6 and relative
+// tolerance 1e-3 on [1, norm] (matrix.hh:866-868).
 __device__ inline double fold_integral(const CovParams& p, double norm)
 {
   const double b = p.fold_beta;
   const double hi = norm < b ? norm : b;
   if (!(hi > 1.))
     return 0.;
-  const double r10 = sqrt(10. / 7.), r70 = sqrt(70.);
-  const double x1 = sqrt(5. - 2. * r10) / 3., x2 = sqrt(5. + 2. * r10) / 3.;
-  const double w0 = 128. / 225., w1 = (322. + 13. * r70) / 900., w2 = (322. - 13. * r70) / 900.;
-  const int panels = 16;
-  const double half = 0.5 * (hi - 1.) / panels;
-  double acc = 0.;
-  for (int k = 0; k < panels; k++) {
-    const double c = 1. + (2 * k + 1) * half;
-    acc += w0 * sigmoid_taper(p, 1., b, c);
-    acc += w1 * (sigmoid_taper(p, 1., b, c - half * x1) + sigmoid_taper(p, 1., b, c + half * x1));
-    acc += w2 * (sigmoid_taper(p, 1., b, c - half * x2) + sigmoid_taper(p, 1., b, c + half * x2));
-  }
-  return acc * half;
+  return gauss_legendre_16x5([&](double x) { return sigmoid_taper(p, 1., b, x); }, 1., hi);
+}
+
+// Cofold (matrix.hh:946-961): integral of the codomain taper
+// sigmoid(0, beta - alpha, beta - t) over t in [alpha, x], alpha <= x <= beta.
+__host__ __device__ inline double cofold_integral(const CovParams& p, double alpha, double beta,
+                                                  double x)
+{
+  if (!(x > alpha))
+    return 0.;
+  const double w = beta - alpha;
+  return gauss_legendre_16x5([&](double t) { return sigmoid_taper(p, 0., w, beta - t); }, alpha,
+                             x);
 }
 
 // Covariance at the embedding grid position idx (integer cell offsets):

@@ -77,6 +77,86 @@ __global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out
   });
 }
 
+// Cofold (matrix.hh:905-1013) on a dense grid a[n2][n1][n0] of classical
+// entries (the DCT half grid or the full R2C grid): max-norm of the wrapped
+// coordinates in units of the extensions, and whether the entry lies in the
+// border band of the extended domain along some axis.
+__device__ inline double cofold_norm(const CovParams& p, const int64_t idx[3], bool& border)
+{
+  double norm = 0.;
+  border = false;
+  for (int i = 0; i < p.dim; i++) {
+    double c = (double)idx[i] * p.h[i];
+    if (c > p.wrap_half[i])
+      c -= p.wrap_len[i];
+    if (fabs(c) / p.ext[i] > norm)
+      norm = fabs(c) / p.ext[i];
+    if (p.wrap_half[i] - fabs(c) < p.border[i])
+      border = true;
+  }
+  return norm;
+}
+
+// Pass 1 (matrix.hh:913-943): per block, the minimum over entries of norm
+// <= 1 and the maximum over border entries of norm > 1; part[2b], part[2b+1].
+__global__ void __launch_bounds__(256) k_cofold_bounds(CovParams p, const double* a, int n0,
+                                                       int n1, int n2, double* part)
+{
+  __shared__ double smin[256], smax[256];
+  double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+  const int64_t total = (int64_t)n0 * n1 * n2;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * 256) {
+    const int64_t idx[3] = { k % n0, (k / n0) % n1, k / ((int64_t)n0 * n1) };
+    bool border;
+    const double norm = cofold_norm(p, idx, border);
+    const double v = a[k];
+    if (norm <= 1.) {
+      if (v < mn)
+        mn = v;
+    } else if (border && v > mx)
+      mx = v;
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      if (smin[threadIdx.x + s] < smin[threadIdx.x])
+        smin[threadIdx.x] = smin[threadIdx.x + s];
+      if (smax[threadIdx.x + s] > smax[threadIdx.x])
+        smax[threadIdx.x] = smax[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// Pass 2 (matrix.hh:968-1006): entries of norm > 1 below alpha take the
+// offset, entries in [alpha, beta) the offset plus the taper's integral from
+// alpha, the others stay.
+__global__ void __launch_bounds__(256) k_cofold_apply(CovParams p, double* a, int n0, int n1,
+                                                      int n2, double alpha, double beta,
+                                                      double offset)
+{
+  const int64_t total = (int64_t)n0 * n1 * n2;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * 256) {
+    const int64_t idx[3] = { k % n0, (k / n0) % n1, k / ((int64_t)n0 * n1) };
+    bool border;
+    if (!(cofold_norm(p, idx, border) > 1.))
+      continue;
+    const double x = a[k];
+    if (x < alpha)
+      a[k] = offset;
+    else if (x < beta)
+      a[k] = cofold_integral(p, alpha, beta, x) + offset;
+  }
+}
+
 // REDFT00 (FFTW's DCT-I) along one axis of a real FP64 array, computed as the
 // DFT of the even extension of length Nfft = 2(P-1); in place. Lines of the
 // axis: stride `stride`, P points. LM = true for the contiguous axis.

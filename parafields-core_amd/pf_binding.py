@@ -141,9 +141,10 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
     none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy.
     periodization / sigmoid / sigmoid_combine take the reference's names
     (embedding.periodization, embedding.sigmoid, stochastic.sigmoidCombine);
-    merge_recursions (embedding.mergeRecursions, or embedding.foldRecursions for
-    fold), sigmoid_start / sigmoid_end and fold_max_factor 0 mean the reference
-    defaults, merge_recursions -1 means zero recursions."""
+    merge_recursions (embedding.mergeRecursions, foldRecursions or
+    cofoldRecursions), sigmoid_start / sigmoid_end and fold_max_factor
+    (foldMaxFactor or cofoldMaxFactor) 0 mean the reference defaults,
+    merge_recursions -1 means zero recursions."""
     d = PfDesc()
     d.dim = dim
     for i in range(3):

@@ -49,6 +49,7 @@ def load():
         "pfo_cov_grid": (i32, [D, vp]),
         "pfo_sigmoid": (dp, [i32, dp, dp, dp, i32]),
         "pfo_fold_integral": (dp, [i32, dp, dp, i32]),
+        "pfo_taper_integral": (dp, [i32, dp, dp, dp, i32]),
         "pfo_pcg_learned": (i32, [D, vp, i32, i32, dp, vp, vp, ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_double)]),
         "pfo_spectral_targets": (i32, [D, i32, dp, vp]),
@@ -185,6 +186,12 @@ def fold_integral(sigmoid, beta, norm, rec=0):
     sigmoid: 'smoothstep' or 'smooth'."""
     k = {"smoothstep": 0, "smooth": 1}[sigmoid]
     return load().pfo_fold_integral(k, beta, norm, rec)
+
+
+def taper_integral(sigmoid, one, zero, hi, rec=0):
+    """Integral of sigmoid(one, zero, x) over [one, hi]."""
+    k = {"smoothstep": 0, "smooth": 1}[sigmoid]
+    return load().pfo_taper_integral(k, one, zero, hi, rec)
 
 
 def bessel_k(nu, x):

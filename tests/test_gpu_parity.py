@@ -151,6 +151,47 @@ def test_fold_generate_matches_oracle():
     assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < FP64_TOL
 
 
+COFOLD_CASES = [
+    (2, [48, 40], dict(covariance="exponential", corr_length=0.3, factor=3)),
+    (2, [32, 24], dict(covariance="matern52", corr_length=0.2, factor=3, sigmoid="smooth",
+                       merge_recursions=-1)),
+    (3, [12, 10, 8], dict(covariance="matern32", anisotropy="axiparallel",
+                          corr_length=[0.3, 0.2, 0.25], factor=3, fold_max_factor=1.5,
+                          merge_recursions=3)),
+    (2, [24, 20], dict(covariance="gaussian", anisotropy="geometric",
+                       corr_length=[0.2, 0.1, 0.05, 0.3], factor=3)),
+    (3, [8, 8, 6], dict(covariance="exponential", corr_length=0.3, factor=4, matrix="r2c")),
+]
+
+
+@pytest.mark.parametrize("dim,cells,kw", COFOLD_CASES)
+def test_cofold_eigs_match_oracle(dim, cells, kw):
+    """Cofold: block minima / maxima on the device and their reduction on the
+    host, then the remap with the device's Gauss-Legendre integral; the oracle
+    reduces serially and integrates in closed form or by adaptive Simpson."""
+    d = desc(dim, cells, periodization="cofold", approximate=True, **kw)
+    p = pfb.Plan(d)
+    status, st = p.eigs_stats_nothrow()
+    want, wst, wstatus = orc.eigs(d, allow_negative=True)
+    assert status == wstatus
+    assert rel_max(p.eig_values(), want) < FP64_TOL
+    for k in ("small", "zero", "negative", "entries"):
+        assert st[k] == wst[k], (k, st, wst)
+
+
+def test_cofold_generate_matches_oracle_and_batches_reject_it():
+    d = desc(2, [48, 40], covariance="exponential", corr_length=0.3, factor=3,
+             periodization="cofold", approximate=True)
+    p = pfb.Plan(d)
+    p.eigs()
+    noise = orc.noise_cpp(37, int(p.sizes.noise_per_field))
+    assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < FP64_TOL
+    d32 = desc(2, [48, 40], covariance="exponential", corr_length=0.3, factor=3,
+               periodization="cofold", approximate=True, precision="fp32")
+    with pytest.raises(RuntimeError, match="cofold"):
+        pfb.Batch([desc(2, [16, 16], precision="fp32"), d32])
+
+
 def test_merge_generate_matches_oracle():
     """A field on the merge embedding (factor 3, no negative eigenvalues left)
     from host noise and from device Philox noise, FP64 and FP32."""
@@ -173,9 +214,6 @@ def test_merge_rejects_unbuilt_periodizations():
     with pytest.raises(RuntimeError):
         pfb.Plan(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="merge"))
     d = desc(2, [8, 8], factor=3)
-    d.periodization = 3  # cofold is not built
-    with pytest.raises(RuntimeError):
-        pfb.Plan(d)
     d.periodization = 4
     with pytest.raises(RuntimeError):
         pfb.Plan(d)

@@ -629,10 +629,77 @@ def test_fold_grid_invariants(sigmoid):
     assert inner and beyond and between
 
 
-def test_fold_rejects_cofold_and_family_without_default_count():
+def test_fold_rejects_unknown_periodization_and_family_without_default_count():
+    d = desc(2, [8, 8], covariance="exponential")
+    d.periodization = 4
     with pytest.raises(RuntimeError):
-        orc.cov_grid(desc(2, [8, 8], covariance="exponential", periodization="cofold"))
+        orc.cov_grid(d)
     with pytest.raises(RuntimeError):
         orc.cov_grid(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="fold"))
     orc.cov_grid(desc(2, [8, 8], covariance="gaussian", factor=3, periodization="fold",
                       merge_recursions=3))
+
+
+# Cofold (matrix.hh:905-1013): the classical entries beyond the domain (max-norm
+# > 1) are lifted smoothly: below alpha (largest border entry) to the offset,
+# in [alpha, beta) along the integral of the codomain taper, beta =
+# min(cofoldMaxFactor * alpha, smallest entry on the domain).
+
+def _cofold_expect(classic, cells, ext, factor, sigmoid, rec, max_factor):
+    """Independent restatement on the DCT half grid with scipy quadrature."""
+    from scipy import integrate
+    N = [c * factor for c in cells]
+    h = [ext[i] / cells[i] for i in range(2)]
+    j, i = np.meshgrid(np.arange(classic.shape[0]), np.arange(classic.shape[1]), indexing="ij")
+    x, y = i * h[0], j * h[1]
+    norm = np.maximum(np.abs(x) / ext[0], np.abs(y) / ext[1])
+    border = ((0.5 * ext[0] * factor - np.abs(x)) < ext[0] / N[0]) | \
+             ((0.5 * ext[1] * factor - np.abs(y)) < ext[1] / N[1])
+    alpha = classic[(norm > 1) & border].max()
+    beta = min(max_factor * alpha, classic[norm <= 1].min())
+    w = beta - alpha
+    offset = 0.5 * (alpha + beta)  # the taper's symmetry: its integral over [alpha, beta] is w / 2
+    want = classic.copy()
+    for (jj, ii) in zip(*np.nonzero(norm > 1)):
+        v = classic[jj, ii]
+        if v < alpha:
+            want[jj, ii] = offset
+        elif v < beta:
+            I, _ = integrate.quad(lambda t: orc.sigmoid(sigmoid, 0.0, w, beta - t, rec), alpha, v,
+                                  epsabs=1e-17, epsrel=1e-13)
+            want[jj, ii] = I + offset
+    return want, norm, alpha, beta
+
+
+@pytest.mark.parametrize("sigmoid,rec,max_factor", [("smoothstep", 1, 5.0), ("smoothstep", 4, 1.2),
+                                                    ("smooth", 0, 5.0)])
+def test_cofold_grid_matches_restatement(sigmoid, rec, max_factor):
+    cells, ext, factor = [24, 20], [1.0, 1.0], 3
+    kw = dict(covariance="exponential", corr_length=0.3, factor=factor, matrix="dct",
+              extensions=ext)
+    classic = orc.cov_grid(desc(2, cells, **kw))
+    got = orc.cov_grid(desc(2, cells, periodization="cofold", sigmoid=sigmoid,
+                            merge_recursions=rec if rec else -1, fold_max_factor=max_factor,
+                            **kw))
+    want, norm, alpha, beta = _cofold_expect(classic, cells, ext, factor, sigmoid, rec, max_factor)
+    assert np.array_equal(got[norm <= 1], classic[norm <= 1])
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(classic).max()
+    # the lifted entries lie in [offset, beta) and keep the order of the originals
+    outer = norm > 1
+    lifted = outer & (classic < beta)
+    assert lifted.any() and (classic[outer] < alpha).any()
+    assert got[lifted].min() >= 0.5 * (alpha + beta) - 1e-15
+    order = np.argsort(classic[lifted], kind="stable")
+    assert np.all(np.diff(got[lifted][order]) >= -1e-16)
+
+
+def test_taper_integral_matches_quadrature():
+    from scipy import integrate
+    for sigmoid, rec in (("smoothstep", 1), ("smoothstep", 5), ("smooth", 0)):
+        for one, zero in ((0.0, 2e-3), (0.5, 0.9)):
+            for frac in (0.2, 0.6, 1.0):
+                hi = one + frac * (zero - one)
+                want, _ = integrate.quad(lambda x: orc.sigmoid(sigmoid, one, zero, x, rec), one,
+                                         hi, epsabs=1e-18, epsrel=1e-13)
+                got = orc.taper_integral(sigmoid, one, zero, hi, rec)
+                assert abs(got - want) <= 1e-12 * abs(want), (sigmoid, rec, one, zero, frac)
