@@ -19,3 +19,14 @@ def test_trend_part(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     print(r.stdout)
     assert r.returncode == 0 and "all OK" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_config_reader_periodization_keys(tmp_path):
+    """make_desc: classical / merge / fold / cofold keys and their defaults."""
+    exe = str(tmp_path / "config_check")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "host", "config_check.cc"), "-o", exe])
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0 and "all OK" in r.stdout, r.stdout + r.stderr
