@@ -136,6 +136,11 @@ def run_gpu(steps, warmup, batch):
         print(f"bench: step {s + 1}/{steps} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # realization 0 of the last timed step (Philox realization r0) as the
+    # Gaussian field, i.e. the log of its log-normal output, for the error
+    # against the FP64 oracle on identical noise
+    r0 = (warmup + steps - 1) * batch
+    field0 = out[0].double().log().cpu().numpy()
     # after the timed steps: one step with the kernels run one after another
     # (no two-stream overlap), for the launch times of each kernel alone
     prev = os.environ.get("PF_OVERLAP")
@@ -149,11 +154,7 @@ def run_gpu(steps, warmup, batch):
     else:
         os.environ["PF_OVERLAP"] = prev
     set_profiling(plan, False)
-    sample = out[0].float().log().cpu().numpy()
-    stats = {"mean_log": float(sample.mean()), "var_log": float(sample.var())}
-    # realization 0 as a Gaussian field (no transform), for the error against
-    # the FP64 oracle on identical Philox noise
-    field0 = plan.generate(1, seed=SEED, first=0)[0]
+    stats = {"mean_log": float(field0.mean()), "var_log": float(field0.var()), "realization": r0}
     # PCIe-inclusive rate: 16 realizations into a host buffer (the library
     # stages through its own device buffers and copies back over the link)
     host = np.empty((16, plan.domain), dtype=np.float32)
@@ -170,7 +171,7 @@ def run_gpu(steps, warmup, batch):
     return elapsed, per_kernel, st, eig_s, stats, field0
 
 
-def run_gpu_fp64_single(reps):
+def run_gpu_fp64_single(reps, r0):
     """Plain FP64 per-realization path on the same grid (batch 1)."""
     import torch
     d = pfb.make_desc(3, CELLS, [1.0, 1.0, 1.0], precision="fp64", **KW)
@@ -184,7 +185,7 @@ def run_gpu_fp64_single(reps):
         plan.generate(1, seed=SEED, first=r, transform="logNormal", out=out)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    field0 = plan.generate(1, seed=SEED, first=0)[0]
+    field0 = plan.generate(1, seed=SEED, first=r0)[0]
     plan.close()
     del out
     torch.cuda.empty_cache()
@@ -201,33 +202,72 @@ def field_error(got, want):
             "rel_l2": float(np.linalg.norm(g - w) / np.linalg.norm(w))}
 
 
-def run_cpu_baseline(gpu_fields=None):
-    """FP64 CPU oracle (restatement of the reference loops) on one realization
-    of the same workload: Philox noise + generate + log-normal transform.
-    Eigenvalue setup is excluded (timed separately). The oracle's Gaussian
-    field (realization 0) is then the reference for the GPU fields of the
-    same noise (`gpu_fields`: name -> field)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(r0, gpu_fields=None, reps=3, single_thread=True):
+    """FP64 CPU oracle (restatement of the reference loops, threaded over
+    transform lines and noise samples) on the same workload: Philox noise of
+    one realization + generate + log-normal transform per field; eigenvalue
+    setup excluded (timed separately). All host cores (at most 16): one warm-up
+    field, then the median of `reps` fields. One thread: one field. The
+    oracle's Gaussian field of realization r0 is then the reference for the
+    GPU fields of the same noise (`gpu_fields`: name -> field)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as orc
-    cores = os.cpu_count() or 1
-    cores = min(cores, 16)
-    orc.set_threads(cores)
+    nproc = os.cpu_count() or 1
+    cores = min(nproc, 16)
     d = pfb.make_desc(3, CELLS, [1.0, 1.0, 1.0], precision="fp64", **KW)
+    n = int(orc.sizes(d).noise_per_field)
+
+    def one(real):
+        t0 = time.perf_counter()
+        nz = orc.noise_philox(SEED, real, 0, n)
+        f = orc.generate(d, nz)
+        del nz
+        g = f.copy()
+        orc.value_transform(g, "logNormal")
+        return time.perf_counter() - t0, f
+
+    orc.set_threads(cores)
     t0 = time.perf_counter()
     orc.eigs(d, allow_negative=True)
     eig_s = time.perf_counter() - t0
-    n = int(orc.sizes(d).noise_per_field)
-    t0 = time.perf_counter()
-    nz = orc.noise_philox(SEED, 0, 0, n)
-    f = orc.generate(d, nz)
-    orc.value_transform(f, "logNormal")
-    el = time.perf_counter() - t0
-    acc = {k: field_error(v, f) for k, v in (gpu_fields or {}).items() if v is not None}
-    return {"value": 1.0 / el, "unit": "fields/s", "cores": cores, "kind": "port",
+    print("bench: cpu baseline warm-up", file=sys.stderr, flush=True)
+    _, f0 = one(r0)
+    times = []
+    for i in range(reps):
+        el, _ = one(r0 + 1 + i)
+        times.append(el)
+        print(f"bench: cpu baseline {cores} threads field {i + 1}/{reps}: {el:.2f} s",
+              file=sys.stderr, flush=True)
+    med = float(np.median(times))
+    single = None
+    if single_thread:
+        orc.set_threads(1)
+        el1, _ = one(r0 + 1 + reps)
+        print(f"bench: cpu baseline 1 thread: {el1:.2f} s", file=sys.stderr, flush=True)
+        single = {"value": round(1.0 / el1, 5), "unit": "fields/s", "cores": 1,
+                  "s_per_field": round(el1, 2), "fields": 1}
+        orc.set_threads(cores)
+    acc = {k: field_error(v, f0) for k, v in (gpu_fields or {}).items() if v is not None}
+    return {"value": round(1.0 / med, 5), "unit": "fields/s", "cores": cores, "kind": "port",
             "accuracy_vs_oracle": acc,
-            "sample": f"1 realization of the same 256^3/512^3 workload on the FP64 CPU oracle "
-                      f"({cores} threads over transform lines; noise generation single-threaded); "
-                      f"{el:.1f} s per field, eigenvalue setup {eig_s:.1f} s excluded"}
+            "s_per_field_median": round(med, 3), "s_per_field_all": [round(t, 3) for t in times],
+            "single_thread": single, "nproc": nproc, "cpu_model": cpu_model(),
+            "eig_setup_s": round(eig_s, 2),
+            "sample": f"FP64 CPU oracle on the same 256^3/512^3 workload, one realization per "
+                      f"field (Philox noise, generation, log-normal transform), threaded over "
+                      f"noise samples and transform lines: {cores} threads, 1 warm-up field "
+                      f"then the median of {reps} fields; 1 thread: one field; eigenvalue "
+                      f"setup ({eig_s:.1f} s) excluded"}
 
 
 def run_uneven(nfields=1000, seed=3, reps=3):
@@ -375,6 +415,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-single", action="store_true",
+                    help="skip the one-thread CPU baseline field (about a minute)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a 1 GiB device-to-device copy of known size (calibrates "
@@ -431,7 +473,8 @@ def main():
     for k in kernels:
         if k in alone:
             kernels[k]["ms_per_field_alone"] = round(float(np.mean(alone[k])) / a.batch, 4)
-    fp64_fps, field64 = (None, None) if a.no_fp64 else run_gpu_fp64_single(3)
+    r0 = stats.pop("realization")
+    fp64_fps, field64 = (None, None) if a.no_fp64 else run_gpu_fp64_single(3, r0)
     extra = {}
     legs = []
     if a.extra:
@@ -445,8 +488,9 @@ def main():
                 extra[name] = fn()
             except Exception as e:  # a failing leg is reported, not hidden
                 extra[name] = {"error": repr(e)}
-    cpu = None if a.no_cpu else run_cpu_baseline({"fp32_batched": field32,
-                                                  "fp64_per_realization": field64})
+    cpu = None if a.no_cpu else run_cpu_baseline(r0, {"fp32_batched": field32,
+                                                      "fp64_per_realization": field64},
+                                                 single_thread=not a.no_cpu_single)
     accuracy = None if cpu is None else cpu.pop("accuracy_vs_oracle")
     line = {
         "metric": METRIC,
@@ -491,9 +535,11 @@ def main():
         "field_log_stats": stats,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_oracle": None if accuracy is None else dict(
-            accuracy, realization=0,
-            note="Gaussian field (before the log-normal transform) against the FP64 CPU oracle "
-                 "on identical Philox noise; bars: FP32 max_abs_over_std <= 2e-5, FP64 "
+            accuracy, realization=r0,
+            note="realization 0 of the last timed batched step (Philox realization r0) as a "
+                 "Gaussian field (log of its log-normal output) and the FP64 per-realization "
+                 "path's field of the same realization, against the FP64 CPU oracle on "
+                 "identical Philox noise; bars: FP32 max_abs_over_std <= 2e-5, FP64 "
                  "max_abs_over_max <= 1e-11"),
         "other_configs": extra,
     }

@@ -1504,20 +1504,31 @@ void philox_normals(uint64_t seed, uint64_t real, uint64_t first, Index n, doubl
 {
   const uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
   const double TWO_PI = 6.283185307179586476925286766559;
-  for (Index i = 0; i < n; i++) {
-    const uint64_t s = first + (uint64_t)i;
-    const uint64_t blk = s >> 2;
-    uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
-                      (uint32_t)(real >> 32) };
-    philox4x32_10(c, key);
-    const int lane = (int)(s & 3);
-    const int pa = lane & 2;
-    const double ua = ((double)c[pa] + 0.5) * (1.0 / 4294967296.0);
-    const double ub = ((double)c[pa + 1] + 0.5) * (1.0 / 4294967296.0);
-    const double r = std::sqrt(-2.0 * std::log(ua));
-    const double t = TWO_PI * ub;
-    out[i] = (lane & 1) ? r * std::sin(t) : r * std::cos(t);
-  }
+  // samples s and s + 1 (s even) are the cosine and sine normals of one
+  // Box-Muller pair; the four samples of block s / 4 share one Philox call
+  parallel_for(n, [&](Index b, Index e) {
+    uint64_t have = ~(uint64_t)0;
+    uint32_t c[4] = { 0, 0, 0, 0 };
+    for (Index i = b; i < e; i++) {
+      const uint64_t s = first + (uint64_t)i;
+      const uint64_t blk = s >> 2;
+      if (blk != have) {
+        c[0] = (uint32_t)blk;
+        c[1] = (uint32_t)(blk >> 32);
+        c[2] = (uint32_t)real;
+        c[3] = (uint32_t)(real >> 32);
+        philox4x32_10(c, key);
+        have = blk;
+      }
+      const int lane = (int)(s & 3);
+      const int pa = lane & 2;
+      const double ua = ((double)c[pa] + 0.5) * (1.0 / 4294967296.0);
+      const double ub = ((double)c[pa + 1] + 0.5) * (1.0 / 4294967296.0);
+      const double r = std::sqrt(-2.0 * std::log(ua));
+      const double t = TWO_PI * ub;
+      out[i] = (lane & 1) ? r * std::sin(t) : r * std::cos(t);
+    }
+  });
 }
 
 // ValueTransform, mutators.hh:35-163.

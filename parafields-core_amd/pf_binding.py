@@ -190,6 +190,15 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    # torch ships its own HIP runtime (SONAME libamdhip64.so.7, loaded by the
+    # unversioned name). Loaded first, it also serves this library's
+    # dependency; loaded after this library pulled in /opt/rocm's copy, it
+    # becomes a second runtime in the process and finds no GPU. So torch, when
+    # present, is imported before the library is opened.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     vp, i32, i64, u64, dp = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
                              ctypes.c_double)

@@ -449,6 +449,59 @@ def test_config2_scale_fp32_vs_fp64_oracle():
     assert err < FP32_TOL
 
 
+CONFIG2_KW = dict(covariance="matern32", variance=2.0, anisotropy="axiparallel",
+                  corr_length=[0.1, 0.1, 0.05], approximate=True)
+
+
+def test_batch64_pipeline_matches_oracle_at_chunk_boundaries():
+    """A batch of 64 runs the two-stream pipeline (8 chunks of 8 realizations).
+    On a config-2-shaped field (3D 64^3 cells, 128^3 embedding, Matern-3/2,
+    axiparallel, DCT matrix + R2C field, FP32), the first and last
+    realizations of chunks (0, 7, 8, 63) match the FP64 oracle on identical
+    Philox noise and equal the single-realization path bit for bit."""
+    cells = [64, 64, 64]
+    p = pfb.Plan(desc(3, cells, precision="fp32", **CONFIG2_KW))
+    d64 = desc(3, cells, **CONFIG2_KW)
+    npf = int(p.sizes.noise_per_field)
+    out = p.generate(64, seed=7, first=1000)
+    for r in (0, 7, 8, 63):
+        single = p.generate(1, seed=7, first=1000 + r)[0]
+        assert np.array_equal(out[r], single), r
+        want = orc.generate(d64, orc.noise_philox(7, 1000 + r, 0, npf))
+        err = np.abs(out[r] - want).max() / want.std()
+        print(f"batch-64 realization {r}: max-abs/std {err:.3e}", flush=True)
+        assert err < FP32_TOL, r
+
+
+def test_config2_batch64_logNormal_full_size():
+    """The bench workload itself: 64 FP32 realizations of the 256^3-cell field
+    on the 512^3 embedding, log-normal transform, into device memory through
+    the two-stream pipeline. Realizations 0, 8 and 63 equal the
+    single-realization path bit for bit; log of realization 63 matches the FP64
+    oracle's Gaussian field on identical Philox noise (seed 7)."""
+    import torch
+    cells = [256, 256, 256]
+    p = pfb.Plan(desc(3, cells, precision="fp32", **CONFIG2_KW))
+    p.eigs()
+    out = torch.empty((64, p.domain), dtype=torch.float32, device="cuda")
+    p.generate(64, seed=7, first=0, transform="logNormal", out=out)
+    torch.cuda.synchronize()
+    for r in (0, 8, 63):
+        single = p.generate(1, seed=7, first=r, transform="logNormal")[0]
+        assert np.array_equal(out[r].cpu().numpy(), single), r
+    got = np.log(out[63].cpu().numpy().astype(np.float64))
+    del out
+    p.close()
+    torch.cuda.empty_cache()
+    orc.set_threads(16)
+    d64 = desc(3, cells, **CONFIG2_KW)
+    want = orc.generate(d64, orc.noise_philox(7, 63, 0, int(orc.sizes(d64).noise_per_field)))
+    err = np.abs(got - want).max() / want.std()
+    print(f"config-2 batch-64 realization 63 (log of logNormal output) vs FP64 oracle: "
+          f"max-abs/std {err:.3e}, rel-L2 {nh.rel_l2(got, want):.3e}", flush=True)
+    assert err < FP32_TOL
+
+
 FASTPATH_CASES = [
     (2, [256, 256], dict(covariance="exponential", corr_length=0.05)),
     (3, [32, 48, 64], dict(covariance="matern32", anisotropy="axiparallel",

@@ -297,6 +297,70 @@ def test_generate_single_mode_noise_order_3d():
             np.testing.assert_allclose(f, want, atol=1e-13)
 
 
+def _dctdst_single_sample_cases():
+    cases = []
+    for dim, cells in ((2, [3, 5]), (3, [2, 3, 4])):
+        for tr in (0, 1):
+            for t in range(1 << dim):
+                cases.append((dim, cells, tr, t))
+    return cases
+
+
+@pytest.mark.parametrize("dim,cells,tr,typ", _dctdst_single_sample_cases())
+def test_generate_dctdst_single_sample_closed_form(dim, cells, tr, typ):
+    """One unit sample at storage index s of parity type t gives the closed form
+    2^{-d/2} 2^{border/2} sqrt(m_k) prod_i B_i(k_i, x_i), with B = c cos(2 pi k x / N)
+    on even axes (c = 1 on the boundary k in {0, N/2}, else 2: inverse REDFT00) and
+    B = 2 sin(2 pi k x / N) on odd axes (inverse RODFT00 on the interior; a boundary
+    index of an odd axis gives the zero field). Type bit i is the parity of axis i
+    (x = bit 0; dctdstfieldbackend.hh:149-165); the samples of type t are the
+    t-th block of prod P_i in the noise stream (matrix.hh:342-372); within a block
+    the storage order is x fastest with the two last axes swapped when the
+    transform is transposed (3D: x, z, y; 2D: y, x; dctdstfieldbackend.hh:859-872).
+    Distinct P_i per axis make a swapped order or a reversed bit order fail."""
+    d = desc(dim, cells, covariance="exponential", corr_length=[0.15], field="dctdst",
+             matrix="dct", transposed=tr, approximate=True)
+    s = orc.sizes(d)
+    N = [2 * c for c in cells]
+    P = [n // 2 + 1 for n in N]
+    count = int(np.prod(P))
+    assert s.noise_per_field == count << dim
+    full = nh.full_eigs(cells, [1.0] * dim, "exponential", "none", [0.15]).real
+    x = nh.cell_coords(cells, [float(c) for c in cells]).round().astype(int)  # integer cell index
+    order = list(range(dim))  # storage axes, fastest first
+    if tr:
+        order[-1], order[-2] = order[-2], order[-1]
+    odd = [(typ >> i) & 1 for i in range(dim)]
+    rng = np.random.default_rng(100 + typ + 10 * tr)
+    picks = {0, count - 1, 1, P[order[0]], P[order[0]] + 1}
+    picks |= set(int(v) for v in rng.integers(0, count, 6))
+    for sidx in sorted(picks):
+        k = [0] * dim
+        r = sidx
+        for ax in order:
+            k[ax] = r % P[ax]
+            r //= P[ax]
+        m = full[tuple(k[::-1])]
+        assert m > 0
+        want = np.full(x.shape[0], np.sqrt(m) * 2.0 ** (-dim / 2))
+        zero = False
+        border = 0
+        for i in range(dim):
+            edge = k[i] in (0, P[i] - 1)
+            ph = 2 * np.pi * k[i] * x[:, i] / N[i]
+            if odd[i]:
+                zero |= edge
+                want = want * 2 * np.sin(ph)
+            else:
+                border += edge
+                want = want * (1.0 if edge else 2.0) * np.cos(ph)
+        want = 0 * want if zero else want * 2.0 ** (border / 2)
+        z = np.zeros(s.noise_per_field)
+        z[typ * count + sidx] = 1.0
+        f = orc.generate(d, z)
+        np.testing.assert_allclose(f, want, atol=1e-13, err_msg=f"s={sidx} k={k}")
+
+
 @pytest.mark.parametrize("dim,cells,kind,corr,aniso,tr", [
     (2, [3, 4], "exponential", [0.3], "none", -1),
     (2, [4, 4], "gaussian", [0.2, 0.1], "axiparallel", 0),
