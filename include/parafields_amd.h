@@ -237,6 +237,21 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
                 int32_t batch, int32_t transform, double transform_param, void* out,
                 int32_t out_mem);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pf_generate with device Philox noise (seed, realization
+ * first_realization + b) into device memory `out`, whose pass work arrays live
+ * in one static library buffer of PF_NOALLOC_SCRATCH_BYTES per GPU (held
+ * from the library's load until the process ends) instead of plan-owned
+ * allocations. R2C field backend only. The plan must be prepared by pf_eigs
+ * (PF_ESTATE otherwise); a realization whose work arrays exceed the buffer
+ * gives PF_EINVAL; a batch runs in groups that fit. Results equal
+ * pf_generate's bit for bit. Every call on one GPU shares that GPU's buffer:
+ * calls that overlap on one GPU (from several threads or plans) race. */
+#define PF_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pf_generate_noalloc(void* plan, uint64_t seed, uint64_t first_realization, int32_t batch,
+                        int32_t transform, double transform_param, void* out);
+
 /* Replaces Matrix::operator* / multiplyRoot / multiplyInverseExtended
  * (matrix.hh:142-176, :1443-1646) on `batch` vectors of domain_size values
  * (element type as in pf_generate; x and y may alias). */

@@ -228,6 +228,9 @@ struct Plan
   int spec_nbins = 0;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tw; // (precision bytes, n) -> table
   bool profiling = false;
+  // set during pf_generate_noalloc: a lookup that would allocate (a missing
+  // twiddle table) fails instead, and no profiling events are created
+  bool noalloc = false;
   std::vector<std::pair<std::string, double>> last_timings;
   std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> pending;
 
@@ -255,6 +258,9 @@ struct Plan
     auto it = tw.find(key);
     if (it != tw.end())
       return it->second->template as<cplx<T>>();
+    if (noalloc)
+      throw PfError(PF_ESTATE, "plan not prepared for generation without allocation: "
+                               "call pf_eigs first");
     std::vector<cplx<T>> h(n);
     for (int j = 0; j < n; j++) {
       long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
@@ -272,7 +278,7 @@ struct Plan
   // optional per-launch timing: events on the stream the launch goes to
   void mark_begin(const char* name, hipEvent_t& a, hipEvent_t& b, hipStream_t s)
   {
-    if (!profiling)
+    if (!profiling || noalloc)
       return;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
@@ -281,7 +287,7 @@ struct Plan
   }
   void mark_end(hipEvent_t b, hipStream_t s)
   {
-    if (!profiling)
+    if (!profiling || noalloc)
       return;
     HIPCHECK(hipEventRecord(b, s));
   }
@@ -892,6 +898,25 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
   P.eig_status = (P.st.negative > 0 && !d.approximate) ? PF_ENEG : PF_OK;
 }
 
+// Twiddle tables of the R2C generation passes (last axis, middle axis, x),
+// built with the eigenvalues so that a prepared plan generates without
+// allocating device memory (pf_generate_noalloc).
+void prepare_generation(Plan& P)
+{
+  if (P.d.field_backend != PF_FIELD_R2C)
+    return;
+  const Geo& g = P.g;
+  const int ns[3] = { g.dim == 3 ? g.N[2] : g.N[1], g.dim == 3 ? g.N[1] : 0, g.N[0] };
+  for (int n : ns) {
+    if (n <= 0)
+      continue;
+    if (P.d.precision == PF_FP32)
+      P.table<float>(n);
+    else
+      P.table<double>(n);
+  }
+}
+
 void ensure_eigs(Plan& P)
 {
   if (!P.eig_valid)
@@ -927,7 +952,7 @@ LastAxis last_axis(const Geo& g)
 template <typename T>
 void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
                         const double* dnoise, uint64_t seed, uint64_t first, int b0, int nb,
-                        int transform, double param, T* out)
+                        int transform, double param, T* out, cplx<T>* W1, cplx<T>* W2)
 {
   const Geo& g = P.g;
   GenGeom gg = gen_geom(P);
@@ -939,8 +964,8 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
   const int64_t nlinesG = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
   const int64_t s1 = (int64_t)g.Hp * (dim == 3 ? g.N[1] : 1) * nlast;  // B1 per realization
   const int64_t s2 = dim == 3 ? (int64_t)g.Hp * g.n[1] * g.n[2] : 0;   // B2 per realization
-  cplx<T>* B1 = P.w1.as<cplx<T>>() + (int64_t)b0 * s1;
-  cplx<T>* B2 = dim == 3 ? P.w2.as<cplx<T>>() + (int64_t)b0 * s2 : nullptr;
+  cplx<T>* B1 = W1 + (int64_t)b0 * s1;
+  cplx<T>* B2 = dim == 3 ? W2 + (int64_t)b0 * s2 : nullptr;
   const double* nz = dnoise ? dnoise + (int64_t)b0 * g.noise_per_field : nullptr;
   out += (int64_t)b0 * g.domain;
   constexpr int ME = maxe<T>();
@@ -1020,7 +1045,7 @@ void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, 
   const bool overlap = batch >= 4 && !(ov && ov[0] == '0') && !P.capturing;
   if (!overlap) {
     generate_r2c_chunk<T>(P, P.stream, P.stream, nullptr, dnoise, seed, first, 0, batch,
-                          transform, param, out);
+                          transform, param, out, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>());
     return;
   }
   if (!P.stream2) {
@@ -1038,7 +1063,8 @@ void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, 
   for (int b0 = 0; b0 < batch; b0 += per, k++) {
     const int nb = std::min(per, batch - b0);
     generate_r2c_chunk<T>(P, P.stream, P.stream2, P.ev_chunk[k % P.ev_chunk.size()], dnoise,
-                          seed, first, b0, nb, transform, param, out);
+                          seed, first, b0, nb, transform, param, out, P.w1.as<cplx<T>>(),
+                          P.w2.as<cplx<T>>());
   }
   HIPCHECK(hipEventRecord(P.ev_join, P.stream2));
   HIPCHECK(hipStreamWaitEvent(P.stream, P.ev_join, 0));
@@ -1810,6 +1836,64 @@ struct Batch
 
 size_t elem_size(const Plan& P) { return P.d.precision == PF_FP32 ? 4 : 8; }
 
+// ---------------------------------------------------------------------------
+// Generation without allocation (pf_generate_noalloc): the work arrays of the
+// passes live in one static device buffer per GPU. On the host `scratch` is
+// a stand-in for the device copies, never an address: host code takes each
+// device's address from hipGetSymbolAddress (once per device, with that
+// device current) and passes that pointer to the kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PF_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+
+void* scratch_ptr(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end())
+    return it->second;
+  void* ptr = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr)
+    throw PfError(PF_EHIP, "hipGetSymbolAddress returned no address for the scratch buffer");
+  g_scratch_ptr[device] = ptr;
+  return ptr;
+}
+
+// Bytes of B1 (+ B2 in 3D) of one realization, each rounded up to 256 bytes.
+size_t noalloc_bytes_per_realization(const Plan& P)
+{
+  const Geo& g = P.g;
+  const size_t cs = 2 * elem_size(P);
+  const int nlast = g.dim == 3 ? g.n[2] : g.n[1];
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  size_t b = up((size_t)g.Hp * (g.dim == 3 ? g.N[1] : 1) * nlast * cs);
+  if (g.dim == 3)
+    b += up((size_t)g.Hp * g.n[1] * g.n[2] * cs);
+  return b;
+}
+
+// Realizations [0, batch) in groups that fit the buffer, on the plan stream,
+// without the two-stream pipeline (which would create a stream and events).
+template <typename T>
+void generate_r2c_noalloc(Plan& P, uint64_t seed, uint64_t first, int batch, int transform,
+                          double param, T* out, void* buf, int per_group)
+{
+  const Geo& g = P.g;
+  const int nlast = g.dim == 3 ? g.n[2] : g.n[1];
+  const size_t s1 = (size_t)g.Hp * (g.dim == 3 ? g.N[1] : 1) * nlast * sizeof(cplx<T>);
+  const size_t s1g = (s1 * per_group + 255) / 256 * 256;
+  cplx<T>* W1 = reinterpret_cast<cplx<T>*>(buf);
+  cplx<T>* W2 = g.dim == 3 ? reinterpret_cast<cplx<T>*>(static_cast<char*>(buf) + s1g) : nullptr;
+  for (int b0 = 0; b0 < batch; b0 += per_group) {
+    const int nb = std::min(per_group, batch - b0);
+    generate_r2c_chunk<T>(P, P.stream, P.stream, nullptr, nullptr, seed, first + b0, 0, nb,
+                          transform, param, out + (int64_t)b0 * g.domain, W1, W2);
+  }
+}
+
 } // namespace
 
 extern "C" {
@@ -1881,6 +1965,7 @@ int pf_eigs(void* plan, pf_eig_stats* stats)
     std::lock_guard<std::mutex> lk(P->mu);
     if (!P->eig_valid)
       compute_eigs(*P, nullptr);
+    prepare_generation(*P);
     if (stats)
       *stats = P->st;
     if (P->eig_status != PF_OK)
@@ -1895,6 +1980,7 @@ int pf_eigs_custom(void* plan, const double* values, pf_eig_stats* stats)
     Plan* P = static_cast<Plan*>(plan);
     std::lock_guard<std::mutex> lk(P->mu);
     compute_eigs(*P, values);
+    prepare_generation(*P);
     if (stats)
       *stats = P->st;
     if (P->eig_status != PF_OK)
@@ -1993,6 +2079,67 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
     if (out_mem == PF_MEM_HOST)
       HIPCHECK(hipMemcpyAsync(out, dout, outb, hipMemcpyDeviceToHost, P->stream));
     P->sync();
+    return PF_OK;
+  })
+}
+
+int pf_generate_noalloc(void* plan, uint64_t seed, uint64_t first_realization, int32_t batch,
+                        int32_t transform, double transform_param, void* out)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    if (!P || !out)
+      throw PfError(PF_EINVAL, "pf_generate_noalloc: null plan or output");
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (batch < 1)
+      throw PfError(PF_EINVAL, "batch must be >= 1");
+    if (transform < 0 || transform > PF_TRANSFORM_BOXCOX)
+      throw PfError(PF_EINVAL, "transform type not known");
+    if (P->d.field_backend != PF_FIELD_R2C)
+      throw PfError(PF_EINVAL, "pf_generate_noalloc supports the R2C field backend only");
+    if (!P->eig_valid)
+      throw PfError(PF_ESTATE, "pf_generate_noalloc needs a prepared plan: call pf_eigs first "
+                               "(it computes the eigenvalues and transform tables, which "
+                               "allocates)");
+    if (P->eig_status != PF_OK)
+      throw PfError(PF_ENEG, "negative eigenvalues in covariance matrix");
+    const size_t per = noalloc_bytes_per_realization(*P);
+    if (per > (size_t)PF_NOALLOC_SCRATCH_BYTES)
+      throw PfError(PF_EINVAL, "one realization needs " + std::to_string(per) +
+                                 " bytes of work arrays, above the " +
+                                 std::to_string((size_t)PF_NOALLOC_SCRATCH_BYTES) +
+                                 "-byte buffer of pf_generate_noalloc");
+    // the output must be device memory (no staging without allocation)
+    hipPointerAttribute_t attr{};
+    const hipError_t pe = hipPointerGetAttributes(&attr, out);
+    (void)hipGetLastError(); // a host pointer leaves an error that the launch check would see
+    if (pe != hipSuccess || attr.type != hipMemoryTypeDevice)
+      throw PfError(PF_EINVAL, "pf_generate_noalloc writes device memory only");
+    const int per_group = (int)std::min<size_t>((size_t)batch, PF_NOALLOC_SCRATCH_BYTES / per);
+    int cur = 0;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != P->device)
+      HIPCHECK(hipSetDevice(P->device));
+    struct Restore
+    {
+      Plan* p;
+      int dev, cur;
+      ~Restore()
+      {
+        p->noalloc = false;
+        if (dev != cur)
+          (void)hipSetDevice(cur);
+      }
+    } restore{ P, P->device, cur };
+    void* buf = scratch_ptr(P->device);
+    P->noalloc = true;
+    if (P->d.precision == PF_FP32)
+      generate_r2c_noalloc<float>(*P, seed, first_realization, batch, transform, transform_param,
+                                  static_cast<float*>(out), buf, per_group);
+    else
+      generate_r2c_noalloc<double>(*P, seed, first_realization, batch, transform,
+                                   transform_param, static_cast<double*>(out), buf, per_group);
+    HIPCHECK(hipStreamSynchronize(P->stream));
     return PF_OK;
   })
 }
