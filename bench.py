@@ -28,15 +28,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "parafields-core_amd"))
 import pf_binding as pfb  # noqa: E402
+import workloads as wl  # noqa: E402
 
 METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
     os.path.join(ROOT, "BASELINE.json")) else "fields/s at 512^3 embedding"
 HBM_PEAK = 8.0e12
 
-CELLS = [256, 256, 256]
-KW = dict(covariance="matern32", variance=2.0, anisotropy="axiparallel",
-          corr_length=[0.1, 0.1, 0.05], approximate=True)
-SEED = 7
+CELLS = wl.ENSEMBLE_CELLS
+KW = wl.ENSEMBLE_KW
+SEED = wl.ENSEMBLE_SEED
 BATCH = 64
 
 
@@ -276,21 +276,9 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     with per-field FP64 eigenvalues, FP32 realizations, whole batch captured
     in one hipGraph and replayed."""
     import torch
-    rng = np.random.default_rng(seed)
-    sides = [64, 96, 128, 192, 256, 384, 512, 768, 1024]
-    descs, alg, shapes, falg = [], 0, [], []
-    for _ in range(nfields):
-        n0, n1 = (int(rng.choice(sides)) for _ in range(2))
-        shapes.append((n0, n1))
-        th, r, L = rng.uniform(0, np.pi), rng.uniform(1, 8), rng.uniform(0.02, 0.2)
-        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
-        A = R @ np.diag([L, L / r])
-        descs.append(pfb.make_desc(2, [n0, n1], [1.0, 1.0], covariance="gaussian",
-                                   anisotropy="geometric", corr_length=list(A.ravel()),
-                                   approximate=True, r2c_check_all=True, precision="fp32"))
-        H, N1 = n0 + 1, 2 * n1
-        falg.append(48 * H * N1 + 16 * H * n1 + 4 * n0 * n1)
-        alg += falg[-1]
+    sides = wl.UNEVEN_SIDES
+    descs, shapes, falg = wl.uneven_batch(nfields, seed)
+    alg = sum(falg)
     t0 = time.perf_counter()
     b = pfb.Batch(descs)
     out = torch.empty(int(b.offsets[-1]), dtype=torch.float32, device="cuda")
@@ -351,18 +339,25 @@ def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):
     train_s = time.perf_counter() - t0
     b = np.random.default_rng(11).standard_normal(cells ** 3)
     res = []
+    modes = ("plain", "embed_pcg", "spectral_pcg", "learned_pcg")
     for l in held:
         case = lp.CovCase(variance=1.0, corr_length=[l], cells=cells)
-        _, sp = lp.solve(case, b, "plain", max_iterations=max_it)
-        _, sl = lp.solve(case, b, "learned_pcg", model, max_iterations=max_it)
-        res.append({"corr_length": l,
-                    "plain": {k: round(v, 6) if isinstance(v, float) else v for k, v in sp.items()},
-                    "learned_pcg": {k: round(v, 6) if isinstance(v, float) else v
-                                    for k, v in sl.items()},
-                    "iteration_ratio": round(sp["iterations"] / max(1, sl["iterations"]), 2),
-                    "time_ratio": round(sp["seconds"] / max(1e-9, sl["seconds"]), 2)})
+        row = {"corr_length": l}
+        for mode in modes:
+            _, st = lp.solve(case, b, mode, model if mode == "learned_pcg" else None,
+                             max_iterations=max_it)
+            row[mode] = {k: round(v, 6) if isinstance(v, float) else v for k, v in st.items()}
+            row[mode]["us_per_iteration"] = round(st["seconds"] / max(1, st["iterations"]) * 1e6, 1)
+        sp, sl = row["plain"], row["learned_pcg"]
+        row["iteration_ratio"] = round(sp["iterations"] / max(1, sl["iterations"]), 2)
+        row["time_ratio"] = round(sp["seconds"] / max(1e-9, sl["seconds"]), 2)
+        res.append(row)
     return {"cells": cells, "train_sets": ntrain, "data_s": round(data_s, 1),
             "train_s": round(train_s, 1), "final_train_mse": round(float(losses[-1]), 5),
+            "modes": {"plain": "unpreconditioned CG", "embed_pcg": "embedding-inverse PCG "
+                      "(the reference's preconditioner)", "spectral_pcg": "radial spectral "
+                      "PCG with the exact bin targets of the case", "learned_pcg": "radial "
+                      "spectral PCG with the network's predicted multipliers"},
             "cases": res}
 
 
@@ -372,12 +367,7 @@ def run_dctdst(batch=8, steps=1):
     [0.01, 0.05] (geometric anisotropy evaluated by the DCT matrix), FP64
     eigenvalues in HBM, 8 FP32 realizations per batch, sign transform at 0.5."""
     import torch
-    rng = np.random.default_rng(5)
-    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
-    A = q @ np.diag(rng.uniform(0.01, 0.05, 3))
-    d = pfb.make_desc(3, [1024] * 3, [1.0] * 3, covariance="matern32", anisotropy="geometric",
-                      corr_length=list(A.ravel()), matrix="dct", field="dctdst",
-                      approximate=True, precision="fp32")
+    d = wl.dctdst_1024("fp32")
     plan = pfb.Plan(d)
     t0 = time.perf_counter()
     st = plan.eigs()

@@ -130,13 +130,21 @@ def train(x: np.ndarray, y: np.ndarray, steps: int = 3000, lr: float = 2e-3, see
 
 def solve(case: CovCase, b: np.ndarray, mode: str, model: Model | None = None,
           rel_tol: float = 1e-8, max_iterations: int = 20000):
-    """multiplyInverse of `b` by (P)CG on the device; returns (x, stats)."""
+    """multiplyInverse of `b` by (P)CG on the device; returns (x, stats).
+    Modes: "plain", "embed_pcg" (the embedding inverse, the reference's
+    preconditioner), "learned_pcg" (the network's multipliers) and
+    "spectral_pcg" (the same radial preconditioner with the exact targets of
+    this case, i.e. what a perfect network would predict)."""
     d = case.desc(cg_iterations=max_iterations)
     plan = pfb.Plan(d)
     try:
         lm = None
         if mode == "learned_pcg":
             lm = model.predict([case])[0].astype(np.float32)
+        elif mode == "spectral_pcg":
+            floor = case.nugget / float(plan.sizes.extended_size)
+            lm = plan.spectral_targets(NBINS, floor).astype(np.float32)
+            mode = "learned_pcg"
         x, st = plan.cg(b, mode=mode, rel_tol=rel_tol, max_iterations=max_iterations,
                         bin_log_multipliers=lm)
         return x, st

@@ -1,0 +1,185 @@
+"""The benchmark's other workloads at their full size, through the C ABI:
+the uneven batch of 1000 2D fields (one captured hipGraph, autotuned), the
+128^3 covariance inverse (plain CG against learned and embedding PCG) and
+the 1024^3 DCT/DST field (8 FP32 realizations, sign transform at 0.5).
+Bars: FP32 fields within 2e-5 of the field's standard deviation of the FP64
+oracle on identical noise; batch fields bit-identical to single plans;
+solves at FP64 true relative residual <= 1e-8."""
+import time
+
+import numpy as np
+import pytest
+
+import learned_precond as lp
+import oracle_py as orc
+import pf_binding as pfb
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 2e-5
+
+
+def test_config3_uneven_batch_1000_fields():
+    """1000 fields, sides 64..1024, one autotuned graph: replays are
+    deterministic, sampled fields (every side class 384, 768, 1024 and a
+    1024x1024) equal their single plans bit for bit, and three of them,
+    the 1024x1024 among them, match the FP64 oracle."""
+    import torch
+    descs, shapes, _ = wl.uneven_batch(1000, seed=3)
+    t0 = time.perf_counter()
+    b = pfb.Batch(descs)
+    out = torch.empty(int(b.offsets[-1]), dtype=torch.float32, device="cuda")
+    best, best_ms = b.autotune(seed=11, out=out)
+    b.generate(seed=11, out=out, use_graph=True)
+    torch.cuda.synchronize()
+    first = out.cpu().numpy()
+    b.generate(seed=11, out=out, use_graph=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), first)
+    print(f"config 3: 1000 fields, {best} branches, {best_ms:.2f} ms per replay, "
+          f"setup+tune {time.perf_counter() - t0:.1f} s", flush=True)
+    pick = []
+    for want in ((384, None), (768, None), (1024, 1024), (1024, None), (64, None)):
+        for i, (n0, n1) in enumerate(shapes):
+            if (want[1] is None and max(n0, n1) == want[0] and i not in pick) or \
+                    (want[1] is not None and (n0, n1) == want):
+                pick.append(i)
+                break
+    assert any(shapes[i] == (1024, 1024) for i in pick)
+    for i in pick:
+        p = pfb.Plan(descs[i])
+        single = p.generate(1, seed=11, first=i)[0]
+        p.close()
+        assert np.array_equal(b.field(first, i), single), (i, shapes[i])
+        assert np.all(np.isfinite(single))
+    orc.set_threads(16)
+    for i in pick[:3] + [j for j in pick if shapes[j] == (1024, 1024)][:1]:
+        d64 = descs[i]
+        d64.precision = 0
+        want = orc.generate(d64, orc.noise_philox(11, i, 0, int(orc.sizes(d64).noise_per_field)))
+        d64.precision = 1
+        err = np.abs(b.field(first, i) - want).max() / want.std()
+        print(f"config 3 field {i} {shapes[i]}: max-abs/std vs FP64 oracle {err:.3e}",
+              flush=True)
+        assert err < FP32_TOL, (i, shapes[i])
+    b.close()
+
+
+def test_config4_plain_cg_learned_and_embedding_pcg_128cube():
+    """128^3 cells (256^3 embedding), exponential covariance plus nugget 1e-3,
+    RHS N(0,1) seed 11; MLP trained on 2000 method-generated parameter sets.
+    Every solve reaches FP64 relative residual 1e-8, recomputed by the
+    oracle's matvec; learned PCG takes at least 3x fewer iterations and less
+    time than plain CG; the three solutions agree."""
+    cells = 128
+    t0 = time.perf_counter()
+    cases = lp.sample_cases(2000, seed=4, cells=cells)
+    x, y = lp.build_training_set(cases)
+    model, losses = lp.train(x, y, steps=3000, lr=2e-3)
+    print(f"config 4: training set + training {time.perf_counter() - t0:.1f} s, "
+          f"final mse {losses[-1]:.4f}", flush=True)
+    b = np.random.default_rng(11).standard_normal(cells ** 3)
+    case = lp.CovCase(variance=1.0, corr_length=[0.1], cells=cells)
+    res = {}
+    for mode in ("plain", "learned_pcg", "embed_pcg"):
+        xs, st = lp.solve(case, b, mode, model if mode == "learned_pcg" else None,
+                          max_iterations=20000)
+        res[mode] = (xs, st)
+        print(f"config 4 {mode}: {st}", flush=True)
+    orc.set_threads(16)
+    d = case.desc()
+    bn = np.linalg.norm(b)
+    for mode, (xs, st) in res.items():
+        rr = np.linalg.norm(b - orc.matvec(d, xs)) / bn
+        print(f"config 4 {mode}: oracle relative residual {rr:.3e}", flush=True)
+        assert rr <= 1.01e-8, mode
+    sp, sl = res["plain"][1], res["learned_pcg"][1]
+    assert sp["iterations"] >= 3 * sl["iterations"]
+    assert sl["seconds"] < sp["seconds"]
+    xe = res["embed_pcg"][0]
+    for mode in ("plain", "learned_pcg"):
+        diff = np.linalg.norm(res[mode][0] - xe) / np.linalg.norm(xe)
+        print(f"config 4 {mode} vs embed_pcg solution: relative difference {diff:.3e}",
+              flush=True)
+        assert diff < 1e-5, mode
+
+
+def test_config5_dctdst_1024cube_sign_transform():
+    """1024^3 cells, DCT/DST field, 8 FP32 realizations with the sign
+    transform at 0.5. The DCT matrix of a rotated (geometric) covariance is
+    not positive: a quarter of its eigenvalues are negative and clamped to 0
+    (embedding.approximate), so the field's pointwise variance is the sum
+    over the whole 2048^3 spectrum of the clamped eigenvalues (DCT-half
+    entries counted with multiplicity 2 per axis off the boundary), above
+    sigma^2 = 1. Every value is -1 or +1 and the fraction of +1 matches
+    P(Z > 0.5 / sd); the Gaussian field's spatial variance matches that sum;
+    FP32 equals the FP64 plan's field of the same Philox noise within 2e-5 of
+    its standard deviation."""
+    import torch
+    from scipy.special import ndtr
+    d = wl.dctdst_1024("fp32")
+    p = pfb.Plan(d)
+    st = p.eigs()
+    P = 1025
+    e = p.eig_values().reshape(P, P, P)
+    w = np.full(P, 2.0)
+    w[0] = w[-1] = 1.0
+    var_model = float(np.tensordot(np.tensordot(e, w, axes=([2], [0])), w, axes=([1], [0])) @ w)
+    del e
+    out = torch.empty((8, p.domain), dtype=torch.float32, device="cuda")
+    p.generate(8, seed=13, first=0, transform="sign", param=0.5, out=out)
+    torch.cuda.synchronize()
+    npos = 0
+    for r in range(8):
+        assert bool(((out[r] == 1.0) | (out[r] == -1.0)).all()), r
+        npos += int((out[r] > 0).sum())
+    frac = npos / (8 * p.domain)
+    want = float(ndtr(-0.5 / np.sqrt(var_model)))
+    print(f"config 5: eig stats {st}; model variance {var_model:.4f}; fraction of +1 "
+          f"{frac:.4f} (P(Z > 0.5 / sd) = {want:.4f})", flush=True)
+    assert abs(frac - want) < 0.02
+    g32 = torch.empty((1, p.domain), dtype=torch.float32, device="cuda")
+    p.generate(1, seed=13, first=7, out=g32)
+    torch.cuda.synchronize()
+    # realization 7 of the batch is the sign of realization 7's Gaussian field
+    assert torch.equal(torch.where(g32[0] < 0.5, -1.0, 1.0), out[7])
+    del out
+    var = float(g32.double().var())
+    print(f"config 5: spatial variance of realization 7 {var:.4f} (model {var_model:.4f})",
+          flush=True)
+    assert abs(var / var_model - 1.0) < 0.05
+    p.close()
+    torch.cuda.empty_cache()
+    p64 = pfb.Plan(wl.dctdst_1024("fp64"))
+    g64 = torch.empty((1, p64.domain), dtype=torch.float64, device="cuda")
+    p64.generate(1, seed=13, first=7, out=g64)
+    torch.cuda.synchronize()
+    err = float((g32[0].double() - g64[0]).abs().max() / g64[0].std())
+    print(f"config 5: FP32 vs FP64 plan on identical Philox noise, max-abs/std {err:.3e}",
+          flush=True)
+    assert err < FP32_TOL
+    p64.close()
+
+
+def test_config5_family_256cube_matches_oracle():
+    """The largest DCT/DST field the FP64 oracle runs in seconds: the config-5
+    covariance (rotated Matern-3/2) on 256^3 cells (512^3 even embedding).
+    FP64 plan within 1e-11 of the oracle's max |value| and the FP32 plan
+    within 2e-5 of its standard deviation, on identical Philox noise."""
+    d64 = wl.dctdst_1024("fp64", cells=256)
+    p64 = pfb.Plan(d64)
+    npf = int(p64.sizes.noise_per_field)
+    orc.set_threads(16)
+    want = orc.generate(d64, orc.noise_philox(13, 7, 0, npf))
+    got = p64.generate(1, seed=13, first=7)[0]
+    err64 = np.abs(got - want).max() / np.abs(want).max()
+    p64.close()
+    p32 = pfb.Plan(wl.dctdst_1024("fp32", cells=256))
+    g32 = p32.generate(1, seed=13, first=7)[0]
+    err32 = np.abs(g32 - want).max() / want.std()
+    p32.close()
+    print(f"DCT/DST 256^3 rotated Matern-3/2: FP64 max-abs/max {err64:.3e}, "
+          f"FP32 max-abs/std {err32:.3e}", flush=True)
+    assert err64 < 1e-11
+    assert err32 < FP32_TOL
