@@ -79,21 +79,23 @@ struct GridTraits
 
 // Configuration of one field (RandomFieldTraits, fieldtraits.hh:169-209),
 // reduced to the descriptor the engine consumes.
-template <typename RF, int dim>
-pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
+// Config: this library's ParameterTree or Dune::ParameterTree (the same
+// get<T>(key), get<T>(key, default) and hasKey).
+template <typename RF, int dim, typename Config>
+pf_desc make_desc(const Config& c, int matrix_backend, int field_backend)
 {
   pf_desc d{};
   d.dim = dim;
-  const auto cells = c.get<std::array<int, dim>>("grid.cells");
-  const auto ext = c.get<std::array<double, dim>>("grid.extensions");
+  const auto cells = c.template get<std::array<int, dim>>("grid.cells");
+  const auto ext = c.template get<std::array<double, dim>>("grid.extensions");
   for (int i = 0; i < dim; i++) {
     d.cells[i] = cells[i];
     d.extensions[i] = ext[i];
   }
-  d.embedding_factor = c.get<int>("embedding.factor", 2);
-  d.periodic = c.get<bool>("randomField.periodic", false);
-  d.variance = c.get<double>("stochastic.variance");
-  const std::string cov = c.get<std::string>("stochastic.covariance");
+  d.embedding_factor = c.template get<int>("embedding.factor", 2);
+  d.periodic = c.template get<bool>("randomField.periodic", false);
+  d.variance = c.template get<double>("stochastic.variance");
+  const std::string cov = c.template get<std::string>("stochastic.covariance");
   static const char* names[] = { "exponential", "gaussian", "spherical", "separableExponential",
                                  "matern", "matern32", "matern52", "dampedOscillation",
                                  "gammaExponential", "cauchy", "generalizedCauchy", "cubic",
@@ -106,7 +108,7 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
     d.covariance = PF_COV_CUSTOM;
   if (d.covariance < 0)
     throw std::runtime_error("covariance structure " + cov + " not known");
-  const std::string an = c.get<std::string>("stochastic.anisotropy", "none");
+  const std::string an = c.template get<std::string>("stochastic.anisotropy", "none");
   if (an == "none")
     d.anisotropy = PF_ANISO_NONE;
   else if (an == "axiparallel")
@@ -116,25 +118,25 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   else
     throw std::runtime_error(
       "stochastic.anisotropy must be \"none\", \"axiparallel\" or \"geometric\"");
-  const auto cl = c.get<std::vector<double>>("stochastic.corrLength");
+  const auto cl = c.template get<std::vector<double>>("stochastic.corrLength");
   for (size_t i = 0; i < cl.size() && i < 9; i++)
     d.corr_length[i] = cl[i];
-  d.matern_nu = c.get<double>("stochastic.maternNu", 1.);
-  d.exp_gamma = c.get<double>("stochastic.expGamma", 1.);
-  d.cauchy_alpha = c.get<double>("stochastic.cauchyAlpha", 1.);
-  d.cauchy_beta = c.get<double>("stochastic.cauchyBeta", 1.);
+  d.matern_nu = c.template get<double>("stochastic.maternNu", 1.);
+  d.exp_gamma = c.template get<double>("stochastic.expGamma", 1.);
+  d.cauchy_alpha = c.template get<double>("stochastic.cauchyAlpha", 1.);
+  d.cauchy_beta = c.template get<double>("stochastic.cauchyBeta", 1.);
   d.matrix_backend = matrix_backend;
   d.field_backend = field_backend;
-  d.transposed = c.hasKey("fftw.transposed") ? (int)c.get<bool>("fftw.transposed") : -1;
-  d.approximate = c.get<bool>("embedding.approximate", false);
-  d.threshold = c.get<double>("embedding.threshold", 1e-14);
+  d.transposed = c.hasKey("fftw.transposed") ? (int)c.template get<bool>("fftw.transposed") : -1;
+  d.approximate = c.template get<bool>("embedding.approximate", false);
+  d.threshold = c.template get<double>("embedding.threshold", 1e-14);
   d.precision = std::is_same<RF, float>::value ? PF_FP32 : PF_FP64;
-  d.cg_iterations = c.get<int>("randomField.cgIterations", 100);
+  d.cg_iterations = c.template get<int>("randomField.cgIterations", 100);
   d.nugget = 0.;
   d.r2c_check_all = 0;
   // embedding.periodization and the merge / fold / cofold taper
   // (matrix.hh:515-580, :652-1013)
-  const std::string per = c.get<std::string>("embedding.periodization", "classical");
+  const std::string per = c.template get<std::string>("embedding.periodization", "classical");
   if (per == "classical")
     d.periodization = PF_PERIOD_CLASSICAL;
   else if (per == "merge")
@@ -146,14 +148,14 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   else
     throw std::runtime_error(
       "stochastic.periodization must be \"classical\", \"merge\", \"fold\", or \"cofold\"");
-  const std::string sg = c.get<std::string>("embedding.sigmoid", "smoothstep");
+  const std::string sg = c.template get<std::string>("embedding.sigmoid", "smoothstep");
   if (sg == "smoothstep")
     d.sigmoid = PF_SIGMOID_SMOOTHSTEP;
   else if (sg == "smooth")
     d.sigmoid = PF_SIGMOID_SMOOTH;
   else
     throw std::runtime_error("embedding.sigmoid must be \"smooth\" or \"smoothstep\"");
-  const std::string sc = c.get<std::string>("stochastic.sigmoidCombine", "radialPlus");
+  const std::string sc = c.template get<std::string>("stochastic.sigmoidCombine", "radialPlus");
   if (sc == "radialPlus")
     d.sigmoid_combine = PF_COMBINE_RADIAL_PLUS;
   else if (sc == "radial")
@@ -163,17 +165,21 @@ pf_desc make_desc(const ParameterTree& c, int matrix_backend, int field_backend)
   else
     throw std::runtime_error("sigmoidCombine must be tensor, radial or radialPlus");
   if (per == "cofold") {
-    const int rec = (int)c.get<double>("embedding.cofoldRecursions", 1.);
+    const int rec = (int)c.template get<double>("embedding.cofoldRecursions", 1.);
     d.merge_recursions = rec == 0 ? -1 : rec;
   } else {
     const int rec =
-      c.get<int>(per == "fold" ? "embedding.foldRecursions" : "embedding.mergeRecursions", 99);
+      c.template get<int>(per == "fold" ? "embedding.foldRecursions" : "embedding.mergeRecursions", 99);
     d.merge_recursions = rec == 99 ? 0 : (rec == 0 ? -1 : rec);
   }
-  d.sigmoid_start = c.get<double>("embedding.sigmoidStart", 1.);
-  d.sigmoid_end = c.hasKey("embedding.sigmoidEnd") ? c.get<double>("embedding.sigmoidEnd") : 0.;
-  d.fold_max_factor = per == "cofold" ? c.get<double>("embedding.cofoldMaxFactor", 5.)
-                                      : c.get<double>("embedding.foldMaxFactor", 1.);
+  d.sigmoid_start = c.template get<double>("embedding.sigmoidStart", 1.);
+  d.sigmoid_end = c.hasKey("embedding.sigmoidEnd") ? c.template get<double>("embedding.sigmoidEnd") : 0.;
+  // pf_desc.fold_max_factor = 0 selects the default (1 for fold, 5 for
+  // cofold), so an explicit 0 cannot be expressed and is rejected
+  const char* mf = per == "cofold" ? "embedding.cofoldMaxFactor" : "embedding.foldMaxFactor";
+  d.fold_max_factor = c.template get<double>(mf, per == "cofold" ? 5. : 1.);
+  if (c.hasKey(mf) && d.fold_max_factor == 0.)
+    throw std::runtime_error(std::string(mf) + " = 0 is not supported by this engine");
   return d;
 }
 
@@ -210,6 +216,7 @@ class Matrix
   void* plan_ = nullptr;
   mutable void* plan64_ = nullptr;
   pf_sizes sizes_{};
+  mutable std::vector<double> custom_; // host covariance values of a callable
 
 public:
   using StochasticPartType = StochasticPart<RF>;
@@ -279,6 +286,7 @@ public:
         }
     pf_eig_stats st{};
     check(pf_eigs_custom(plan_, vals.data(), &st));
+    custom_ = std::move(vals);
     return st;
   }
 
@@ -343,6 +351,10 @@ public:
         pf_desc d = desc_;
         d.precision = PF_FP64;
         check(pf_plan_create(&d, &plan64_));
+        if (!custom_.empty()) {
+          pf_eig_stats st{};
+          check(pf_eigs_custom(plan64_, custom_.data(), &st));
+        }
       }
       p = plan64_;
     }

@@ -1253,10 +1253,110 @@ Engine* get_engine(const pf_desc* d, const double* custom = nullptr)
   return en;
 }
 
+// r2r along one axis of the P-grid array for one parity: REDFT00 on all P
+// entries of an even axis, RODFT00 on the P-2 interior entries of an odd axis
+// with the boundary entries kept at zero (dctdstfieldbackend.hh:236-404, the
+// zero strip/reinsert of :960-1171). Both are unnormalised and their own
+// inverses up to 2(P-1) and 2(P-1) respectively.
+void r2r_axis(double* a, const Index P[3], int ax, bool odd)
+{
+  if (!odd) {
+    redft00_axis(a, P, ax);
+    return;
+  }
+  Index stride = 1;
+  for (int i = 0; i < ax; i++)
+    stride *= P[i];
+  const int n = (int)P[ax];
+  const Index lines = P[0] * P[1] * P[2] / n;
+  parallel_for(lines, [&](Index b, Index en) {
+    std::vector<double> in(n), out(n);
+    for (Index l = b; l < en; l++) {
+      Index inner = l % stride, outer = l / stride;
+      Index base = inner + outer * stride * n;
+      for (int j = 0; j < n - 2; j++)
+        in[j] = a[base + (j + 1) * stride];
+      rodft00_line(in.data(), out.data(), n - 2);
+      a[base] = 0.;
+      a[base + (n - 1) * stride] = 0.;
+      for (int j = 0; j < n - 2; j++)
+        a[base + (j + 1) * stride] = out[j];
+    }
+  });
+}
+
+// Matrix::multiplyExtended / multiplyRootExtended / multiplyInverseExtended,
+// DCT/DST version (matrix.hh:1443-1470, :1514-1541, :1587-1614):
+// fieldToExtendedField (dctdstfieldbackend.hh:489-500: zeroed P-grid array,
+// the first n_i entries of every axis from the field), then for every parity
+// type t: setComponent (:440-458: value / 2^{d - border}, zero on the boundary
+// of an odd axis), forwardTransform (:236-316: r2r per axis, / N_ext),
+// mult by f(lambda) (:461-466), backwardTransform (:326-404) and the
+// extraction (:631-648, :905-955), which adds into the output for every type
+// (the non-additive walk adds too, :923). The forward and backward transforms
+// both leave the array in the storage order (transposed or not) in which the
+// matrix backend stores its eigenvalues, so index k of one meets index k of
+// the other; here both stay in the logical order. `y` is accumulated into:
+// Matrix::operator* passes the input itself as the output (matrix.hh:142-151),
+// so the reference returns input + C x / N_ext there.
+void matvec_dctdst(const pf_desc& d, const Geo& g, const Eigs& e, int op, const double* x,
+                   double* y)
+{
+  Index P[3];
+  for (int i = 0; i < 3; i++)
+    P[i] = (i < g.dim) ? g.N[i] / 2 + 1 : 1;
+  const Index count = P[0] * P[1] * P[2];
+  std::vector<double> ext(count, 0.), a(count);
+  for (Index z = 0; z < g.n[2]; z++)
+    for (Index yy = 0; yy < g.n[1]; yy++)
+      for (Index xx = 0; xx < g.n[0]; xx++)
+        ext[xx + P[0] * (yy + P[1] * z)] = x[xx + g.n[0] * (yy + g.n[1] * z)];
+  for (unsigned type = 0; type < (1u << g.dim); type++) {
+    bool odd[3] = { false, false, false };
+    for (int i = 0; i < g.dim; i++)
+      odd[i] = (type >> i) & 1;
+    for (Index c = 0; c < count; c++) {
+      const Index k[3] = { c % P[0], (c / P[0]) % P[1], c / (P[0] * P[1]) };
+      unsigned border = 0;
+      bool zero = false;
+      for (int i = 0; i < g.dim; i++)
+        if (k[i] == 0 || k[i] == P[i] - 1) {
+          if (odd[i])
+            zero = true;
+          border++;
+        }
+      a[c] = zero ? 0. : ext[c] / (double)(1u << (g.dim - border));
+    }
+    for (int ax = 0; ax < g.dim; ax++)
+      r2r_axis(a.data(), P, ax, odd[ax]);
+    for (Index c = 0; c < count; c++) {
+      const double m = e.v[c];
+      double f;
+      if (op == 3) {
+        const Index k0 = c % P[0], k1 = (c / P[0]) % P[1], k2 = c / (P[0] * P[1]);
+        f = g_spec_table[radial_bin(g, k0, k1, k2, (int)g_spec_table.size())];
+      } else
+        f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+      a[c] = a[c] / (double)g.Next * f;
+    }
+    for (int ax = 0; ax < g.dim; ax++)
+      r2r_axis(a.data(), P, ax, odd[ax]);
+    for (Index z = 0; z < g.n[2]; z++)
+      for (Index yy = 0; yy < g.n[1]; yy++)
+        for (Index xx = 0; xx < g.n[0]; xx++)
+          y[xx + g.n[0] * (yy + g.n[1] * z)] += a[xx + P[0] * (yy + P[1] * z)];
+  }
+}
+
+// The product M x (no accumulation) for either field backend.
 void matvec(const Engine& en, int op, const double* x, double* y)
 {
-  if (en.d.field_backend != PF_FIELD_R2C)
-    throw std::runtime_error("matvec implemented for the R2C field backend");
+  if (en.d.field_backend == PF_FIELD_DCTDST) {
+    std::vector<double> acc(en.g.domain, 0.);
+    matvec_dctdst(en.d, en.g, en.e, op, x, acc.data());
+    std::copy(acc.begin(), acc.end(), y);
+    return;
+  }
   matvec_r2c(en.d, en.g, en.e, op, x, y);
 }
 

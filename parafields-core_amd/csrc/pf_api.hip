@@ -1320,6 +1320,15 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
   }
 }
 
+// The DCT/DST field backend takes the same products: its route in the
+// reference (matrix.hh:1456-1470, per parity type setComponent, r2r forward /
+// N_ext, times f(lambda), r2r backward, summed into the output) adds up to
+// R f(C_ext) E x / N_ext with the DCT eigenvalues, which matvec_r2c applies
+// through the half-spectrum passes (the fused pass folds kx, ky, kz onto the
+// DCT-half table). The reference's accumulation into its output
+// (dctdstfieldbackend.hh:923) is the caller's: the C++ Matrix adds the input
+// back for operator* and multiplyRoot.
+
 // ---------------------------------------------------------------------------
 // Conjugate gradients (FP64 vectors and dot products on the device)
 // ---------------------------------------------------------------------------
@@ -1369,8 +1378,6 @@ void matvec64(Plan& P, int op, const double* x, double* y)
 {
   if (op == PF_OP_SPECTRAL && P.spec_nbins <= 0)
     throw PfError(PF_EINVAL, "no learned multipliers loaded");
-  if (P.d.field_backend != PF_FIELD_R2C)
-    throw PfError(PF_EINVAL, "matvec is implemented for the R2C field backend");
   matvec_r2c<double>(P, op, x, y, 1);
 }
 
@@ -2151,8 +2158,6 @@ int pf_matvec(void* plan, int32_t op, const void* x, void* y, int32_t batch, int
     std::lock_guard<std::mutex> lk(P->mu);
     if (op < 0 || op > PF_OP_INVERSE_EMBED)
       throw PfError(PF_EINVAL, "unknown matvec operator");
-    if (P->d.field_backend != PF_FIELD_R2C)
-      throw PfError(PF_EINVAL, "matvec is implemented for the R2C field backend");
     ensure_eigs(*P);
     const Geo& g = P->g;
     const size_t es = elem_size(*P);
@@ -2183,8 +2188,6 @@ int pf_cg(void* plan, const double* b, double* x, int32_t mem, const pf_cg_opts*
   PF_GUARD({
     Plan* P = static_cast<Plan*>(plan);
     std::lock_guard<std::mutex> lk(P->mu);
-    if (P->d.field_backend != PF_FIELD_R2C)
-      throw PfError(PF_EINVAL, "multiplyInverse is implemented for the R2C field backend");
     ensure_eigs(*P);
     const int64_t n = P->g.domain;
     const int mode = opts ? opts->mode : PF_CG_REFERENCE;

@@ -81,6 +81,60 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
+// philox4x32_10 of the counter (c0, 0, u2, u3) where only c0 varies across the
+// wave (a block index below 2^32; u2, u3 = the realization, wave-uniform).
+// The same ten rounds, with the wave-uniform halves of the first three
+// rounds written out so the compiler keeps them in scalar registers: round 1
+// multiplies the uniform word u2, round 2 the uniform output of round 1, so
+// two of the twenty 32x32->64 products and one XOR leave the vector unit.
+// Outputs equal philox4x32_10 on { c0, 0, u2, u3 } bit for bit.
+__device__ __forceinline__ void philox4x32_10_lane0(uint32_t c0, uint32_t u2, uint32_t u3,
+                                                    uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  // round 1 (keys k0, k1): uniform product M1 * u2
+  const uint64_t q1 = (uint64_t)M1 * u2;
+  const uint32_t a0 = (uint32_t)(q1 >> 32) ^ k0; // ^ c1 = 0
+  const uint32_t a1 = (uint32_t)q1;
+  const uint64_t p0 = (uint64_t)M0 * c0;
+  const uint32_t a2 = (uint32_t)(p0 >> 32) ^ (u3 ^ k1);
+  const uint32_t a3 = (uint32_t)p0;
+  // round 2 (keys + W): uniform product M0 * a0
+  const uint64_t q0 = (uint64_t)M0 * a0;
+  const uint64_t p1 = (uint64_t)M1 * a2;
+  const uint32_t b0 = (uint32_t)(p1 >> 32) ^ (a1 ^ (k0 + W0));
+  const uint32_t b1 = (uint32_t)p1;
+  const uint32_t b2 = a3 ^ ((uint32_t)(q0 >> 32) ^ (k1 + W1));
+  const uint32_t b3 = (uint32_t)q0;
+  // round 3 (keys + 2W): b3 is uniform
+  const uint64_t r0 = (uint64_t)M0 * b0;
+  const uint64_t r1 = (uint64_t)M1 * b2;
+  uint32_t c[4];
+  c[0] = xor3((uint32_t)(r1 >> 32), b1, k0 + 2 * W0);
+  c[1] = (uint32_t)r1;
+  c[2] = (uint32_t)(r0 >> 32) ^ (b3 ^ (k1 + 2 * W1));
+  c[3] = (uint32_t)r0;
+  // rounds 4..10
+  uint32_t kk0 = k0 + 2 * W0, kk1 = k1 + 2 * W1;
+#pragma unroll
+  for (int r = 3; r < 10; r++) {
+    kk0 += W0;
+    kk1 += W1;
+    const uint64_t s0 = (uint64_t)M0 * c[0];
+    const uint64_t s1 = (uint64_t)M1 * c[2];
+    const uint32_t n0 = xor3((uint32_t)(s1 >> 32), c[1], kk0);
+    const uint32_t n2 = xor3((uint32_t)(s0 >> 32), c[3], kk1);
+    c[0] = n0;
+    c[1] = (uint32_t)s1;
+    c[2] = n2;
+    c[3] = (uint32_t)s0;
+  }
+  out[0] = c[0];
+  out[1] = c[1];
+  out[2] = c[2];
+  out[3] = c[3];
+}
+
 // Pair of normals (xi_1, xi_2) = samples (2e, 2e+1) of the stream: one
 // Box-Muller pair of block e/2.
 template <typename T>

@@ -303,6 +303,24 @@ __device__ __forceinline__ cplx<float> gen_entry<float>(uint32_t xa, uint32_t xb
   return { t * __builtin_amdgcn_cosf(ub), selfconj ? 0.0f : t * __builtin_amdgcn_sinf(ub) };
 }
 
+// Philox block `blk` of realization `real`. With 32-bit slot walks every
+// block index of a realization is below 2^31 (its high counter word is 0), so
+// the rounds with wave-uniform halves run partly on the scalar unit.
+template <bool IDX64>
+__device__ __forceinline__ void philox_block(uint64_t blk, uint64_t real, uint32_t k0,
+                                             uint32_t k1, uint32_t c[4])
+{
+  if constexpr (IDX64) {
+    c[0] = (uint32_t)blk;
+    c[1] = (uint32_t)(blk >> 32);
+    c[2] = (uint32_t)real;
+    c[3] = (uint32_t)(real >> 32);
+    philox4x32_10(c, k0, k1);
+  } else {
+    philox4x32_10_lane0((uint32_t)blk, (uint32_t)real, (uint32_t)(real >> 32), k0, k1, c);
+  }
+}
+
 // logical (k0, k1, k2) of line (kx, km) element e
 __device__ __forceinline__ void line_k(const GenGeom& g, int kx, int km, int e, int& k1, int& k2)
 {
@@ -411,9 +429,8 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
           const IX row = (g.dct && e >= Pe) ? dn : up;
           const T sa = sq_at(row + (IX)lac), sb = sq_at(row + (IX)lb);
           const uint64_t blk = (uint64_t)base >> 1;
-          uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
-                            (uint32_t)(real >> 32) };
-          philox4x32_10(c, sk0, sk1);
+          uint32_t c[4];
+          philox_block<IDX64>(blk, real, sk0, sk1, c);
           const bool sce = SC && self_conj(e, Ne);
           const cplx<T> va = gen_entry<T>(c[0], c[1], sa, sce && la == 0);
           const cplx<T> vb = gen_entry<T>(c[2], c[3], sb, sce && lb == 0);
@@ -447,9 +464,8 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
           const IX row = (g.dct && e >= Pe) ? est * (IX)(Ne - e) : est * (IX)e;
           const T sa = sq_at(row + (IX)(L - 1));
           const uint64_t blk = (uint64_t)(IX)(base0 + rowstep * (IX)e + (IX)(L - 1)) >> 1;
-          uint32_t c[4] = { (uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)real,
-                            (uint32_t)(real >> 32) };
-          philox4x32_10(c, sk0, sk1);
+          uint32_t c[4];
+          philox_block<IDX64>(blk, real, sk0, sk1, c);
           buf[e * L + L - 1] = gen_entry<T>(c[0], c[1], sa, false);
         }
       }

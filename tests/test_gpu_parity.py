@@ -937,3 +937,94 @@ def test_philox_fields_mean_and_covariance(precision, matrix):
     assert np.all(np.abs(S - C) < 6 * se)
     # the deficit is real: the model covariance differs from A·Aᵀ on the diagonal
     assert abs(np.diag(C).mean() - 1.5) > 6 * np.sqrt(2 * 1.5 ** 2 / (B * len(sd)))
+
+
+def test_gpumatrix_adapter_end_to_end(tmp_path):
+    """The reference-side adapter (include/parafields_amd/gpumatrix.hh) run
+    through reference-shaped traits (tests/host/gpumatrix_check.cc): every
+    Matrix method against the oracle on the same libstdc++ noise, including
+    refinement, geometric anisotropy (R2C matrix) and the DCT/DST field with
+    the reference's accumulation (a second generate adds onto the field;
+    operator* and multiplyRoot return input + product)."""
+    import os
+    import subprocess
+    import test_host_api as tha
+    exe = tha._build_gpumatrix_check(tmp_path)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([exe, "run", str(out)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "all OK" in r.stdout, r.stdout + r.stderr
+
+    def rd(name):
+        return np.fromfile(str(out / f"{name}.bin"), dtype=np.float64)
+
+    base = dict(variance=1.5, approximate=True)
+    d = desc(2, [40, 24], extensions=[1.0, 0.6], covariance="exponential", corr_length=0.1,
+             **base)
+    n = int(orc.sizes(d).noise_per_field)
+    g = orc.generate(d, orc.noise_cpp(42, n))
+    assert rel_max(rd("iso_gen"), g) < FP64_TOL
+    assert rel_max(rd("iso_mult"), orc.matvec(d, g)) < FP64_TOL
+    assert rel_max(rd("iso_root"), orc.matvec(d, g, "root")) < FP64_TOL
+    want_inv, _ = orc.multiply_inverse(d, g)
+    assert np.linalg.norm(rd("iso_inv") - want_inv) / np.linalg.norm(want_inv) < 1e-7
+    assert np.array_equal(rd("iso_uncorr"), orc.noise_cpp(7, 40 * 24))
+    assert np.all(rd("iso_var") == 1.5)
+    d2 = desc(2, [80, 48], extensions=[1.0, 0.6], covariance="exponential", corr_length=0.1,
+              **base)
+    assert rel_max(rd("iso_refined_gen"),
+                   orc.generate(d2, orc.noise_cpp(43, int(orc.sizes(d2).noise_per_field)))) < FP64_TOL
+    da = desc(3, [12, 10, 8], covariance="exponential", anisotropy="geometric",
+              corr_length=[0.1, 0.02, 0, 0.01, 0.08, 0, 0, 0, 0.05], matrix="r2c", **base)
+    ga = orc.generate(da, orc.noise_cpp(5, int(orc.sizes(da).noise_per_field)))
+    assert rel_max(rd("aniso_gen"), ga) < FP64_TOL
+    assert rel_max(rd("aniso_mult"), orc.matvec(da, ga)) < FP64_TOL
+    dd = desc(3, [16, 12, 10], covariance="exponential", anisotropy="axiparallel",
+              corr_length=[0.2, 0.1, 0.15], matrix="dct", field="dctdst", **base)
+    nd = int(orc.sizes(dd).noise_per_field)
+    z = orc.noise_cpp(9, 2 * nd)
+    g1 = orc.generate(dd, z[:nd])
+    g2 = g1 + orc.generate(dd, z[nd:])
+    assert rel_max(rd("dct_gen1"), g1) < FP64_TOL
+    assert rel_max(rd("dct_gen2"), g2) < FP64_TOL
+    assert rel_max(rd("dct_mult"), g2 + orc.matvec(dd, g2)) < FP64_TOL
+    assert rel_max(rd("dct_root"), g2 + orc.matvec(dd, g2, "root")) < FP64_TOL
+    want_dinv, _ = orc.multiply_inverse(dd, g2)
+    assert np.linalg.norm(rd("dct_inv") - want_dinv) / np.linalg.norm(want_dinv) < 1e-7
+    assert np.all(np.isfinite(rd("custom_f32_gen"))) and np.all(np.isfinite(rd("custom_f32_inv")))
+
+
+@pytest.mark.parametrize("dim,cells,kw", [
+    (2, [24, 16], dict(covariance="exponential", corr_length=0.1)),
+    (3, [16, 12, 10], dict(covariance="exponential", anisotropy="axiparallel",
+                           corr_length=[0.2, 0.1, 0.15])),
+    (3, [8, 12, 16], dict(covariance="matern52", corr_length=0.1, transposed=0)),
+])
+@pytest.mark.parametrize("op", ["multiply", "root", "inverse_embed"])
+def test_dctdst_matvec_matches_oracle(dim, cells, kw, op):
+    """pf_matvec on a DCT/DST-field plan against the oracle's restatement of
+    the reference's per-type DCT/DST route (FP64 <= 1e-11; FP32 <= 2e-5 of
+    the product's max)."""
+    d = desc(dim, cells, field="dctdst", matrix="dct", approximate=True, **kw)
+    x = np.random.default_rng(3).standard_normal(int(np.prod(cells)))
+    want = orc.matvec(d, x, op)
+    assert rel_max(pfb.Plan(d).matvec(x, op=op), want) < (FP64_TOL if op != "inverse_embed"
+                                                          else 1e-9)
+    d32 = desc(dim, cells, field="dctdst", matrix="dct", approximate=True, precision="fp32", **kw)
+    if op != "inverse_embed":
+        got = pfb.Plan(d32).matvec(x.astype(np.float32), op=op)
+        assert np.abs(got - want).max() / np.abs(want).max() < FP32_TOL
+
+
+def test_dctdst_multiply_inverse_matches_oracle():
+    d = desc(3, [12, 10, 8], covariance="exponential", corr_length=0.2, field="dctdst",
+             matrix="dct", cg_iterations=5000)
+    b = np.random.default_rng(8).standard_normal(12 * 10 * 8)
+    want, _ = orc.multiply_inverse(d, b)
+    got, st = pfb.Plan(d).cg(b, mode="reference")
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-7
+    for mode in ("plain", "embed_pcg"):
+        x, st = pfb.Plan(d).cg(b, mode=mode, rel_tol=1e-10, max_iterations=5000)
+        assert st["rel_residual"] <= 1e-10, mode
+        assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-6, mode

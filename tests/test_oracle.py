@@ -394,6 +394,32 @@ def test_matvec_is_dense_covariance_over_next(dim, cells, kind, corr, aniso, mat
     np.testing.assert_allclose(orc.matvec(d, x), C @ x / s.extended_size, atol=1e-14)
 
 
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,tr", [
+    (2, [5, 4], "exponential", [0.2], "none", -1),
+    (2, [4, 6], "gaussian", [0.2, 0.1], "axiparallel", 0),
+    (3, [3, 4, 2], "matern32", [0.2, 0.1, 0.15], "axiparallel", -1),
+    (3, [4, 2, 3], "exponential", [0.3], "none", 0),
+])
+def test_dctdst_matvec_route(dim, cells, kind, corr, aniso, tr):
+    """The DCT/DST route of multiplyExtended (per parity type: setComponent,
+    r2r forward / N_ext, times f(lambda), r2r backward, added into the output)
+    sums to R f(C_ext) E x / N_ext: the multiply equals the dense covariance
+    C x / N_ext, and every operator equals the R2C-field route on the same
+    DCT eigenvalues (independent transforms)."""
+    kw = dict(covariance=kind, corr_length=corr, anisotropy=aniso, matrix="dct",
+              approximate=True, transposed=tr)
+    d = desc(dim, cells, field="dctdst", **kw)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0] * dim, kind, aniso, corr)
+    x = np.random.default_rng(7).standard_normal(C.shape[0])
+    np.testing.assert_allclose(orc.matvec(d, x), C @ x / s.extended_size, atol=1e-14)
+    d2 = desc(dim, cells, field="r2c", **dict(kw, transposed=-1 if dim == 3 else 0))
+    for op in ("multiply", "root", "inverse_embed"):
+        want = orc.matvec(d2, x, op)
+        np.testing.assert_allclose(orc.matvec(d, x, op), want,
+                                   atol=1e-12 * np.abs(want).max(), err_msg=op)
+
+
 def test_root_and_inverse_are_exact_when_periodic():
     d = desc(2, [8, 6], covariance="exponential", corr_length=0.1, periodic=True)
     x = np.random.default_rng(4).standard_normal(48)
