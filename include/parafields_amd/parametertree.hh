@@ -111,6 +111,8 @@ public:
       line = line.substr(b, line.find_last_not_of(" \t\r") - b + 1);
       if (line.front() == '[' && line.back() == ']') {
         section = line.substr(1, line.size() - 2);
+        const auto sb = section.find_first_not_of(" \t"), se = section.find_last_not_of(" \t");
+        section = sb == std::string::npos ? std::string() : section.substr(sb, se - sb + 1);
         continue;
       }
       const auto eq = line.find('=');
@@ -121,9 +123,31 @@ public:
         return bb == std::string::npos ? std::string() : s.substr(bb, ee - bb + 1);
       };
       std::string key = trim(line.substr(0, eq)), val = trim(line.substr(eq + 1));
+      if (val.size() >= 2 && val.front() == '"' && val.back() == '"')
+        val = val.substr(1, val.size() - 2); // quoted, as report() writes
       if (!section.empty())
         key = section + "." + key;
       values_[key] = val;
+    }
+  }
+
+  // Dune::ParameterTree::report layout: the keys without a section as
+  // key = "value", then each section as [ name ] followed by its keys.
+  // readINI (and Dune's INI parser) read it back.
+  void report(std::ostream& out) const
+  {
+    std::map<std::string, std::map<std::string, std::string>> sections;
+    for (const auto& kv : values_) {
+      const auto dot = kv.first.rfind('.');
+      if (dot == std::string::npos)
+        out << kv.first << " = \"" << kv.second << "\"" << std::endl;
+      else
+        sections[kv.first.substr(0, dot)][kv.first.substr(dot + 1)] = kv.second;
+    }
+    for (const auto& sec : sections) {
+      out << "[ " << sec.first << " ]" << std::endl;
+      for (const auto& kv : sec.second)
+        out << kv.first << " = \"" << kv.second << "\"" << std::endl;
     }
   }
 

@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstring>
 #include <ctime>
+#include <fstream>
+#include <sstream>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -23,6 +25,7 @@
 #include <vector>
 
 #include "../parafields_amd.h"
+#include "io.hh"
 #include "parametertree.hh"
 #include "trend.hh"
 
@@ -427,9 +430,58 @@ public:
   }
 };
 
+// Refinement of cell data (StochasticPart::refine / coarsen,
+// stochastic.hh:597-760): refine copies every cell into its 2^d children,
+// coarsen averages the 2^d children of every coarse cell. `cells` is the
+// grid after the operation. An empty vector stays empty.
+template <typename RF, int dim>
+std::vector<RF> refine_cells(const std::vector<RF>& in, const std::array<int, dim>& cells)
+{
+  if (in.empty())
+    return in;
+  std::array<size_t, 3> n{ 1, 1, 1 };
+  for (int i = 0; i < dim; i++)
+    n[i] = (size_t)cells[i];
+  std::vector<RF> out(n[0] * n[1] * n[2]);
+  for (size_t z = 0; z < n[2]; z++)
+    for (size_t y = 0; y < n[1]; y++)
+      for (size_t x = 0; x < n[0]; x++) {
+        const size_t ox = x / 2, oy = dim > 1 ? y / 2 : 0, oz = dim > 2 ? z / 2 : 0;
+        out[x + n[0] * (y + n[1] * z)] = in.at(ox + (n[0] / 2) * (oy + (n[1] / (dim > 1 ? 2 : 1)) * oz));
+      }
+  return out;
+}
+
+template <typename RF, int dim>
+std::vector<RF> coarsen_cells(const std::vector<RF>& in, const std::array<int, dim>& cells)
+{
+  if (in.empty())
+    return in;
+  std::array<size_t, 3> n{ 1, 1, 1 }, f{ 1, 1, 1 };
+  for (int i = 0; i < dim; i++) {
+    n[i] = (size_t)cells[i];
+    f[i] = 2 * n[i];
+  }
+  std::vector<RF> out(n[0] * n[1] * n[2]);
+  for (size_t z = 0; z < n[2]; z++)
+    for (size_t y = 0; y < n[1]; y++)
+      for (size_t x = 0; x < n[0]; x++) {
+        RF sum = 0.;
+        for (size_t c = 0; c < ((size_t)1 << dim); c++) {
+          const size_t fx = 2 * x + (c & 1), fy = dim > 1 ? 2 * y + ((c >> 1) & 1) : 0,
+                       fz = dim > 2 ? 2 * z + ((c >> 2) & 1) : 0;
+          sum += in.at(fx + f[0] * (fy + f[1] * fz));
+        }
+        out[x + n[0] * (y + n[1] * z)] = sum / (RF)((size_t)1 << dim);
+      }
+  return out;
+}
+
 // RandomField (randomfield.hh:42-1389), the hot-path surface: generation,
-// products with the covariance matrix, its root and its inverse, and
-// evaluation with the value transform. Iso/aniso dispatch as in
+// products with the covariance matrix, its root and its inverse (with the
+// optional cached inverse products of randomField.cacheInvMatvec and
+// randomField.cacheInvRootMatvec), evaluation with the value transform,
+// refinement, arithmetic and the on-disk format. Iso/aniso dispatch as in
 // randomfield.hh:153-164.
 template <typename GT, typename IsoMB = DCTMatrixBackend, typename IsoFB = R2CFieldBackend,
           typename AnisoMB = R2CMatrixBackend, typename AnisoFB = R2CFieldBackend>
@@ -453,21 +505,123 @@ private:
   std::array<int, dim> cells_;
   std::array<double, dim> ext_;
   TrendPart<RF, dim> trend_;
+  // cached products (randomfield.hh:100-107, :220-226): invMatvec_ holds
+  // C^{-1} sp_ and invRootMatvec_ C^{-1/2} sp_ when their flags are set
+  bool cacheInvMatvec_ = false, cacheInvRootMatvec_ = false;
+  StochasticPart<RF> invMatvec_, invRootMatvec_;
+  bool invMatvecValid_ = false, invRootMatvecValid_ = false;
+
+  static ParameterTree read_field_file(const std::string& fileName)
+  {
+    ParameterTree c;
+    c.readINIFile(fileName + ".field");
+    return c;
+  }
+
+  void make_matrix()
+  {
+    const std::string an = config_.get<std::string>("stochastic.anisotropy", "none");
+    const std::string cov = config_.get<std::string>("stochastic.covariance");
+    auto with_cells = [&](pf_desc d) {
+      for (int i = 0; i < dim; i++)
+        d.cells[i] = cells_[i];
+      return d;
+    };
+    iso_.reset();
+    aniso_.reset();
+    if (cov != "custom-aniso" && (an == "none" || an == "axiparallel"))
+      iso_ = std::make_shared<IsoMatrix>(with_cells(make_desc<RF, dim>(config_, IsoMB::id, IsoFB::id)));
+    else
+      aniso_ = std::make_shared<AnisoMatrix>(
+        with_cells(make_desc<RF, dim>(config_, AnisoMB::id, AnisoFB::id)));
+  }
+  StochasticPart<RF> times(const StochasticPart<RF>& x) const { return aniso_ ? (*aniso_) * x : (*iso_) * x; }
+  StochasticPart<RF> root(const StochasticPart<RF>& x) const
+  {
+    return aniso_ ? aniso_->multiplyRoot(x) : iso_->multiplyRoot(x);
+  }
+  StochasticPart<RF> inverse(const StochasticPart<RF>& x) const
+  {
+    return aniso_ ? aniso_->multiplyInverse(x) : iso_->multiplyInverse(x);
+  }
+  static void scale(StochasticPart<RF>& x, RF alpha)
+  {
+    for (auto& v : x.dataVector)
+      v *= alpha;
+  }
+  static void axpy_part(StochasticPart<RF>& x, const StochasticPart<RF>& o, RF alpha)
+  {
+    if (o.dataVector.empty())
+      return;
+    if (x.dataVector.empty())
+      x.dataVector.assign(o.dataVector.size(), RF(0));
+    for (size_t i = 0; i < x.dataVector.size(); i++)
+      x.dataVector[i] += o.dataVector.at(i) * alpha;
+    x.evalValid = false;
+  }
+  void read_config()
+  {
+    cells_ = config_.get<std::array<int, dim>>("grid.cells");
+    ext_ = config_.get<std::array<double, dim>>("grid.extensions");
+    cacheInvMatvec_ = config_.get<bool>("randomField.cacheInvMatvec", false);
+    cacheInvRootMatvec_ = config_.get<bool>("randomField.cacheInvRootMatvec", false);
+  }
 
 public:
   explicit RandomField(const ParameterTree& config)
     : config_(config)
     , transform_(config)
   {
-    cells_ = config.get<std::array<int, dim>>("grid.cells");
-    ext_ = config.get<std::array<double, dim>>("grid.extensions");
-    trend_ = TrendPart<RF, dim>(config, ext_);
-    const std::string an = config.get<std::string>("stochastic.anisotropy", "none");
-    const std::string cov = config.get<std::string>("stochastic.covariance");
-    if (cov != "custom-aniso" && (an == "none" || an == "axiparallel"))
-      iso_ = std::make_shared<IsoMatrix>(make_desc<RF, dim>(config, IsoMB::id, IsoFB::id));
-    else
-      aniso_ = std::make_shared<AnisoMatrix>(make_desc<RF, dim>(config, AnisoMB::id, AnisoFB::id));
+    read_config();
+    trend_ = TrendPart<RF, dim>(config_, ext_);
+    make_matrix();
+  }
+
+  // randomfield.hh:192-227: the configuration from <fileName>.field, the
+  // trend coefficients from <fileName>.trend and the field values from
+  // <fileName>.stoch.h5 (dataset /stochastic); the caches start from the
+  // field read in, not valid
+  explicit RandomField(const std::string& fileName)
+    : config_(read_field_file(fileName))
+    , transform_(config_)
+  {
+    read_config();
+    trend_ = TrendPart<RF, dim>(config_, ext_, fileName);
+    std::array<unsigned int, dim> c;
+    for (int i = 0; i < dim; i++)
+      c[i] = (unsigned int)cells_[i];
+    sp_.dataVector = readStochasticHDF5<RF, dim>(c, "/stochastic", fileName + ".stoch.h5");
+    make_matrix();
+    invMatvec_ = sp_;
+    invRootMatvec_ = sp_;
+  }
+
+  // randomfield.hh:706-713 with StochasticPart::writeToFile
+  // (stochastic.hh:273-293) and TrendPart::writeToFile (trend.hh:1515-1527)
+  void writeToFile(const std::string& fileName) const
+  {
+    std::array<unsigned int, dim> c;
+    std::array<RF, dim> e;
+    for (int i = 0; i < dim; i++) {
+      c[i] = (unsigned int)cells_[i];
+      e[i] = (RF)ext_[i];
+    }
+    std::vector<RF> data = sp_.dataVector;
+    if (data.empty())
+      data.assign((size_t)domain_size(), RF(0));
+    writeStochasticHDF5<RF, dim>(c, data, "/stochastic", fileName + ".stoch.h5");
+    writeToXDMF<RF, dim>(c, e, fileName);
+    {
+      std::ofstream t(fileName + ".trend", std::ofstream::trunc);
+      trend_.writeToFile(t);
+    }
+    std::ofstream f(fileName + ".field", std::ofstream::trunc);
+    ParameterTree out = config_;
+    std::ostringstream cs;
+    for (int i = 0; i < dim; i++)
+      cs << (i ? " " : "") << cells_[i];
+    out["grid.cells"] = cs.str(); // the current resolution
+    out.report(f);
   }
 
   // randomfield.hh:491-499: time-based seed
@@ -487,7 +641,7 @@ public:
   }
 
   // randomfield.hh:541-558: the field, then the trend coefficients from the
-  // same RNG
+  // same RNG; the cached products become invalid
   template <typename RNG>
   void generateWithRNG(RNG& rng, bool = false)
   {
@@ -496,6 +650,8 @@ public:
     else
       iso_->generateField(rng, sp_);
     trend_.generate(rng);
+    invMatvecValid_ = false;
+    invRootMatvecValid_ = false;
   }
 
   // randomfield.hh:589-603
@@ -506,6 +662,8 @@ public:
     else
       iso_->generateUncorrelatedField(seed, sp_);
     trend_.generateUncorrelated(seed);
+    invMatvecValid_ = false;
+    invRootMatvecValid_ = false;
   }
 
   template <typename Covariance>
@@ -517,28 +675,184 @@ public:
       iso_->fillTransformedMatrix(std::forward<Covariance>(cov));
   }
 
-  // randomfield.hh:1181-1309: the stochastic part through the matrix, the
-  // trend coefficients scaled by their variances
+  // randomfield.hh:1181-1205: caches the input as C^{-1} (result) and its
+  // root product as C^{-1/2} (result)
   void timesMatrix()
   {
-    sp_ = aniso_ ? (*aniso_) * sp_ : (*iso_) * sp_;
+    if (cacheInvMatvec_) {
+      invMatvec_ = sp_;
+      invMatvecValid_ = true;
+    }
+    if (cacheInvRootMatvec_) {
+      invRootMatvec_ = root(sp_);
+      invRootMatvecValid_ = true;
+    }
+    sp_ = times(sp_);
     trend_.timesMatrix();
   }
+
+  // randomfield.hh:1214-1243: a valid cached C^{-1} x replaces the solve
   void timesInverseMatrix()
   {
-    sp_ = aniso_ ? aniso_->multiplyInverse(sp_) : iso_->multiplyInverse(sp_);
+    if (cacheInvMatvec_ && invMatvecValid_) {
+      if (cacheInvRootMatvec_) {
+        invRootMatvec_ = root(invMatvec_);
+        invRootMatvecValid_ = true;
+      }
+      sp_ = invMatvec_;
+      invMatvecValid_ = false;
+    } else {
+      sp_ = inverse(sp_);
+      if (cacheInvMatvec_)
+        invMatvecValid_ = false;
+      if (cacheInvRootMatvec_)
+        invRootMatvecValid_ = false;
+    }
     trend_.timesInverseMatrix();
   }
+
+  // randomfield.hh:1255-1272
   void timesMatrixRoot()
   {
-    sp_ = aniso_ ? aniso_->multiplyRoot(sp_) : iso_->multiplyRoot(sp_);
+    if (cacheInvMatvec_ && cacheInvRootMatvec_) {
+      invMatvec_ = invRootMatvec_;
+      invMatvecValid_ = invRootMatvecValid_;
+    }
+    if (cacheInvRootMatvec_) {
+      invRootMatvec_ = sp_;
+      invRootMatvecValid_ = true;
+    }
+    sp_ = root(sp_);
     trend_.timesMatrixRoot();
   }
+
+  // randomfield.hh:1282-1310: C^{-1/2} x = root(inverse(x)), or the cached
+  // C^{-1/2} x when valid
   void timesInvMatRoot()
   {
-    sp_ = aniso_ ? aniso_->multiplyInverse(sp_) : iso_->multiplyInverse(sp_);
-    sp_ = aniso_ ? aniso_->multiplyRoot(sp_) : iso_->multiplyRoot(sp_);
+    if (cacheInvRootMatvec_ && invRootMatvecValid_) {
+      sp_ = invRootMatvec_;
+      invRootMatvecValid_ = false;
+      if (cacheInvMatvec_) {
+        invRootMatvec_ = invMatvec_;
+        invRootMatvecValid_ = invMatvecValid_;
+        invMatvecValid_ = false;
+      }
+    } else {
+      sp_ = inverse(sp_);
+      if (cacheInvRootMatvec_) {
+        invRootMatvec_ = sp_;
+        invRootMatvecValid_ = true;
+      }
+      sp_ = root(sp_);
+      if (cacheInvMatvec_)
+        invMatvecValid_ = false;
+    }
     trend_.timesInvMatRoot();
+  }
+
+  // randomfield.hh:854-869
+  void zero()
+  {
+    trend_.zero();
+    for (auto& v : sp_.dataVector)
+      v = RF(0);
+    if (cacheInvMatvec_) {
+      for (auto& v : invMatvec_.dataVector)
+        v = RF(0);
+      invMatvecValid_ = true;
+    }
+    if (cacheInvRootMatvec_) {
+      for (auto& v : invRootMatvec_.dataVector)
+        v = RF(0);
+      invRootMatvecValid_ = true;
+    }
+  }
+
+  // randomfield.hh:880-886: the grid doubles its cells in every axis and the
+  // matrix is rebuilt; refine() then moves the field to the new grid
+  void refineMatrix()
+  {
+    for (auto& c : cells_)
+      c *= 2;
+    make_matrix();
+  }
+  void coarsenMatrix()
+  {
+    for (auto& c : cells_) {
+      if (c % 2)
+        throw std::runtime_error("coarsening needs an even number of cells");
+      c /= 2;
+    }
+    make_matrix();
+  }
+
+  // randomfield.hh:896-940: with a valid cached C^{-1} x the refined field is
+  // C (refined cache) 2^d (smoother than copying cells), likewise from a valid
+  // C^{-1/2} x through the root; otherwise cells are copied
+  void refine() { regrid(true); }
+  // randomfield.hh:959-1004: the same with averaging of cells, scale 2^-d
+  void coarsen() { regrid(false); }
+
+  // arithmetic (randomfield.hh:1023-1150): the caches follow when both
+  // operands hold them, else they become invalid
+  RandomField& operator+=(const RandomField& o) { return axpy(o, RF(1)); }
+  RandomField& operator-=(const RandomField& o) { return axpy(o, RF(-1)); }
+  RandomField& operator*=(RF alpha)
+  {
+    trend_ *= alpha;
+    scale(sp_, alpha);
+    if (cacheInvMatvec_)
+      scale(invMatvec_, alpha);
+    if (cacheInvRootMatvec_)
+      scale(invRootMatvec_, alpha);
+    return *this;
+  }
+  RandomField& axpy(const RandomField& o, RF alpha)
+  {
+    trend_.axpy(o.trend_, alpha);
+    axpy_part(sp_, o.sp_, alpha);
+    if (cacheInvMatvec_) {
+      if (o.cacheInvMatvec_) {
+        axpy_part(invMatvec_, o.invMatvec_, alpha);
+        invMatvecValid_ = invMatvecValid_ && o.invMatvecValid_;
+      } else
+        invMatvecValid_ = false;
+    }
+    if (cacheInvRootMatvec_) {
+      if (o.cacheInvRootMatvec_) {
+        axpy_part(invRootMatvec_, o.invRootMatvec_, alpha);
+        invRootMatvecValid_ = invRootMatvecValid_ && o.invRootMatvecValid_;
+      } else
+        invRootMatvecValid_ = false;
+    }
+    return *this;
+  }
+  RandomField& axpy(RF alpha, const RandomField& o) { return axpy(o, alpha); }
+
+  // scalar product and norms (randomfield.hh:1163-1343)
+  RF operator*(const RandomField& o) const
+  {
+    RF out = 0.;
+    if (sp_.dataVector.size() == o.sp_.dataVector.size())
+      for (size_t i = 0; i < sp_.dataVector.size(); i++)
+        out += sp_.dataVector[i] * o.sp_.dataVector[i];
+    return out + trend_ * o.trend_;
+  }
+  RF oneNorm() const
+  {
+    RF out = 0.;
+    for (RF v : sp_.dataVector)
+      out += std::abs(v);
+    return out + trend_.oneNorm();
+  }
+  RF twoNorm() const { return std::sqrt(*this * *this); }
+  RF infNorm() const
+  {
+    RF out = 0.;
+    for (RF v : sp_.dataVector)
+      out = std::max(out, std::abs(v));
+    return std::max(out, trend_.infNorm());
   }
 
   // randomfield.hh:641-651: stochastic value (cell lookup) plus trend, then
@@ -578,11 +892,65 @@ public:
   }
 
   const TrendPart<RF, dim>& trendPart() const { return trend_; }
+  TrendPart<RF, dim>& trendPart() { return trend_; }
 
   const std::vector<RF>& dataVector() const { return sp_.dataVector; }
   std::vector<RF>& dataVector() { return sp_.dataVector; }
+  const std::vector<RF>& invMatvecData() const { return invMatvec_.dataVector; }
+  const std::vector<RF>& invRootMatvecData() const { return invRootMatvec_.dataVector; }
+  bool invMatvecValid() const { return invMatvecValid_; }
+  bool invRootMatvecValid() const { return invRootMatvecValid_; }
+  const std::array<int, dim>& cells() const { return cells_; }
   bool usesAnisoMatrix() const { return (bool)aniso_; }
   void* plan() const { return aniso_ ? aniso_->plan() : iso_->plan(); }
+
+private:
+  int64_t domain_size() const
+  {
+    int64_t n = 1;
+    for (int i = 0; i < dim; i++)
+      n *= cells_[i];
+    return n;
+  }
+
+  // the field's data is on the grid before the last refineMatrix /
+  // coarsenMatrix; `up` moves it to the current cells
+  void regrid(bool up)
+  {
+    const size_t want = (size_t)domain_size();
+    auto move = [&](StochasticPart<RF>& x) {
+      if (!x.dataVector.empty() && x.dataVector.size() != want)
+        x.dataVector = up ? refine_cells<RF, dim>(x.dataVector, cells_)
+                          : coarsen_cells<RF, dim>(x.dataVector, cells_);
+    };
+    const RF s = (RF)std::pow(2., up ? (double)dim : -(double)dim);
+    if (cacheInvMatvec_ && invMatvecValid_) {
+      move(invMatvec_);
+      sp_ = times(invMatvec_);
+      scale(sp_, s);
+      scale(invMatvec_, s);
+      if (cacheInvRootMatvec_) {
+        invRootMatvec_ = root(invMatvec_);
+        scale(invRootMatvec_, s);
+        invRootMatvecValid_ = true;
+      }
+    } else if (cacheInvRootMatvec_ && invRootMatvecValid_) {
+      move(invRootMatvec_);
+      sp_ = root(invRootMatvec_);
+      scale(sp_, s);
+      scale(invRootMatvec_, s);
+      if (cacheInvMatvec_) {
+        invMatvec_ = sp_;
+        invMatvecValid_ = false;
+      }
+    } else {
+      move(sp_);
+      if (cacheInvMatvec_)
+        move(invMatvec_);
+      if (cacheInvRootMatvec_)
+        move(invRootMatvec_);
+    }
+  }
 };
 
 } // namespace parafields_amd

@@ -10,8 +10,10 @@
 // variances (trend.hh:394-441). The image component (PNG input) is not built.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
+#include <ostream>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -30,6 +32,7 @@ public:
   struct Component
   {
     Kind kind;
+    std::string name; // mean, slope, disk<k>, block<k>
     std::vector<RF> mean, variance, shift;
 
     // value at location x (trend.hh:585-620)
@@ -64,19 +67,92 @@ public:
   TrendPart() = default;
 
   // Components in the reference's order: mean, slope, disk0, disk1, ...,
-  // block0, block1, ... (trend.hh:905-1035); keys <name>.mean and
-  // <name>.variance.
-  TrendPart(const ParameterTree& config, const std::array<double, dim>& ext)
+  // block0, block1, ... (trend.hh:895-1035); keys <name>.mean and
+  // <name>.variance. With a file name, the coefficients come from
+  // <fileName>.trend (one line "<name> = c_0 c_1 ..." per component, as
+  // writeToFile writes them) and the shifts are coefficient - mean
+  // (trend.hh:111-150); without one every shift is 0.
+  TrendPart(const ParameterTree& config, const std::array<double, dim>& ext,
+            const std::string& fileName = "")
     : ext_(ext)
   {
-    add(config, "mean", Kind::Mean, 1);
-    add(config, "slope", Kind::Slope, dim);
-    for (int k = 0; add(config, "disk" + std::to_string(k), Kind::Disk, dim + 2); k++) {
+    ParameterTree stored;
+    if (!fileName.empty())
+      stored.readINIFile(fileName + ".trend");
+    const ParameterTree* file = fileName.empty() ? nullptr : &stored;
+    add(config, file, "mean", Kind::Mean, 1);
+    add(config, file, "slope", Kind::Slope, dim);
+    for (int k = 0; add(config, file, "disk" + std::to_string(k), Kind::Disk, dim + 2); k++) {
     }
-    for (int k = 0; add(config, "block" + std::to_string(k), Kind::Block, 2 * dim + 1); k++) {
+    for (int k = 0; add(config, file, "block" + std::to_string(k), Kind::Block, 2 * dim + 1);
+         k++) {
     }
     if (config.hasKey("image.mean"))
       throw std::runtime_error("the image trend component (PNG input) is not built");
+  }
+
+  // TrendComponent::writeToFile / TrendPart::writeToFile (trend.hh:656-675,
+  // :1515-1527): "<name> = mean_i + shift_i ..." per component
+  void writeToFile(std::ostream& out) const
+  {
+    // 17 significant digits: the coefficients read back bit for bit (the
+    // reference's stream default of 6 digits would round them)
+    const auto prec = out.precision(17);
+    for (const auto& c : comps_) {
+      out << c.name << " =";
+      for (size_t i = 0; i < c.shift.size(); i++)
+        out << " " << c.mean[i] + c.shift[i];
+      out << std::endl;
+    }
+    out.precision(prec);
+  }
+
+  // arithmetic on the shifts (trend.hh:450-645, :1294-1420)
+  TrendPart& operator+=(const TrendPart& o) { return axpy(o, RF(1)); }
+  TrendPart& operator-=(const TrendPart& o) { return axpy(o, RF(-1)); }
+  TrendPart& operator*=(RF alpha)
+  {
+    for (auto& c : comps_)
+      for (auto& s : c.shift)
+        s *= alpha;
+    return *this;
+  }
+  TrendPart& axpy(const TrendPart& o, RF alpha)
+  {
+    for (size_t k = 0; k < comps_.size(); k++)
+      for (size_t i = 0; i < comps_[k].shift.size(); i++)
+        comps_[k].shift[i] += o.comps_.at(k).shift.at(i) * alpha;
+    return *this;
+  }
+  void zero()
+  {
+    for (auto& c : comps_)
+      for (auto& s : c.shift)
+        s = RF(0);
+  }
+  RF operator*(const TrendPart& o) const
+  {
+    RF out = 0.;
+    for (size_t k = 0; k < comps_.size(); k++)
+      for (size_t i = 0; i < comps_[k].shift.size(); i++)
+        out += comps_[k].shift[i] * o.comps_.at(k).shift.at(i);
+    return out;
+  }
+  RF oneNorm() const
+  {
+    RF out = 0.;
+    for (const auto& c : comps_)
+      for (RF s : c.shift)
+        out += std::abs(s);
+    return out;
+  }
+  RF infNorm() const
+  {
+    RF out = 0.;
+    for (const auto& c : comps_)
+      for (RF s : c.shift)
+        out = std::max(out, std::abs(s));
+    return out;
   }
 
   size_t size() const { return comps_.size(); }
@@ -125,12 +201,14 @@ private:
   std::array<double, dim> ext_{};
   std::vector<Component> comps_;
 
-  bool add(const ParameterTree& config, const std::string& name, Kind kind, size_t n)
+  bool add(const ParameterTree& config, const ParameterTree* file, const std::string& name,
+           Kind kind, size_t n)
   {
     if (!config.hasKey(name + ".mean"))
       return false;
     Component c;
     c.kind = kind;
+    c.name = name;
     c.mean = config.get<std::vector<RF>>(name + ".mean");
     c.variance = config.get<std::vector<RF>>(name + ".variance");
     if (c.mean.size() != c.variance.size())
@@ -139,6 +217,13 @@ private:
       throw std::runtime_error("trend component " + name + " must contain " + std::to_string(n) +
                                " parameters");
     c.shift.assign(n, RF(0));
+    if (file) {
+      const auto coef = file->get<std::vector<RF>>(name);
+      if (coef.size() != n)
+        throw std::runtime_error("trend component size does not match");
+      for (size_t i = 0; i < n; i++)
+        c.shift[i] = coef[i] - c.mean[i];
+    }
     comps_.push_back(c);
     return true;
   }

@@ -1,0 +1,144 @@
+// GPU driver of the C++ RandomField beyond generation
+// (include/parafields_amd/randomfield.hh): the on-disk format through a
+// write and a read, the cached inverse products (randomField.cacheInvMatvec,
+// randomField.cacheInvRootMatvec) in timesMatrix / timesInverseMatrix /
+// timesMatrixRoot / timesInvMatRoot, refinement with and without a valid
+// cache, and the arithmetic. Results go to <dir>/<name>.bin (raw doubles)
+// for the test to compare with the oracle.
+//
+//   field_check <dir>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "parafields_amd/randomfield.hh"
+
+using namespace parafields_amd;
+using Field = RandomField<GridTraits<double, double, 2>>;
+
+static std::string dir;
+static int bad = 0;
+
+static void dump(const char* name, const std::vector<double>& v)
+{
+  FILE* f = std::fopen((dir + "/" + name + ".bin").c_str(), "wb");
+  std::fwrite(v.data(), sizeof(double), v.size(), f);
+  std::fclose(f);
+}
+static void expect(bool ok, const char* what)
+{
+  if (!ok) {
+    std::printf("FAIL %s\n", what);
+    bad++;
+  }
+}
+
+static ParameterTree config(bool cache)
+{
+  ParameterTree c;
+  c["grid.cells"] = "32 24";
+  c["grid.extensions"] = "1 0.75";
+  c["stochastic.variance"] = "1.2";
+  c["stochastic.covariance"] = "exponential";
+  c["stochastic.corrLength"] = "0.15";
+  c["mean.mean"] = "0.5";
+  c["mean.variance"] = "0.25";
+  c["slope.mean"] = "1 -1";
+  c["slope.variance"] = "0.5 0.5";
+  c["randomField.transform"] = "logNormal";
+  c["randomField.cgIterations"] = "500";
+  if (cache) {
+    c["randomField.cacheInvMatvec"] = "true";
+    c["randomField.cacheInvRootMatvec"] = "true";
+  }
+  return c;
+}
+
+int main(int argc, char** argv)
+{
+  if (argc < 2)
+    return 2;
+  dir = argv[1];
+  try {
+    // files: write, read back, same values and evaluation
+    {
+      Field f(config(false));
+      f.generate(42u);
+      dump("gen42", f.dataVector());
+      f.writeToFile(dir + "/field");
+      Field g(dir + "/field");
+      expect(g.dataVector() == f.dataVector(), "field values through the HDF5 file");
+      std::vector<double> a, b;
+      std::array<unsigned int, 2> sa, sb;
+      f.bulkEvaluate(a, sa);
+      g.bulkEvaluate(b, sb);
+      double d = 0;
+      for (size_t i = 0; i < a.size(); i++)
+        d = std::max(d, std::abs(a[i] - b[i]) / std::abs(a[i]));
+      expect(sa == sb && d < 1e-14, "bulkEvaluate of the field read back");
+      // the read field generates and multiplies like the written one
+      g.timesMatrix();
+      f.timesMatrix();
+      expect(g.dataVector() == f.dataVector(), "matrix of the field read back");
+    }
+    // caches
+    {
+      Field f(config(true));
+      f.generate(42u);
+      const std::vector<double> x = f.dataVector();
+      f.timesMatrix(); // caches C^{-1}(C x) = x and C^{-1/2}(C x) = root(x)
+      dump("cx", f.dataVector());
+      expect(f.invMatvecValid() && f.invRootMatvecValid(), "timesMatrix fills both caches");
+      expect(f.invMatvecData() == x, "cached inverse product is the input");
+      dump("cache_root", f.invRootMatvecData());
+      f.timesInverseMatrix(); // from the cache: x exactly
+      expect(f.dataVector() == x, "timesInverseMatrix returns the cached input");
+      expect(!f.invMatvecValid(), "the inverse cache is consumed");
+      // timesMatrixRoot caches its input as the inverse-root product
+      f.timesMatrixRoot();
+      dump("root_x", f.dataVector());
+      f.timesInvMatRoot(); // from the cache: x exactly
+      expect(f.dataVector() == x, "timesInvMatRoot returns the cached input");
+      // without valid caches: C^{-1/2} x = root(inverse(x)) on the device
+      Field h(config(false));
+      h.generate(42u);
+      h.timesInvMatRoot();
+      dump("invroot_x", h.dataVector());
+      // refinement: with a valid inverse cache the refined field is
+      // 2^d C_fine refine(C^{-1} x)
+      Field r(config(true));
+      r.generate(42u);
+      r.timesMatrix(); // now C x with the cache x
+      r.refineMatrix();
+      r.refine();
+      dump("refined_cached", r.dataVector());
+      dump("refined_cache", r.invMatvecData());
+      Field q(config(false));
+      q.generate(42u);
+      q.refineMatrix();
+      q.refine(); // copies cells
+      dump("refined_plain", q.dataVector());
+      q.coarsenMatrix();
+      q.coarsen();
+      expect(q.dataVector() == x, "coarsen of a refined field gives the field back");
+      // arithmetic
+      Field s(config(false));
+      s.generate(42u);
+      Field t(config(false));
+      t.generate(7u);
+      const double dot = s * t;
+      s.axpy(t, 2.0);
+      dump("axpy", s.dataVector());
+      dump("gen7", t.dataVector());
+      std::printf("dot %.17g\n", dot);
+      FILE* fd = std::fopen((dir + "/dot.txt").c_str(), "w");
+      std::fprintf(fd, "%.17g\n", dot);
+      std::fclose(fd);
+    }
+  } catch (const std::exception& e) {
+    std::printf("error: %s\n", e.what());
+    return 1;
+  }
+  std::printf(bad ? "FAILURES %d\n" : "all OK\n", bad);
+  return bad;
+}

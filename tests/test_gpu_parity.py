@@ -1028,3 +1028,57 @@ def test_dctdst_multiply_inverse_matches_oracle():
         x, st = pfb.Plan(d).cg(b, mode=mode, rel_tol=1e-10, max_iterations=5000)
         assert st["rel_residual"] <= 1e-10, mode
         assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-6, mode
+
+
+def test_cpp_field_files_caches_refinement(tmp_path):
+    """The C++ RandomField beyond generation (tests/host/field_check.cc): the
+    .stoch.h5 / .trend / .field files written and read back, the cached
+    inverse products of timesMatrix / timesInverseMatrix / timesMatrixRoot /
+    timesInvMatRoot (randomfield.hh:1181-1310), timesInvMatRoot without a
+    cache, refinement with a valid cache (2^d C_fine refine(C^{-1} x)) and
+    without (cell copies), and the arithmetic, against the oracle."""
+    import os
+    import subprocess
+    import test_host_api as tha
+    inc, libs = tha._hdf5_link_args(tmp_path)
+    exe = str(tmp_path / "field_check")
+    libdir = os.path.dirname(pfb.LIB_PATH)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"), *inc,
+                           os.path.join(root, "tests", "host", "field_check.cc"), "-o", exe, *libs,
+                           "-L", libdir, "-lparafields_amd", "-Wl,-rpath," + libdir])
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "all OK" in r.stdout, r.stdout + r.stderr
+
+    def rd(name):
+        return np.fromfile(str(out / f"{name}.bin"), dtype=np.float64)
+
+    d = desc(2, [32, 24], extensions=[1.0, 0.75], covariance="exponential", corr_length=0.15,
+             variance=1.2)
+    n = int(orc.sizes(d).noise_per_field)
+    z42 = orc.noise_cpp(42, n + 3)
+    x = orc.generate(d, z42[:n])
+    assert rel_max(rd("gen42"), x) < FP64_TOL
+    assert rel_max(rd("cx"), orc.matvec(d, x)) < FP64_TOL
+    assert rel_max(rd("cache_root"), orc.matvec(d, x, "root")) < FP64_TOL
+    assert rel_max(rd("root_x"), orc.matvec(d, x, "root")) < FP64_TOL
+    inv, _ = orc.multiply_inverse(d, x)
+    want = orc.matvec(d, inv, "root")
+    assert np.linalg.norm(rd("invroot_x") - want) / np.linalg.norm(want) < 1e-7
+    # cell copies of the device's own field (bit for bit)
+    fine = np.kron(rd("gen42").reshape(24, 32), np.ones((2, 2))).ravel()
+    assert np.array_equal(rd("refined_plain"), fine)
+    assert np.array_equal(rd("refined_cache"), 4 * fine)
+    df = desc(2, [64, 48], extensions=[1.0, 0.75], covariance="exponential", corr_length=0.15,
+              variance=1.2)
+    assert rel_max(rd("refined_cached"), 4 * orc.matvec(df, fine)) < FP64_TOL
+    z7 = orc.noise_cpp(7, n + 3)
+    y = orc.generate(d, z7[:n])
+    assert rel_max(rd("axpy"), x + 2 * y) < FP64_TOL
+    sd = np.sqrt([0.25, 0.5, 0.5])
+    dot = float(open(out / "dot.txt").read())
+    want_dot = x @ y + (z42[n:] * sd) @ (z7[n:] * sd)
+    assert abs(dot - want_dot) <= 1e-10 * abs(want_dot)
