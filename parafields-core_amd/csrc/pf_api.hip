@@ -1429,25 +1429,37 @@ int inner_cg(Plan& P, double* iter, const double* solution, int64_t n, unsigned 
   return (int)count;
 }
 
-// (P)CG with a relative-residual stop. The residual is updated recursively
-// and replaced by the FP64 true residual b - A x every 50 iterations and
-// whenever the recursion claims convergence; the stop is decided on the true
-// residual. Preconditioner: none, the embedding inverse, or the learned
-// radial spectral multiplier (PF_OP_SPECTRAL).
+// (P)CG with a relative-residual stop, device resident: alpha, beta and the
+// dot products stay in device memory (CgScalars), the x/r update is one fused
+// pass that also sums r . r, and the host reads the scalars every
+// kCheck iterations instead of once per dot product. Once the recursive
+// residual meets the tolerance the device stops updating (done); the host
+// then recomputes the FP64 true residual b - A x and either stops or goes on
+// from it (residual replacement: the direction is kept). Every kRefresh
+// iterations the residual is replaced by the true residual as well.
+// Preconditioner: none, the embedding inverse, or the learned radial
+// spectral multiplier (PF_OP_SPECTRAL).
 int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, double rel_tol,
         double* relres)
 {
+  constexpr int kCheck = 10, kRefresh = 50;
   Vec vR, vZ, vP, vAp, vAx;
   double* r = vR.p(n);
-  double* z = vZ.p(n);
+  double* z = mode == PF_CG_PLAIN ? r : vZ.p(n);
   double* p = vP.p(n);
   double* Ap = vAp.p(n);
   double* Ax = vAx.p(n);
-  const unsigned gb = grid1d(n);
+  const unsigned gb = std::min<unsigned>(1024, grid1d(n));
+  DevBuf dsc, dpart;
+  dsc.ensure(sizeof(CgScalars) + sizeof(double));
+  dpart.ensure(1024 * sizeof(double));
+  CgScalars* sc = dsc.as<CgScalars>();
+  double* rz_new = reinterpret_cast<double*>(sc + 1);
+  double* part = dpart.as<double>();
   HIPCHECK(hipMemsetAsync(x, 0, n * sizeof(double), P.stream));
   HIPCHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
-  const double bnorm = std::sqrt(dot(P, b, b, n));
-  if (bnorm == 0.) {
+  const double bb = dot(P, b, b, n);
+  if (bb == 0.) {
     *relres = 0.;
     return 0;
   }
@@ -1456,42 +1468,86 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
       matvec64(P, PF_OP_INVERSE_EMBED, rr, zz);
     else if (mode == PF_CG_LEARNED_PCG)
       matvec64(P, PF_OP_SPECTRAL, rr, zz);
-    else
-      HIPCHECK(hipMemcpyAsync(zz, rr, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
   };
-  prec(r, z);
-  HIPCHECK(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
-  double rz = dot(P, r, z, n);
-  int it = 0;
-  double rr = 1.;
-  while (it < maxit) {
-    matvec64(P, PF_OP_MULTIPLY, p, Ap);
-    const double alpha = rz / dot(P, p, Ap, n);
-    k_axpy<<<gb, 256, 0, P.stream>>>(alpha, p, x, n);
-    k_axpy<<<gb, 256, 0, P.stream>>>(-alpha, Ap, r, n);
-    check_launch();
-    it++;
-    rr = std::sqrt(dot(P, r, r, n)) / bnorm;
-    if (rr <= rel_tol || it % 50 == 0) {
-      matvec64(P, PF_OP_MULTIPLY, x, Ax);
-      k_sub<<<gb, 256, 0, P.stream>>>(b, Ax, r, n);
-      check_launch();
-      rr = std::sqrt(dot(P, r, r, n)) / bnorm;
-      if (rr <= rel_tol)
-        break;
-    }
-    prec(r, z);
-    const double rzn = dot(P, r, z, n);
-    const double beta = rzn / rz;
-    rz = rzn;
-    k_xpby<<<gb, 256, 0, P.stream>>>(z, beta, p, n);
-    check_launch();
-  }
-  if (rr > rel_tol || it == 0) {
+  auto put = [&](const CgScalars& h) {
+    HIPCHECK(hipMemcpyAsync(sc, &h, sizeof h, hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+  };
+  auto true_residual = [&]() {
     matvec64(P, PF_OP_MULTIPLY, x, Ax);
-    k_sub<<<gb, 256, 0, P.stream>>>(b, Ax, r, n);
+    k_sub<<<grid1d(n), 256, 0, P.stream>>>(b, Ax, r, n);
     check_launch();
-    rr = std::sqrt(dot(P, r, r, n)) / bnorm;
+    return dot(P, r, r, n);
+  };
+  {
+    // start: z = M^{-1} r, p = z, rz = r . z
+    if (mode != PF_CG_PLAIN)
+      prec(r, z);
+    HIPCHECK(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
+    CgScalars h{};
+    h.rr = bb;
+    h.rz = mode == PF_CG_PLAIN ? bb : dot(P, r, z, n);
+    h.bb = bb;
+    h.tol2 = rel_tol * rel_tol * bb;
+    h.done = h.rr <= h.tol2 ? 1 : 0;
+    put(h);
+  }
+  int it = 0, since_refresh = 0;
+  double rr = 1.;
+  while (true) {
+    // the plan stream is non-blocking: read the scalars in its order
+    CgScalars h{};
+    HIPCHECK(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+    it = h.it;
+    if (h.done || it >= maxit || since_refresh >= kRefresh) {
+      const double rr2 = true_residual(); // r = b - A x
+      rr = std::sqrt(rr2 / bb);
+      if (rr <= rel_tol || it >= maxit)
+        break;
+      h.rr = rr2;
+      // z and r . z of the replaced residual
+      if (mode != PF_CG_PLAIN)
+        prec(r, z);
+      const double rzn = mode == PF_CG_PLAIN ? rr2 : dot(P, r, z, n);
+      if (h.done) {
+        // the iteration that set `done` stopped before its direction update:
+        // finish it from the true residual, p = z + (r . z / rz) p
+        h.done = 0;
+        put(h);
+        HIPCHECK(hipMemcpyAsync(rz_new, &rzn, sizeof rzn, hipMemcpyHostToDevice, P.stream));
+        HIPCHECK(hipStreamSynchronize(P.stream));
+        k_cg_update_p<<<gb, 256, 0, P.stream>>>(z, p, n, rz_new, sc);
+        k_cg_commit_rz<<<1, 64, 0, P.stream>>>(sc, rz_new);
+        check_launch();
+      } else {
+        // periodic replacement inside the run: the next step size uses the
+        // replaced residual's r . z
+        h.rz = rzn;
+        put(h);
+      }
+      since_refresh = 0;
+      continue;
+    }
+    const int steps = std::min(kCheck, maxit - it);
+    for (int k = 0; k < steps; k++) {
+      matvec64(P, PF_OP_MULTIPLY, p, Ap);
+      k_cg_dot<<<gb, 256, 0, P.stream>>>(p, Ap, n, part, sc);
+      k_cg_finish_pap<<<1, 256, 0, P.stream>>>(part, (int)gb, sc);
+      k_cg_update_xr<<<gb, 256, 0, P.stream>>>(p, Ap, x, r, n, part, sc);
+      k_cg_finish_rr<<<1, 256, 0, P.stream>>>(part, (int)gb, sc);
+      if (mode == PF_CG_PLAIN) {
+        k_cg_finish_rz<<<1, 256, 0, P.stream>>>(nullptr, 0, sc, rz_new);
+      } else {
+        prec(r, z);
+        k_cg_dot<<<gb, 256, 0, P.stream>>>(r, z, n, part, sc);
+        k_cg_finish_rz<<<1, 256, 0, P.stream>>>(part, (int)gb, sc, rz_new);
+      }
+      k_cg_update_p<<<gb, 256, 0, P.stream>>>(z, p, n, rz_new, sc);
+      k_cg_commit_rz<<<1, 64, 0, P.stream>>>(sc, rz_new);
+      check_launch();
+    }
+    since_refresh += steps;
   }
   *relres = rr;
   return it;

@@ -868,4 +868,137 @@ __global__ void k_energy_partial(const double* it, const double* ait, const doub
   }
 }
 
+// ---------------------------------------------------------------------------
+// Device-resident (P)CG: the scalars live in device memory and every kernel
+// reads them there, so an iteration is enqueued without a host round trip.
+// A solve stops updating once `done` is set (the recursive residual met the
+// tolerance); the host looks at the scalars every few iterations.
+// ---------------------------------------------------------------------------
+struct CgScalars
+{
+  double rz;     // r . z (z = M^{-1} r; r . r for plain CG)
+  double pAp;    // p . A p
+  double rr;     // r . r
+  double bb;     // b . b
+  double tol2;   // (rel_tol)^2 * b . b
+  int done;      // 1: the recursive residual met the tolerance
+  int it;        // iterations taken
+};
+
+// Block sums of a . b into partial[blockIdx.x] (deterministic order).
+__device__ __forceinline__ double block_sum(double s)
+{
+  __shared__ double sh[1024 / 64];
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_down(s, off);
+  if (threadIdx.x % 64 == 0)
+    sh[threadIdx.x / 64] = s;
+  __syncthreads();
+  double t = 0.;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++)
+      t += sh[w];
+  return t;
+}
+
+// Sum of nb partials into *out by one block (fixed order of the partials
+// per thread, then the block tree).
+__device__ __forceinline__ double reduce_partials_dev(const double* partial, int nb)
+{
+  double s = 0.;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+    s += partial[i];
+  return block_sum(s);
+}
+
+// partial = a . b over the vector (skipped once the solve is done)
+__global__ void k_cg_dot(const double* a, const double* b, int64_t n, double* partial,
+                         const CgScalars* sc)
+{
+  if (sc->done)
+    return;
+  double s = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i] * b[i];
+  const double t = block_sum(s);
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = t;
+}
+
+// pAp = sum of the partials (one block)
+__global__ void k_cg_finish_pap(const double* partial, int nb, CgScalars* sc)
+{
+  if (sc->done)
+    return;
+  const double t = reduce_partials_dev(partial, nb);
+  if (threadIdx.x == 0)
+    sc->pAp = t;
+}
+
+// x += alpha p, r -= alpha Ap (alpha = rz / pAp), partial sums of r . r
+__global__ void k_cg_update_xr(const double* p, const double* Ap, double* x, double* r, int64_t n,
+                               double* partial, const CgScalars* sc)
+{
+  if (sc->done)
+    return;
+  const double alpha = sc->rz / sc->pAp;
+  double s = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  const double t = block_sum(s);
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = t;
+}
+
+// rr = r . r; counts the iteration and sets done when rr <= tol^2 b.b
+__global__ void k_cg_finish_rr(const double* partial, int nb, CgScalars* sc)
+{
+  if (sc->done)
+    return;
+  const double t = reduce_partials_dev(partial, nb);
+  if (threadIdx.x == 0) {
+    sc->rr = t;
+    sc->it += 1;
+    if (t <= sc->tol2)
+      sc->done = 1;
+  }
+}
+
+// p = z + beta p with beta = rz_new / rz; rz_new from the partials of r . z
+// (plain CG: rz_new = rr, no partials). One kernel per iteration updates p;
+// the first block publishes rz_new after every block has read the old rz, so
+// the scalar is finished by a separate one-block kernel.
+__global__ void k_cg_finish_rz(const double* partial, int nb, CgScalars* sc, double* rz_new)
+{
+  if (sc->done)
+    return;
+  const double t = partial ? reduce_partials_dev(partial, nb) : sc->rr;
+  if (threadIdx.x == 0)
+    *rz_new = t;
+}
+
+__global__ void k_cg_update_p(const double* z, double* p, int64_t n, const double* rz_new,
+                              const CgScalars* sc)
+{
+  if (sc->done)
+    return;
+  const double beta = *rz_new / sc->rz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+__global__ void k_cg_commit_rz(CgScalars* sc, const double* rz_new)
+{
+  if (sc->done || threadIdx.x != 0)
+    return;
+  sc->rz = *rz_new;
+}
+
 } // namespace pfamd
