@@ -338,6 +338,22 @@ def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):
     model, losses = lp.train(x, y, steps=3000, lr=2e-3)
     train_s = time.perf_counter() - t0
     b = np.random.default_rng(11).standard_normal(cells ** 3)
+    # algorithmic HBM bytes of one iteration. A product with the 2x embedding
+    # (N = 2 cells, H = cells + 1 kept x frequencies) is five passes: x R2C
+    # reads the vector (V) and writes the x half spectrum (A = cells^2 H
+    # complex); the forward y pass reads A and writes B = H N cells complex;
+    # the fused z pass reads B, the folded DCT eigenvalue table ((cells+1)^3
+    # doubles) and writes B; the inverse y pass reads B and writes A; the x
+    # C2R pass reads A and writes V. The CG vector work is p.Ap (2V), the x/r
+    # update with r.r (6V) and p = z + beta p (3V); a preconditioned
+    # iteration adds a second product and r.z (2V).
+    V = 8 * cells ** 3
+    A = 16 * cells ** 2 * (cells + 1)
+    B = 16 * (cells + 1) * 2 * cells * cells
+    product = 2 * V + 4 * A + 4 * B + 8 * (cells + 1) ** 3
+    alg_iter = {"plain": product + 11 * V}
+    for mode in ("embed_pcg", "spectral_pcg", "learned_pcg"):
+        alg_iter[mode] = 2 * product + 13 * V
     res = []
     modes = ("plain", "embed_pcg", "spectral_pcg", "learned_pcg")
     for l in held:
@@ -347,7 +363,10 @@ def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):
             _, st = lp.solve(case, b, mode, model if mode == "learned_pcg" else None,
                              max_iterations=max_it)
             row[mode] = {k: round(v, 6) if isinstance(v, float) else v for k, v in st.items()}
-            row[mode]["us_per_iteration"] = round(st["seconds"] / max(1, st["iterations"]) * 1e6, 1)
+            per_it = st["seconds"] / max(1, st["iterations"])
+            row[mode]["us_per_iteration"] = round(per_it * 1e6, 1)
+            row[mode]["alg_MB_per_iteration"] = round(alg_iter[mode] / 1e6, 1)
+            row[mode]["frac_of_hbm"] = round(alg_iter[mode] / per_it / HBM_PEAK, 4)
         sp, sl = row["plain"], row["learned_pcg"]
         row["iteration_ratio"] = round(sp["iterations"] / max(1, sl["iterations"]), 2)
         row["time_ratio"] = round(sp["seconds"] / max(1e-9, sl["seconds"]), 2)

@@ -59,11 +59,15 @@ enum pf_covariance {
 /* stochastic.anisotropy, matrix.hh:476-494 / covariance.hh:18-138 */
 enum pf_anisotropy { PF_ANISO_NONE = 0, PF_ANISO_AXIPARALLEL = 1, PF_ANISO_GEOMETRIC = 2 };
 
-/* MatrixBackend template argument of Matrix, matrix.hh:55-60 */
-enum pf_matrix_backend { PF_MATRIX_DCT = 0, PF_MATRIX_R2C = 1 };
+/* MatrixBackend template argument of Matrix, matrix.hh:55-60; PF_MATRIX_DFT is
+   DFTMatrixBackend (dftmatrixbackend.hh:19-384, the 1-D default) */
+enum pf_matrix_backend { PF_MATRIX_DCT = 0, PF_MATRIX_R2C = 1, PF_MATRIX_DFT = 2 };
 
-/* FieldBackend template argument of Matrix, matrix.hh:55-60 */
-enum pf_field_backend { PF_FIELD_R2C = 0, PF_FIELD_DCTDST = 1 };
+/* FieldBackend template argument of Matrix, matrix.hh:55-60; PF_FIELD_DFT is
+   DFTFieldBackend (dftfieldbackend.hh:19-477, the 1-D default): one draw of
+   2 N_ext normals gives two fields, the real part and the imaginary part (the
+   reference's spare field, matrix.hh:336, :411-420) */
+enum pf_field_backend { PF_FIELD_R2C = 0, PF_FIELD_DCTDST = 1, PF_FIELD_DFT = 2 };
 
 /* arithmetic of the per-realization transforms (eigenvalues are always FP64) */
 enum pf_precision { PF_FP64 = 0, PF_FP32 = 1 };
@@ -108,7 +112,7 @@ enum pf_matvec_op {
 
 /* ---- descriptor: the configuration keys a reference Matrix reads ------- */
 typedef struct pf_desc {
-  int32_t dim;                 /* 2 or 3 */
+  int32_t dim;                 /* 1 (DFT backends), 2 or 3 */
   int32_t cells[3];            /* grid.cells */
   double extensions[3];        /* grid.extensions */
   int32_t embedding_factor;    /* embedding.factor (default 2) */
@@ -251,6 +255,36 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
 #define PF_NOALLOC_SCRATCH_BYTES (64u << 20)
 int pf_generate_noalloc(void* plan, uint64_t seed, uint64_t first_realization, int32_t batch,
                         int32_t transform, double transform_param, void* out);
+
+/* No counterpart in the reference; its analogue is FFTW's planner with wisdom
+ * files (fftw.useWisdom, fftw.measure: dftmatrixbackend.hh:52-70, :164-185).
+ * Launch choices of a plan's generation that change no result beyond
+ * rounding (tile_div, radix_order, lines) or none at all (chunks, overlap,
+ * sub_batch); 0 (overlap: -1) keeps the default. */
+typedef struct pf_tuning {
+  int32_t tile_div;    /* R2C field, y and x passes: tiles of L / tile_div lines (1 or 2) */
+  int32_t chunks;      /* R2C field: chunks of the two-stream pipeline (at most 8) */
+  int32_t overlap;     /* R2C field: two-stream pipeline 0 off, 1 on, -1 default */
+  int32_t lines;       /* generic kernels: at most this many lines per LDS tile */
+  int32_t radix_order; /* generic kernels: 0 radix 8 first, 1 reversed, 2 radix 4 */
+  int32_t sub_batch;   /* DCT/DST and DFT fields: realizations per sub-batch */
+} pf_tuning;
+typedef struct pf_tune_result {
+  pf_tuning best;
+  double ms;           /* one pf_generate of `batch` fields under `best` */
+  double default_ms;   /* the same under the defaults (0 for a cache hit) */
+  int32_t candidates;  /* configurations timed (0 for a cache hit) */
+  int32_t from_cache;
+} pf_tune_result;
+/* The autotuner: times pf_generate of `batch` device-Philox realizations into
+ * `out` (device memory, batch * domain_size values) under each candidate
+ * choice, keeps the fastest in the plan and reports it. cache_path
+ * (optional): a text file of earlier results keyed by device and field
+ * shape; a hit applies the stored choice without timing, a miss appends. */
+int pf_autotune(void* plan, int32_t batch, uint64_t seed, int32_t transform, double transform_param,
+                void* out, const char* cache_path, pf_tune_result* result);
+int pf_set_tuning(void* plan, const pf_tuning* tuning);
+int pf_get_tuning(void* plan, pf_tuning* tuning);
 
 /* Replaces Matrix::operator* / multiplyRoot / multiplyInverseExtended
  * (matrix.hh:142-176, :1443-1646) on `batch` vectors of domain_size values

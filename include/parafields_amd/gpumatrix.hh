@@ -68,6 +68,34 @@ struct GpuDCTDSTFieldBackend
 {
   static constexpr int id = PF_FIELD_DCTDST;
 };
+template <typename Traits>
+struct GpuDFTMatrixBackend
+{
+  static constexpr int id = PF_MATRIX_DFT;
+};
+template <typename Traits>
+struct GpuDFTFieldBackend
+{
+  static constexpr int id = PF_FIELD_DFT;
+};
+// The defaults of DefaultIsoMatrix / DefaultAnisoMatrix, which depend on the
+// dimension (matrix.hh:1660-1736): DCT (iso) or R2C (aniso) matrix with the
+// R2C field for dim > 1, the DFT matrix and field for 1-D.
+template <typename Traits>
+struct GpuIsoMatrixBackend
+{
+  static constexpr int id = Traits::dim == 1 ? PF_MATRIX_DFT : PF_MATRIX_DCT;
+};
+template <typename Traits>
+struct GpuAnisoMatrixBackend
+{
+  static constexpr int id = Traits::dim == 1 ? PF_MATRIX_DFT : PF_MATRIX_R2C;
+};
+template <typename Traits>
+struct GpuDefaultFieldBackend
+{
+  static constexpr int id = Traits::dim == 1 ? PF_FIELD_DFT : PF_FIELD_R2C;
+};
 
 template <typename Traits, template <typename> class MB, template <typename> class FB>
 class GpuMatrix
@@ -92,6 +120,9 @@ private:
   pf_sizes sizes_{};
   mutable bool filled_ = false;
   mutable std::vector<double> custom_; // host covariance values of a callable
+  // DFT field: the second field of the last draw (spareField, matrix.hh:90);
+  // like the reference's, it survives update()
+  mutable std::vector<RF> spare_;
 
   void release()
   {
@@ -229,10 +260,11 @@ public:
       for (int i = 0; i < dim; i++)
         N[i] = (long)f * desc_.cells[i];
       const bool dct = MatrixBackendType::id == PF_MATRIX_DCT;
+      const bool r2c = MatrixBackendType::id == PF_MATRIX_R2C;
       for (int i = 0; i < dim; i++)
         cnt[i] = dct ? N[i] / 2 + 1 : N[i];
-      if (!dct)
-        cnt[0] = 2 * (N[0] / 2 + 1); // padded r2c rows
+      if (r2c)
+        cnt[0] = 2 * (N[0] / 2 + 1); // padded r2c rows (the DFT matrix: every entry)
       std::vector<double> vals((size_t)cnt[0] * cnt[1] * cnt[2]);
       const double inv = 1. / desc_.corr_length[0];
       std::array<RF, dim> x;
@@ -264,6 +296,28 @@ public:
   void generateField(RNG& rngBackend, StochasticPartType& stochasticPart) const
   {
     ensure_filled();
+    if (FieldBackendType::id == PF_FIELD_DFT) {
+      // one draw, two fields: the real part now, the imaginary part on the
+      // next call without drawing (matrix.hh:336, :411-420)
+      std::vector<RF>& out = data(stochasticPart);
+      if (!spare_.empty()) {
+        out = std::move(spare_);
+        spare_.clear();
+        stochasticPart.evalValid = false;
+        return;
+      }
+      std::vector<double> noise((size_t)sizes_.noise_per_field);
+      for (auto& v : noise)
+        v = (double)rngBackend.sample();
+      std::vector<RF> both(2 * (size_t)sizes_.domain_size);
+      check(pf_generate(plan_, noise.data(), 0, 0, 2, PF_TRANSFORM_NONE, 0., both.data(),
+                        PF_MEM_HOST));
+      const auto mid = both.begin() + (std::ptrdiff_t)sizes_.domain_size;
+      out.assign(both.begin(), mid);
+      spare_.assign(mid, both.end());
+      stochasticPart.evalValid = false;
+      return;
+    }
     std::vector<double> noise((size_t)sizes_.noise_per_field);
     for (auto& v : noise)
       v = (double)rngBackend.sample();
@@ -304,13 +358,16 @@ public:
   void* plan() const { return plan_; }
 };
 
-// The reference's default matrices (matrix.hh:1696-1724): DCT matrix with the
+// The reference's default matrices (matrix.hh:1660-1736): DCT matrix with the
 // R2C field for isotropic and axiparallel covariances, the R2C matrix for
-// geometric anisotropy, and the DCT/DST field as a drop-in for either.
+// geometric anisotropy (the DFT matrix and field in 1-D), and the DCT/DST
+// and DFT fields as drop-ins for either.
 template <typename Traits>
-using GpuIsoMatrix = GpuMatrix<Traits, GpuDCTMatrixBackend, GpuR2CFieldBackend>;
+using GpuIsoMatrix = GpuMatrix<Traits, GpuIsoMatrixBackend, GpuDefaultFieldBackend>;
 template <typename Traits>
-using GpuAnisoMatrix = GpuMatrix<Traits, GpuR2CMatrixBackend, GpuR2CFieldBackend>;
+using GpuAnisoMatrix = GpuMatrix<Traits, GpuAnisoMatrixBackend, GpuDefaultFieldBackend>;
+template <typename Traits>
+using GpuDFTMatrix = GpuMatrix<Traits, GpuDFTMatrixBackend, GpuDFTFieldBackend>;
 template <typename Traits>
 using GpuDCTDSTMatrix = GpuMatrix<Traits, GpuDCTMatrixBackend, GpuDCTDSTFieldBackend>;
 

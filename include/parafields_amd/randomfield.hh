@@ -66,6 +66,36 @@ struct DCTDSTFieldBackend
 {
   static constexpr int id = PF_FIELD_DCTDST;
 };
+// DFTMatrixBackend / DFTFieldBackend (dftmatrixbackend.hh, dftfieldbackend.hh):
+// the 1-D defaults (matrix.hh:1660-1736)
+struct DFTMatrixBackend
+{
+  static constexpr int id = PF_MATRIX_DFT;
+};
+struct DFTFieldBackend
+{
+  static constexpr int id = PF_FIELD_DFT;
+};
+
+// Default backends per dimension (DefaultIsoMatrix / DefaultAnisoMatrix,
+// matrix.hh:1696-1736): DCT matrix + R2C field (iso) and R2C matrix + R2C
+// field (aniso) for dim > 1; the DFT matrix and field for 1-D.
+template <int dim>
+struct DefaultBackends
+{
+  using IsoMB = DCTMatrixBackend;
+  using IsoFB = R2CFieldBackend;
+  using AnisoMB = R2CMatrixBackend;
+  using AnisoFB = R2CFieldBackend;
+};
+template <>
+struct DefaultBackends<1>
+{
+  using IsoMB = DFTMatrixBackend;
+  using IsoFB = DFTFieldBackend;
+  using AnisoMB = DFTMatrixBackend;
+  using AnisoFB = DFTFieldBackend;
+};
 
 // test/traits.hh analogue
 template <typename DF, typename RF, unsigned int dimension>
@@ -220,6 +250,7 @@ class Matrix
   mutable void* plan64_ = nullptr;
   pf_sizes sizes_{};
   mutable std::vector<double> custom_; // host covariance values of a callable
+  mutable std::vector<RF> spare_;      // DFT field: the second field of the last draw
 
 public:
   using StochasticPartType = StochasticPart<RF>;
@@ -268,8 +299,8 @@ public:
     const bool dct = MB::id == PF_MATRIX_DCT;
     for (int i = 0; i < dim; i++)
       cnt[i] = dct ? N[i] / 2 + 1 : N[i];
-    if (!dct)
-      cnt[0] = 2 * (N[0] / 2 + 1); // padded r2c rows
+    if (MB::id == PF_MATRIX_R2C)
+      cnt[0] = 2 * (N[0] / 2 + 1); // padded r2c rows (the DFT matrix: every entry)
     std::vector<double> vals((size_t)cnt[0] * cnt[1] * cnt[2]);
     const double inv = 1. / desc_.corr_length[0];
     std::array<RF, dim> x;
@@ -298,6 +329,28 @@ public:
   template <typename RNG>
   void generateField(RNG& rng, StochasticPartType& sp) const
   {
+    if (FB::id == PF_FIELD_DFT) {
+      // one draw gives two fields: the first call returns the real part and
+      // keeps the imaginary part, the next call returns it without drawing
+      // (spareField, matrix.hh:336, :411-420)
+      if (!spare_.empty()) {
+        sp.dataVector = std::move(spare_);
+        spare_.clear();
+        sp.evalValid = false;
+        return;
+      }
+      std::vector<double> noise((size_t)sizes_.noise_per_field);
+      for (auto& v : noise)
+        v = (double)rng.sample();
+      std::vector<RF> both(2 * (size_t)sizes_.domain_size);
+      check(pf_generate(plan_, noise.data(), 0, 0, 2, PF_TRANSFORM_NONE, 0., both.data(),
+                        PF_MEM_HOST));
+      const auto mid = both.begin() + (std::ptrdiff_t)sizes_.domain_size;
+      sp.dataVector.assign(both.begin(), mid);
+      spare_.assign(mid, both.end());
+      sp.evalValid = false;
+      return;
+    }
     std::vector<double> noise((size_t)sizes_.noise_per_field);
     for (auto& v : noise)
       v = (double)rng.sample();
@@ -483,8 +536,10 @@ std::vector<RF> coarsen_cells(const std::vector<RF>& in, const std::array<int, d
 // randomField.cacheInvRootMatvec), evaluation with the value transform,
 // refinement, arithmetic and the on-disk format. Iso/aniso dispatch as in
 // randomfield.hh:153-164.
-template <typename GT, typename IsoMB = DCTMatrixBackend, typename IsoFB = R2CFieldBackend,
-          typename AnisoMB = R2CMatrixBackend, typename AnisoFB = R2CFieldBackend>
+template <typename GT, typename IsoMB = typename DefaultBackends<GT::dim>::IsoMB,
+          typename IsoFB = typename DefaultBackends<GT::dim>::IsoFB,
+          typename AnisoMB = typename DefaultBackends<GT::dim>::AnisoMB,
+          typename AnisoFB = typename DefaultBackends<GT::dim>::AnisoFB>
 class RandomField
 {
 public:

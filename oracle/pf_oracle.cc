@@ -276,8 +276,10 @@ struct Geo
 
 Geo make_geo(const pf_desc& d)
 {
-  if (d.dim != 2 && d.dim != 3)
-    throw std::runtime_error("dim must be 2 or 3");
+  if (d.dim < 1 || d.dim > 3)
+    throw std::runtime_error("dim must be 1, 2 or 3");
+  if (d.dim == 1 && (d.matrix_backend != PF_MATRIX_DFT || d.field_backend != PF_FIELD_DFT))
+    throw std::runtime_error("1-D fields use DFTMatrixBackend and DFTFieldBackend");
   Geo g;
   g.dim = d.dim;
   g.factor = d.periodic ? 1 : d.embedding_factor;
@@ -307,7 +309,9 @@ Geo make_geo(const pf_desc& d)
   }
   g.H = g.N[0] / 2 + 1;
   // fieldtraits.hh:229-267: default dim > 1; 2D with an R2C backend -> false.
-  if (d.transposed >= 0)
+  if (d.dim == 1)
+    g.transposed = false; // fieldtraits.hh:229-233
+  else if (d.transposed >= 0)
     g.transposed = d.transposed != 0;
   else if (d.dim == 2)
     g.transposed = !(d.matrix_backend == PF_MATRIX_R2C || d.field_backend == PF_FIELD_R2C);
@@ -339,7 +343,9 @@ struct Geometry
     } else if (kind == PF_ANISO_GEOMETRIC) {
       // GeneralMatrix, covariance.hh:79-138: explicit cofactor inverse
       const double* c = d.corr_length;
-      if (dim == 3) {
+      if (dim == 1) {
+        m[0] = 1. / c[0];
+      } else if (dim == 3) {
         m[0] = c[4] * c[8] - c[5] * c[7];
         m[1] = c[2] * c[7] - c[1] * c[8];
         m[2] = c[1] * c[5] - c[2] * c[4];
@@ -498,9 +504,11 @@ struct Covariance
           v = 0.;
         else if (dim == 3)
           v = var * (1. - 1.5 * h + 0.5 * std::pow(h, 3));
-        else
+        else if (dim == 2)
           v = var * (1. - 2. / 3.14159265358979323846 *
                             (h * std::sqrt(1. - std::pow(h, 2)) + std::asin(h)));
+        else
+          v = var * (1. - h); // covariance.hh:331-335
         break;
       case PF_COV_EXPONENTIAL:
       case PF_COV_CUSTOM: // custom-iso nugget model: exponential + nugget
@@ -987,12 +995,79 @@ Eigs eigs_r2c(const pf_desc& d, const Geo& g, const double* custom)
   return e;
 }
 
+// Storage index -> logical (k0,k1,k2) of the full grid of the DFT backends:
+// transposeIfNeeded swaps the two last extents (dftmatrixbackend.hh:150-158,
+// dftfieldbackend.hh:140-147).
+void dft_storage_to_k(const Geo& g, Index s, Index k[3])
+{
+  const Index N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  if (g.dim == 2 && g.transposed) {
+    k[1] = s % N1;
+    k[0] = s / N1;
+    k[2] = 0;
+  } else if (g.dim == 3 && g.transposed) {
+    k[0] = s % N0;
+    k[2] = (s / N0) % N2;
+    k[1] = s / (N0 * N2);
+  } else {
+    k[0] = s % N0;
+    k[1] = (s / N0) % N1;
+    k[2] = s / (N0 * N1);
+  }
+}
+
+// DFTMatrixBackend (dftmatrixbackend.hh:209-273): the covariance on every
+// entry of the extended grid (computeMatrixEntries, real part, imaginary part
+// 0), the complex forward transform divided by N_ext, then
+// fillTransformedMatrix's classification (matrix.hh:266-291) over
+// localMatrixSize() = N_ext entries of get(index) = the real part, in the
+// storage order. `custom`: N0 x N1 x N2 values, x fastest.
+Eigs eigs_dft(const pf_desc& d, const Geo& g, const double* custom)
+{
+  Geometry geo(d);
+  Covariance cov(d);
+  const Index N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const Index total = N0 * N1 * N2;
+  std::vector<double> real(total);
+  parallel_for(total, [&](Index b, Index en) {
+    for (Index f = b; f < en; f++) {
+      Index idx[3] = { f % N0, (f / N0) % N1, f / (N0 * N1) };
+      real[f] = custom ? custom[f] : embedded_cov(g, geo, cov, d, idx);
+    }
+  });
+  if (!custom && d.periodization == PF_PERIOD_COFOLD) {
+    const Index C[3] = { N0, N1, N2 };
+    cofold(d, g, C, [&](Index f) -> double& { return real[f]; });
+  }
+  std::vector<cd> spec(total);
+  for (Index f = 0; f < total; f++)
+    spec[f] = cd(real[f], 0.);
+  Index dims[3] = { N0, N1, N2 };
+  for (int ax = 0; ax < g.dim; ax++)
+    c2c_axis(spec.data(), dims, ax, -1);
+  Eigs e;
+  e.v.resize(total);
+  for (Index f = 0; f < total; f++)
+    e.v[f] = spec[f].real() / (double)g.Next;
+  classify(
+    total, d.threshold,
+    [&](Index s) -> double& {
+      Index k[3];
+      dft_storage_to_k(g, s, k);
+      return e.v[k[0] + N0 * (k[1] + N1 * k[2])];
+    },
+    e.st);
+  return e;
+}
+
 Eigs compute_eigs(const pf_desc& d, const Geo& g, const double* custom)
 {
   if (d.matrix_backend == PF_MATRIX_DCT)
     return eigs_dct(d, g, custom);
   if (d.matrix_backend == PF_MATRIX_R2C)
     return eigs_r2c(d, g, custom);
+  if (d.matrix_backend == PF_MATRIX_DFT)
+    return eigs_dft(d, g, custom);
   throw std::runtime_error("unknown matrix backend");
 }
 
@@ -1001,6 +1076,9 @@ Eigs compute_eigs(const pf_desc& d, const Geo& g, const double* custom)
 // R2CMatrixBackend::eval reads the same position, r2cmatrixbackend.hh:337-342).
 double eig_at(const pf_desc& d, const Geo& g, const Eigs& e, Index k0, Index k1, Index k2)
 {
+  // DFTMatrixBackend::eval(indices) reads the full grid (dftmatrixbackend.hh:309-318)
+  if (d.matrix_backend == PF_MATRIX_DFT)
+    return e.v[k0 + g.N[0] * (k1 + g.N[1] * k2)];
   if (d.matrix_backend == PF_MATRIX_DCT) {
     Index P0 = g.N[0] / 2 + 1, P1 = g.N[1] / 2 + 1, P2 = g.N[2] / 2 + 1;
     if (g.dim < 3)
@@ -1183,6 +1261,74 @@ void generate_dctdst(const pf_desc& d, const Geo& g, const Eigs& e, const double
 // used by matvec op 3: bin of the folded position |k/(N/2)|/sqrt(dim).
 thread_local std::vector<double> g_spec_table;
 
+// Backward complex transform of the full grid (dftfieldbackend.hh:166-196)
+// followed by extendedFieldToField of component `comp` (:280-318): the first
+// n_i entries per axis of the real (0) or imaginary (1) part.
+void dft_backward_extract(const Geo& g, std::vector<cd>& spec, double* re, double* im)
+{
+  Index dims[3] = { g.N[0], g.N[1], g.N[2] };
+  for (int ax = 0; ax < g.dim; ax++)
+    c2c_axis(spec.data(), dims, ax, +1);
+  for (Index z = 0; z < g.n[2]; z++)
+    for (Index y = 0; y < g.n[1]; y++)
+      for (Index x = 0; x < g.n[0]; x++) {
+        const cd v = spec[x + g.N[0] * (y + g.N[1] * z)];
+        const Index o = x + g.n[0] * (y + g.n[1] * z);
+        if (re)
+          re[o] = v.real();
+        if (im)
+          im[o] = v.imag();
+      }
+}
+
+// Matrix::generateField, general version (matrix.hh:374-421) with
+// DFTFieldBackend::set (dftfieldbackend.hh:236-240): entry `index` of the
+// storage order gets lambda (rand1 + i rand2) with lambda = sqrt(eval(index))
+// (the matrix and field layouts agree for the DFT and DCT matrices); the
+// backward transform gives the field (real part) and the spare field
+// (imaginary part, hasSpareField, :232).
+void generate_dft(const pf_desc& d, const Geo& g, const Eigs& e, const double* noise,
+                  double* field, double* spare)
+{
+  const Index count = g.N[0] * g.N[1] * g.N[2];
+  std::vector<cd> spec(count);
+  for (Index s = 0; s < count; s++) {
+    Index k[3];
+    dft_storage_to_k(g, s, k);
+    const double lambda = std::sqrt(eig_at(d, g, e, k[0], k[1], k[2]));
+    spec[k[0] + g.N[0] * (k[1] + g.N[1] * k[2])] =
+      cd(lambda * noise[2 * s], lambda * noise[2 * s + 1]);
+  }
+  dft_backward_extract(g, spec, field, spare);
+}
+
+// Products with the DFT field backend (matrix.hh:1443-1646):
+// fieldToExtendedField (dftfieldbackend.hh:244-278: real part, zero
+// padding), forward complex transform / N_ext (:198-226), mult by f(lambda)
+// on both parts (:242), backward transform, real part of the first n_i.
+void matvec_dft(const pf_desc& d, const Geo& g, const Eigs& e, int op, const double* x, double* y)
+{
+  const Index N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  std::vector<cd> spec(N0 * N1 * N2, cd(0., 0.));
+  for (Index z = 0; z < g.n[2]; z++)
+    for (Index yy = 0; yy < g.n[1]; yy++)
+      for (Index xx = 0; xx < g.n[0]; xx++)
+        spec[xx + N0 * (yy + N1 * z)] = cd(x[xx + g.n[0] * (yy + g.n[1] * z)], 0.);
+  Index dims[3] = { N0, N1, N2 };
+  for (int ax = 0; ax < g.dim; ax++)
+    c2c_axis(spec.data(), dims, ax, -1);
+  for (auto& c : spec)
+    c /= (double)g.Next;
+  for (Index k2 = 0; k2 < N2; k2++)
+    for (Index k1 = 0; k1 < N1; k1++)
+      for (Index k0 = 0; k0 < N0; k0++) {
+        const double m = eig_at(d, g, e, k0, k1, k2);
+        const double f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+        spec[k0 + N0 * (k1 + N1 * k2)] *= f;
+      }
+  dft_backward_extract(g, spec, y, nullptr);
+}
+
 int radial_bin(const Geo& g, Index k0, Index k1, Index k2, int nbins)
 {
   Index a = 2 * k0 > g.N[0] ? g.N[0] - k0 : k0;
@@ -1355,6 +1501,10 @@ void matvec(const Engine& en, int op, const double* x, double* y)
     std::vector<double> acc(en.g.domain, 0.);
     matvec_dctdst(en.d, en.g, en.e, op, x, acc.data());
     std::copy(acc.begin(), acc.end(), y);
+    return;
+  }
+  if (en.d.field_backend == PF_FIELD_DFT) {
+    matvec_dft(en.d, en.g, en.e, op, x, y);
     return;
   }
   matvec_r2c(en.d, en.g, en.e, op, x, y);
@@ -1695,8 +1845,12 @@ int pfo_sizes(const pf_desc* d, pf_sizes* s)
     for (int i = 0; i < g.dim; i++)
       P *= g.N[i] / 2 + 1;
     const Index half = g.H * g.N[1] * g.N[2];
-    s->eig_entries = d->matrix_backend == PF_MATRIX_DCT ? P : half;
-    s->noise_per_field = d->field_backend == PF_FIELD_DCTDST ? (P << g.dim) : 2 * half;
+    s->eig_entries = d->matrix_backend == PF_MATRIX_DCT ? P
+                     : d->matrix_backend == PF_MATRIX_DFT ? g.Next
+                                                          : half;
+    s->noise_per_field = d->field_backend == PF_FIELD_DCTDST ? (P << g.dim)
+                         : d->field_backend == PF_FIELD_DFT  ? 2 * g.Next
+                                                             : 2 * half;
   });
   return PF_OK;
 }
@@ -1712,7 +1866,13 @@ int pfo_eigs(const pf_desc* d, const double* custom, double* out, pf_eig_stats* 
       *st = en->e.st;
     if (out) {
       const Geo& g = en->g;
-      if (d->matrix_backend == PF_MATRIX_DCT) {
+      if (d->matrix_backend == PF_MATRIX_DFT) {
+        for (Index s = 0; s < g.Next; s++) {
+          Index k[3];
+          dft_storage_to_k(g, s, k);
+          out[s] = en->e.v[k[0] + g.N[0] * (k[1] + g.N[1] * k[2])];
+        }
+      } else if (d->matrix_backend == PF_MATRIX_DCT) {
         Index P[3];
         for (int i = 0; i < 3; i++)
           P[i] = (i < g.dim) ? g.N[i] / 2 + 1 : 1;
@@ -1739,6 +1899,8 @@ int pfo_eigs(const pf_desc* d, const double* custom, double* out, pf_eig_stats* 
 }
 
 // One realization: noise consumed in generateField's order, untransformed.
+// DFT field: one draw (2 N_ext normals), `field` receives 2 domain values:
+// the field, then the spare field (matrix.hh:411-420).
 int pfo_generate(const pf_desc* d, const double* noise, double* field)
 {
   PFO_TRY({
@@ -1749,6 +1911,8 @@ int pfo_generate(const pf_desc* d, const double* noise, double* field)
     }
     if (d->field_backend == PF_FIELD_DCTDST)
       generate_dctdst(en->d, en->g, en->e, noise, field);
+    else if (d->field_backend == PF_FIELD_DFT)
+      generate_dft(en->d, en->g, en->e, noise, field, field + en->g.domain);
     else
       generate_r2c(en->d, en->g, en->e, noise, field);
   });

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "pf_dctdst.hpp"
+#include "pf_dft.hpp"
 #include "pf_fast.hpp"
 #include "pf_fast_launch.hpp"
 #include "pf_kernels.hpp"
@@ -81,7 +82,11 @@ struct DevBuf
   ~DevBuf() { release(); }
 };
 
-Sched make_sched(int n)
+// Radix schedule of an n-point Stockham transform. order 0: radix 8 first,
+// then 4 / 2, then 3, 5, 7; order 1: the same radices in reverse order;
+// order 2: radix 4 (and one 2) instead of 8. Every order computes the same
+// transform; the autotuner picks one per line length for the generic kernels.
+Sched make_sched(int n, int order = 0)
 {
   Sched s;
   s.n = n;
@@ -92,22 +97,31 @@ Sched make_sched(int n)
     p2++;
   }
   std::vector<int> r;
-  int n8 = p2 / 3, rem = p2 % 3;
-  if (rem == 1 && n8 > 0) {
-    n8--;
-    r.push_back(4);
-    r.push_back(4);
-  } else if (rem == 1)
-    r.push_back(2);
-  else if (rem == 2)
-    r.push_back(4);
-  for (int i = 0; i < n8; i++)
-    r.insert(r.begin(), 8);
+  if (order == 2) {
+    for (int i = 0; i < p2 / 2; i++)
+      r.push_back(4);
+    if (p2 % 2)
+      r.push_back(2);
+  } else {
+    int n8 = p2 / 3, rem = p2 % 3;
+    if (rem == 1 && n8 > 0) {
+      n8--;
+      r.push_back(4);
+      r.push_back(4);
+    } else if (rem == 1)
+      r.push_back(2);
+    else if (rem == 2)
+      r.push_back(4);
+    for (int i = 0; i < n8; i++)
+      r.insert(r.begin(), 8);
+  }
   for (int q : { 3, 5, 7 })
     while (m % q == 0) {
       m /= q;
       r.push_back(q);
     }
+  if (order == 1)
+    std::reverse(r.begin(), r.end());
   if (m != 1)
     throw PfError(PF_EINVAL, "transform length " + std::to_string(n) +
                                " has a prime factor above 7 (supported: 2, 3, 5, 7)");
@@ -141,8 +155,9 @@ size_t lds_budget()
     if (hipGetDevice(&dev) == hipSuccess)
       (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     budget = std::min<size_t>((size_t)v, 65536);
+    // PF_LDS_BUDGET can only lower the 64 KB cap
     if (const char* e = getenv("PF_LDS_BUDGET"))
-      budget = std::min<size_t>((size_t)v, (size_t)atol(e));
+      budget = std::min<size_t>(budget, (size_t)atol(e));
   }
   return budget;
 }
@@ -163,15 +178,21 @@ size_t dct_fast_budget() { return fast_budget(); }
 size_t dct_generic_budget() { return lds_budget(); }
 
 template <typename T>
-Cfg make_cfg(int n, int64_t nlines, int ntab)
+Cfg make_cfg(int n, int64_t nlines, int ntab, int max_lines = 0)
 {
   const size_t cs = sizeof(cplx<T>);
   int L = sizeof(T) == 4 ? 16 : 8;
+  while (max_lines > 0 && L > max_lines)
+    L /= 2;
   while (L > 1 && ((size_t)L * n + ntab) * cs > lds_budget())
     L /= 2;
   if (((size_t)L * n + ntab) * cs > lds_budget())
     throw PfError(PF_EINVAL, "line length " + std::to_string(n) + " exceeds the LDS budget");
   while (L > 1 && L / 2 >= nlines)
+    L /= 2;
+  // small problems: fewer lines per tile until the launch has ~2 workgroups
+  // per CU (a 71-line pass on 16-line tiles is 5 workgroups)
+  while (L > 1 && (nlines + L - 1) / L < 512)
     L /= 2;
   const int me = maxe<T>();
   int threads = (L * n + me - 1) / me;
@@ -206,10 +227,24 @@ struct Timing
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
 };
 
+// Launch choices an autotuner can change without recompiling: the
+// analogue of the FFTW planner choices the reference keeps in wisdom files
+// (fftw.useWisdom, dftmatrixbackend.hh:52-70). Zero / -1 keep the defaults.
+struct Tuning
+{
+  int ldiv = 0;     // y and x passes of the R2C generation: tiles of L / ldiv lines
+  int chunks = 0;   // chunks of the two-stream generation pipeline (0: 8)
+  int overlap = -1; // two-stream pipeline: 0 off, 1 on, -1 default (PF_OVERLAP)
+  int lines = 0;    // generic kernels: at most this many lines per tile
+  int radix = 0;    // generic kernels: radix order of make_sched
+  int sub = 0;      // DCT/DST and DFT fields: realizations per sub-batch (0: the work cap)
+};
+
 struct Plan
 {
   pf_desc d;
   Geo g;
+  Tuning tune;
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -324,6 +359,14 @@ struct Scope
   ~Scope() noexcept(false) { p->mark_end(b, s); }
 };
 
+// Launch configuration of a generic kernel under the plan's tuning.
+template <typename T>
+Cfg pcfg(const Plan& P, int n, int64_t nlines, int ntab)
+{
+  return make_cfg<T>(n, nlines, ntab, P.tune.lines);
+}
+Sched psched(const Plan& P, int n) { return make_sched(n, P.tune.radix); }
+
 void check_launch()
 {
   hipError_t e = hipGetLastError();
@@ -363,11 +406,13 @@ int merge_recursions(const pf_desc& d)
   }
 }
 
+void dft_split_1d(int64_t N, size_t cs, int& A, int& B);
+
 Geo make_geo(const pf_desc& d)
 {
   Geo g{};
-  if (d.dim != 2 && d.dim != 3)
-    throw PfError(PF_EINVAL, "dim must be 2 or 3");
+  if (d.dim < 1 || d.dim > 3)
+    throw PfError(PF_EINVAL, "dim must be 1, 2 or 3");
   g.dim = d.dim;
   g.factor = d.periodic ? 1 : d.embedding_factor;
   if (g.factor < 1)
@@ -394,7 +439,9 @@ Geo make_geo(const pf_desc& d)
     throw PfError(PF_EINVAL, "the extended x size must be even");
   g.H = g.N[0] / 2 + 1;
   g.Hp = (g.H + 15) / 16 * 16;
-  if (d.transposed >= 0)
+  if (d.dim == 1)
+    g.transposed = 0; // fieldtraits.hh:229-233: transposed output needs two dims
+  else if (d.transposed >= 0)
     g.transposed = d.transposed != 0;
   else if (d.dim == 2)
     g.transposed = !(d.matrix_backend == PF_MATRIX_R2C || d.field_backend == PF_FIELD_R2C);
@@ -402,10 +449,18 @@ Geo make_geo(const pf_desc& d)
     g.transposed = 1;
   if (d.field_backend == PF_FIELD_R2C && d.dim == 2 && g.transposed)
     throw PfError(PF_EINVAL, "R2CFieldBackend supports transposed output only for dim > 2");
-  if (d.matrix_backend != PF_MATRIX_DCT && d.matrix_backend != PF_MATRIX_R2C)
+  if (d.matrix_backend != PF_MATRIX_DCT && d.matrix_backend != PF_MATRIX_R2C &&
+      d.matrix_backend != PF_MATRIX_DFT)
     throw PfError(PF_EINVAL, "unknown matrix backend");
-  if (d.field_backend != PF_FIELD_R2C && d.field_backend != PF_FIELD_DCTDST)
+  if (d.field_backend != PF_FIELD_R2C && d.field_backend != PF_FIELD_DCTDST &&
+      d.field_backend != PF_FIELD_DFT)
     throw PfError(PF_EINVAL, "unknown field backend");
+  // 1-D fields: the reference's default DFTMatrixBackend + DFTFieldBackend
+  // (matrix.hh:1660-1736)
+  if (d.dim == 1 && (d.matrix_backend != PF_MATRIX_DFT || d.field_backend != PF_FIELD_DFT))
+    throw PfError(PF_EINVAL, "1-D fields use DFTMatrixBackend and DFTFieldBackend");
+  if (d.field_backend == PF_FIELD_DFT && d.matrix_backend == PF_MATRIX_R2C)
+    throw PfError(PF_EINVAL, "DFTFieldBackend needs DFTMatrixBackend or DCTMatrixBackend");
   if (d.field_backend == PF_FIELD_DCTDST && d.matrix_backend != PF_MATRIX_DCT)
     throw PfError(PF_EINVAL, "DCTDSTFieldBackend requires DCTMatrixBackend");
   // the reference's DCT/DST extraction walk (dctdstfieldbackend.hh:905-930)
@@ -442,11 +497,19 @@ Geo make_geo(const pf_desc& d)
   const int64_t P = (int64_t)g.P[0] * g.P[1] * g.P[2];
   g.half = half;
   g.eig_entries = d.matrix_backend == PF_MATRIX_DCT ? P : half;
-  g.noise_per_field = d.field_backend == PF_FIELD_DCTDST ? (P << g.dim) : 2 * half;
+  // DFT field: the 2 N_ext normals of one draw, which fills two fields
+  g.noise_per_field = d.field_backend == PF_FIELD_DCTDST ? (P << g.dim)
+                      : d.field_backend == PF_FIELD_DFT  ? 2 * g.Next
+                                                         : 2 * half;
   // every transform length must factor into 2, 3, 5, 7
   for (int i = 0; i < d.dim; i++)
     (void)make_sched(g.N[i]);
   (void)make_sched(g.N[0] / 2);
+  if (d.dim == 1) {
+    // the FP64 eigenvalue and product transforms must split into two tiles
+    int A, B;
+    dft_split_1d(g.N[0], sizeof(cplx<double>), A, B);
+  }
   return g;
 }
 
@@ -463,6 +526,8 @@ CovParams make_cov(const pf_desc& d, const Geo& g)
   } else if (p.aniso == PF_ANISO_AXIPARALLEL) {
     for (int i = 0; i < d.dim; i++)
       p.m[i] = 1. / d.corr_length[i];
+  } else if (d.dim == 1) {
+    p.m[0] = 1. / d.corr_length[0]; // GeneralMatrix in 1D, covariance.hh:124-125
   } else {
     const double* c = d.corr_length;
     if (d.dim == 3) {
@@ -745,6 +810,8 @@ bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial
 
 // dev_stats != nullptr: capture-safe mode for batches (no host syncs, no
 // frees); the classification lands in *dev_stats and the caller finalises.
+void compute_eigs_dft_1d(Plan& P, const double* custom_host, const CovParams& cp);
+
 void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats = nullptr)
 {
   const Geo& g = P.g;
@@ -753,7 +820,9 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
   const double inv_next = 1.0 / (double)g.Next;
   const int nbp = 1024;
   P.partial.ensure(nbp * sizeof(ClassifyPartial));
-  if (d.matrix_backend == PF_MATRIX_DCT) {
+  if (g.dim == 1) {
+    compute_eigs_dft_1d(P, custom_host, cp);
+  } else if (d.matrix_backend == PF_MATRIX_DCT) {
     const int64_t total = (int64_t)g.P[0] * g.P[1] * g.P[2];
     P.eig.ensure(total * sizeof(double));
     double* a = P.eig.as<double>();
@@ -803,7 +872,8 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     else
       P.st = reduce_partials(P, nb);
     P.st.entries = total;
-  } else if (g.dim == 2 && d.r2c_check_all && fast_budget() > 0 &&
+  } else if (g.dim == 2 && d.matrix_backend == PF_MATRIX_R2C && d.r2c_check_all &&
+             fast_budget() > 0 &&
              compute_eigs_r2c2d_fast(P, custom_host, dev_stats, cp, inv_next, nbp)) {
     // eigenvalues, classification and square roots done by the specialised
     // 2D path (full real-part check)
@@ -822,10 +892,12 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     double* real = P.scratch.as<double>();
     cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + N0 * rows);
     if (custom_host) {
-      // custom values arrive as the padded 2H x N1 x N2 real array
+      // custom values arrive as the padded 2H x N1 x N2 real array (R2C) or
+      // the N0 x N1 x N2 array (DFT)
+      const int64_t pitch = d.matrix_backend == PF_MATRIX_DFT ? N0 : 2 * H;
       std::vector<double> compact(N0 * rows);
       for (int64_t r = 0; r < rows; r++)
-        std::memcpy(&compact[r * N0], custom_host + r * 2 * H, N0 * sizeof(double));
+        std::memcpy(&compact[r * N0], custom_host + r * pitch, N0 * sizeof(double));
       HIPCHECK(hipMemcpyAsync(real, compact.data(), compact.size() * sizeof(double),
                               hipMemcpyHostToDevice, P.stream));
       HIPCHECK(hipStreamSynchronize(P.stream));
@@ -866,15 +938,19 @@ void compute_eigs(Plan& P, const double* custom_host, ClassifyPartial* dev_stats
     k_scale_complex<<<grid1d(count), 256, 0, P.stream>>>(spec, count, inv_next);
     check_launch();
     const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
-    k_classify_r2c<<<nb, 256, 0, P.stream>>>(spec, count, g.H, g.N[1], g.N[2],
-                                             (g.dim == 3 && g.transposed) ? 1 : 0, d.threshold,
-                                             P.partial.as<ClassifyPartial>(), d.r2c_check_all);
+    if (d.matrix_backend == PF_MATRIX_DFT)
+      kdft_classify<<<nb, 256, 0, P.stream>>>(spec, count, g.H, g.N[0], d.threshold,
+                                              P.partial.as<ClassifyPartial>());
+    else
+      k_classify_r2c<<<nb, 256, 0, P.stream>>>(spec, count, g.H, g.N[1], g.N[2],
+                                               (g.dim == 3 && g.transposed) ? 1 : 0, d.threshold,
+                                               P.partial.as<ClassifyPartial>(), d.r2c_check_all);
     check_launch();
     if (dev_stats)
       k_reduce_partials<<<1, 64, 0, P.stream>>>(P.partial.as<ClassifyPartial>(), nb, dev_stats);
     else
       P.st = reduce_partials(P, nb);
-    P.st.entries = count;
+    P.st.entries = d.matrix_backend == PF_MATRIX_DFT ? g.Next : count;
     k_real_part<<<grid1d(count), 256, 0, P.stream>>>(spec, P.eig.as<double>(), count);
     check_launch();
     if (!dev_stats)
@@ -974,8 +1050,8 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
     Scope sc(&P, "gen_last", sa);
     const cplx<T>* tw = P.table<T>(Nlast);
     if (!fast_gen_last<T>(fc, Nlast, gg, la, sq, nz, seed, first + b0, B1, tw, nb)) {
-      Sched s = make_sched(Nlast);
-      Cfg c = make_cfg<T>(Nlast, nlinesG, Nlast);
+      Sched s = psched(P, Nlast);
+      Cfg c = pcfg<T>(P, Nlast, nlinesG, Nlast);
       dim3 grid(nblocks(nlinesG, c.L), nb);
       if (nz)
         k_gen_last<T, ME, 0><<<grid, c.threads, c.lds, sa>>>(gg, sq, nz, seed, first + b0, B1, s,
@@ -990,9 +1066,11 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
     HIPCHECK(hipEventRecord(ev, sa));
     HIPCHECK(hipStreamWaitEvent(sb, ev, 0));
   }
-  // PF_TILE_DIV=2: half-width tiles for the y and x passes of the generation
+  // half-width tiles for the y and x passes of the generation: the plan's
+  // tuning, else PF_TILE_DIV=2
   const char* td = getenv("PF_TILE_DIV");
-  FastCtx fb{ sb, fast_budget(), (td && td[0] == '2') ? 2 : 1 };
+  const int ldiv = P.tune.ldiv > 0 ? P.tune.ldiv : (td && td[0] == '2') ? 2 : 1;
+  FastCtx fb{ sb, fast_budget(), ldiv };
   const cplx<T>* xin = B1;
   if (dim == 3) {
     Scope sc(&P, "gen_mid", sb);
@@ -1000,8 +1078,8 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
     if (!fast_mid<T>(fb, g.N[1], true, B1, B2, g.Hp, g.N[0] / 2, g.n[2], g.N[1], g.N[1], g.n[1],
                      g.n[1], tw, nb)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
-      Sched s = make_sched(g.N[1]);
-      Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+      Sched s = psched(P, g.N[1]);
+      Cfg c = pcfg<T>(P, g.N[1], nl, g.N[1]);
       dim3 grid(nblocks(nl, c.L), nb);
       k_pass_mid<T, ME, true><<<grid, c.threads, c.lds, sb>>>(
         B1, B2, g.H, g.Hp, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], s, tw, c.L);
@@ -1015,8 +1093,8 @@ void generate_r2c_chunk(Plan& P, hipStream_t sa, hipStream_t sb, hipEvent_t ev,
     Scope sc(&P, "gen_c2r_x", sb);
     const cplx<T>* tw = P.table<T>(g.N[0]);
     if (!fast_c2r<T>(fb, M, xin, out, rows, g.Hp, g.n[0], tw, transform, (T)param, nb)) {
-      Sched s = make_sched(M);
-      Cfg c = make_cfg<T>(M, rows, g.N[0]);
+      Sched s = psched(P, M);
+      Cfg c = pcfg<T>(P, M, rows, g.N[0]);
       dim3 grid(nblocks(rows, c.L), nb);
       k_c2r_x<T, T, ME><<<grid, c.threads, c.lds, sb>>>(xin, out, rows, g.Hp, g.n[0], s, tw, c.L,
                                                        transform, (T)param);
@@ -1042,7 +1120,8 @@ void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, 
   if (dim == 3)
     P.w2.ensure((size_t)g.Hp * g.n[1] * g.n[2] * sizeof(cplx<T>) * batch);
   const char* ov = getenv("PF_OVERLAP");
-  const bool overlap = batch >= 4 && !(ov && ov[0] == '0') && !P.capturing;
+  const bool want = P.tune.overlap >= 0 ? P.tune.overlap != 0 : !(ov && ov[0] == '0');
+  const bool overlap = batch >= 4 && want && !P.capturing;
   if (!overlap) {
     generate_r2c_chunk<T>(P, P.stream, P.stream, nullptr, dnoise, seed, first, 0, batch,
                           transform, param, out, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>());
@@ -1057,7 +1136,7 @@ void generate_r2c(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, 
   // stream2 starts after everything already queued on the plan stream
   HIPCHECK(hipEventRecord(P.ev_join, P.stream));
   HIPCHECK(hipStreamWaitEvent(P.stream2, P.ev_join, 0));
-  const int nchunks = std::min(batch, 8);
+  const int nchunks = std::min(batch, P.tune.chunks > 0 ? std::min(P.tune.chunks, 8) : 8);
   const int per = (batch + nchunks - 1) / nchunks;
   int k = 0;
   for (int b0 = 0; b0 < batch; b0 += per, k++) {
@@ -1135,6 +1214,8 @@ void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t firs
   const size_t gsz = (size_t)ncomb * Pm * dg.P0p * nlast * sizeof(T);
   const size_t hsz = dim == 3 ? (size_t)2 * g.n[1] * g.n[2] * dg.P0p * sizeof(T) : 0;
   int sub = (int)std::max<size_t>(1, work_cap() / (gsz + hsz));
+  if (P.tune.sub > 0)
+    sub = std::min(sub, P.tune.sub);
   sub = std::min(sub, batch);
   P.w1.ensure(gsz * sub);
   if (dim == 3)
@@ -1190,10 +1271,10 @@ void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t firs
       Scope sc(&P, "dct_gen_last");
       if (nz)
         kd_gen_last<T, 0, 0, 0, 0><<<grid, nt, lds, P.stream>>>(
-          dg, sq, nz, seed, first + b0, P.w1.as<T>(), P.table<T>(Nlast), make_sched(M), M, L);
+          dg, sq, nz, seed, first + b0, P.w1.as<T>(), P.table<T>(Nlast), psched(P, M), M, L);
       else
         kd_gen_last<T, 0, 0, 0, 1><<<grid, nt, lds, P.stream>>>(
-          dg, sq, nullptr, seed, first + b0, P.w1.as<T>(), P.table<T>(Nlast), make_sched(M), M, L);
+          dg, sq, nullptr, seed, first + b0, P.w1.as<T>(), P.table<T>(Nlast), psched(P, M), M, L);
       check_launch();
     }
     const T* hin = P.w1.as<T>();
@@ -1207,7 +1288,7 @@ void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t firs
       dim3 grid((g.P[0] - 1) / L + 1, g.n[2], nb);
       Scope sc(&P, "dct_mid");
       kd_mid<T, 0, 0, 0><<<grid, nt, lds, P.stream>>>(dg, P.w1.as<T>(), P.w2.as<T>(),
-                                                      P.table<T>(g.N[1]), make_sched(M), M, L);
+                                                      P.table<T>(g.N[1]), psched(P, M), M, L);
       check_launch();
       hin = P.w2.as<T>();
     }
@@ -1224,11 +1305,342 @@ void generate_dctdst(Plan& P, const double* dnoise, uint64_t seed, uint64_t firs
       dim3 grid((unsigned)((rows + L - 1) / L), nb);
       Scope sc(&P, "dct_x");
       kd_x<T, 0, 0, 0><<<grid, nt, lds, P.stream>>>(dg, hin, out + (int64_t)b0 * g.domain, rows,
-                                                    P.table<T>(g.N[0]), make_sched(M), M, L,
+                                                    P.table<T>(g.N[0]), psched(P, M), M, L,
                                                     transform, (T)param);
       check_launch();
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// DFT field backend (dftfieldbackend.hh) and the 1-D DFT matrix transforms
+// ---------------------------------------------------------------------------
+
+// Longest line a generic tile holds (one line plus its twiddle table).
+int dft_max_line(size_t cs) { return (int)(lds_budget() / (2 * cs)); }
+
+// Four-step split N = A * B of a 1-D transform: A the largest divisor of N
+// that one tile holds, B = N / A (B = 1 when N fits one tile).
+void dft_split_1d(int64_t N, size_t cs, int& A, int& B)
+{
+  const int cap = dft_max_line(cs);
+  for (int64_t a = std::min<int64_t>(N, cap); a >= 1; a--)
+    if (N % a == 0) {
+      if (N / a > cap)
+        break;
+      A = (int)a;
+      B = (int)(N / a);
+      return;
+    }
+  throw PfError(PF_EINVAL, "1-D extended size " + std::to_string(N) +
+                             " has no split into two transforms of at most " +
+                             std::to_string(cap) + " points");
+}
+
+DftGeom dft_geom(const Plan& P, size_t cs)
+{
+  const Geo& g = P.g;
+  DftGeom d{};
+  d.dim = g.dim;
+  d.transposed = g.transposed;
+  d.N0 = g.N[0];
+  d.N1 = g.N[1];
+  d.N2 = g.N[2];
+  d.H = g.H;
+  d.P0 = g.P[0];
+  d.P1 = g.P[1];
+  d.P2 = g.P[2];
+  d.dct = P.d.matrix_backend == PF_MATRIX_DCT ? 1 : 0;
+  d.noise_stride = g.noise_per_field;
+  if (g.dim == 1)
+    dft_split_1d(g.N[0], cs, d.A, d.B);
+  else {
+    d.A = g.N[0];
+    d.B = 1;
+  }
+  return d;
+}
+
+template <typename T>
+void launch_cols(Plan& P, const char* name, bool inv, const cplx<T>* in, cplx<T>* out,
+                 const ColPass& p, int nb)
+{
+  Sched s = psched(P, p.E);
+  Cfg c = pcfg<T>(P, p.E, (int64_t)p.C * p.O, p.E);
+  Scope sc(&P, name);
+  const dim3 grid(nblocks((int64_t)p.C * p.O, c.L), nb);
+  if (inv)
+    kdft_cols<T, maxe<T>(), true><<<grid, c.threads, c.lds, P.stream>>>(in, out, p, s,
+                                                                          P.table<T>(p.E), c.L);
+  else
+    kdft_cols<T, maxe<T>(), false><<<grid, c.threads, c.lds, P.stream>>>(in, out, p, s,
+                                                                           P.table<T>(p.E), c.L);
+  check_launch();
+}
+
+template <typename T, typename TO, bool INV, int OUT>
+void launch_rows(Plan& P, const char* name, const cplx<T>* in, cplx<T>* outc, TO* outf,
+                 const RowPass& r, int E, int nb, int64_t f_first, int64_t nfields, int transform,
+                 double param)
+{
+  Sched s = make_sched(E);
+  Cfg c = pcfg<T>(P, E, r.R, E);
+  Scope sc(&P, name);
+  kdft_rows<T, TO, maxe<T>(), INV, OUT><<<dim3(nblocks(r.R, c.L), nb), c.threads, c.lds,
+                                          P.stream>>>(in, outc, outf, r, s, P.table<T>(E), c.L,
+                                                      f_first, nfields, transform, (T)param);
+  check_launch();
+}
+
+// Forward 1-D transform of `nb` complex arrays of N values in natural order
+// (in -> X through the work array W): column pass over the [nA][nB] view
+// with the twiddle e^{-2 pi i nB kA / N}, then the B-point rows kA written
+// to k = kA + A kB.
+template <typename T>
+void dft_forward_1d(Plan& P, const DftGeom& dg, const cplx<T>* in, cplx<T>* W, cplx<T>* X, int nb)
+{
+  const int64_t N = dg.N0;
+  ColPass p{};
+  p.C = dg.B;
+  p.E = dg.A;
+  p.O = 1;
+  p.keep = dg.A;
+  p.in_es = dg.B;
+  p.in_bs = N;
+  p.out_es = dg.B;
+  p.out_bs = N;
+  p.twN = dg.B > 1 ? N : 0;
+  launch_cols<T>(P, "dft_fwd_cols", false, in, W, p, nb);
+  RowPass r{};
+  r.R = dg.A;
+  r.keep = dg.B;
+  r.in_rs = dg.B;
+  r.in_bs = N;
+  r.out_rs = 1;
+  r.out_es = dg.A;
+  r.out_bs = N;
+  r.limit = N;
+  launch_rows<T, T, false, 0>(P, "dft_fwd_rows", W, X, (T*)nullptr, r, dg.B, nb, 0, 0,
+                              PF_TRANSFORM_NONE, 0.);
+}
+
+// Inverse 1-D transform's row pass geometry: rows nB of A values, element nA
+// of row nB to n = nB + B nA, the first n_keep outputs.
+RowPass dft_inverse_rows_1d(const DftGeom& dg, int64_t n_keep, int64_t out_bs)
+{
+  RowPass r{};
+  r.R = dg.B;
+  r.keep = dg.A;
+  r.in_rs = dg.A;
+  r.in_bs = dg.N0;
+  r.out_rs = 1;
+  r.out_es = dg.B;
+  r.out_bs = out_bs;
+  r.limit = n_keep;
+  return r;
+}
+
+ColPass dft_inverse_cols_1d(const DftGeom& dg)
+{
+  ColPass p{};
+  p.C = dg.A;
+  p.E = dg.B;
+  p.O = 1;
+  p.keep = dg.B;
+  p.in_es = dg.A;
+  p.in_bs = dg.N0;
+  p.out_es = dg.A;
+  p.out_bs = dg.N0;
+  p.twN = dg.B > 1 ? dg.N0 : 0;
+  return p;
+}
+
+// DFT matrix eigenvalues of a 1-D embedding (dftmatrixbackend.hh:209-236):
+// covariance on the N-point grid, forward transform, real parts / N; the
+// classification of fillTransformedMatrix (matrix.hh:266-291) over the full
+// array. `custom_host`: N covariance values instead of the covariance family.
+void compute_eigs_dft_1d(Plan& P, const double* custom_host, const CovParams& cp)
+{
+  const Geo& g = P.g;
+  const pf_desc& d = P.d;
+  const int64_t N = g.N[0];
+  DftGeom dg = dft_geom(P, sizeof(cplx<double>));
+  P.eig.ensure((size_t)g.H * sizeof(double));
+  P.scratch.ensure((size_t)N * (sizeof(double) + 3 * sizeof(cplx<double>)));
+  double* real = P.scratch.as<double>();
+  cplx<double>* Z = reinterpret_cast<cplx<double>*>(real + N);
+  cplx<double>* W = Z + N;
+  cplx<double>* X = W + N;
+  if (custom_host) {
+    HIPCHECK(hipMemcpyAsync(real, custom_host, N * sizeof(double), hipMemcpyHostToDevice,
+                            P.stream));
+  } else {
+    Scope sc(&P, "eig_cov");
+    with_cov(cp.covariance, [&](auto c) {
+      constexpr int C = decltype(c)::value;
+      with_period(cp.period, [&](auto q) {
+        k_eval_cov_1d<C, decltype(q)::value><<<grid1d(N), 256, 0, P.stream>>>(cp, N, real);
+      });
+    });
+    check_launch();
+    if (d.periodization == PF_PERIOD_COFOLD)
+      apply_cofold(P, cp, real, g.N[0], 1, 1);
+  }
+  kdft_pad<double, double><<<grid1d(N), 256, 0, P.stream>>>(real, Z, N, N, 1);
+  check_launch();
+  dft_forward_1d<double>(P, dg, Z, W, X, 1);
+  const unsigned nb = std::min<unsigned>(1024, grid1d(g.H));
+  kdft_finish_eigs_1d<<<nb, 256, 0, P.stream>>>(X, N, g.H, 1.0 / (double)N, d.threshold,
+                                                 P.eig.as<double>(),
+                                                 P.partial.as<ClassifyPartial>());
+  check_launch();
+  P.st = reduce_partials(P, nb);
+  P.st.entries = N;
+  P.scratch.release();
+}
+
+// Matrix::generateField with the DFT field backend (matrix.hh:374-421,
+// dftfieldbackend.hh:236-240, :280-318): draw j fills the spectrum with
+// lambda (rand1 + i rand2), the backward transform gives field 2j as its real
+// part and field 2j + 1 (the reference's spare field, handed out by the next
+// generateField call) as its imaginary part. Output field i of this call is
+// field first + i: draw (first + i) / 2, part (first + i) % 2. With host noise
+// the draws are 0 .. ceil(batch / 2) - 1 of the noise array.
+template <typename T>
+void generate_dft(Plan& P, const double* dnoise, uint64_t seed, uint64_t first, int batch,
+                  int transform, double param, T* out)
+{
+  const Geo& g = P.g;
+  const size_t cs = sizeof(cplx<T>);
+  DftGeom dg = dft_geom(P, cs);
+  const T* sq = sizeof(T) == 4 ? (const T*)P.sq32.p : (const T*)P.sq64.p;
+  const uint64_t d_begin = dnoise ? 0 : first / 2;
+  const int64_t f_first = dnoise ? 0 : 2 * (int64_t)d_begin - (int64_t)first;
+  const int ndraws = (int)((batch - f_first + 1) / 2);
+  // per-draw work arrays
+  int64_t w1 = 0, w2 = 0;
+  if (g.dim == 3) {
+    w1 = (int64_t)g.n[2] * g.N[1] * g.N[0];
+    w2 = (int64_t)g.n[2] * g.n[1] * g.N[0];
+  } else if (g.dim == 2) {
+    w1 = (int64_t)g.n[1] * g.N[0];
+  } else {
+    w1 = g.N[0];
+  }
+  int sub = (int)std::max<int64_t>(1, (int64_t)(work_cap() / ((w1 + w2) * cs)));
+  if (P.tune.sub > 0)
+    sub = std::min(sub, P.tune.sub);
+  sub = std::min(sub, ndraws);
+  P.w1.ensure((size_t)w1 * cs * sub);
+  if (w2)
+    P.w2.ensure((size_t)w2 * cs * sub);
+  cplx<T>* W1 = P.w1.as<cplx<T>>();
+  cplx<T>* W2 = w2 ? P.w2.as<cplx<T>>() : nullptr;
+  for (int j0 = 0; j0 < ndraws; j0 += sub) {
+    const int nb = std::min(sub, ndraws - j0);
+    const double* nz = dnoise ? dnoise + (int64_t)j0 * g.noise_per_field : nullptr;
+    // pass 1: spectrum drawn in registers, backward along the last axis
+    ColPass p{};
+    if (g.dim == 3) {
+      p.C = g.N[0];
+      p.E = g.N[2];
+      p.O = g.N[1];
+      p.keep = g.n[2];
+      p.out_es = (int64_t)g.N[1] * g.N[0];
+      p.out_os = g.N[0];
+    } else if (g.dim == 2) {
+      p.C = g.N[0];
+      p.E = g.N[1];
+      p.O = 1;
+      p.keep = g.n[1];
+      p.out_es = g.N[0];
+    } else {
+      p = dft_inverse_cols_1d(dg);
+    }
+    p.out_bs = w1;
+    {
+      Sched s = psched(P, p.E);
+      Cfg c = pcfg<T>(P, p.E, (int64_t)p.C * p.O, p.E);
+      Scope sc(&P, "dft_gen");
+      const dim3 grid(nblocks((int64_t)p.C * p.O, c.L), nb);
+      if (nz)
+        kdft_gen<T, maxe<T>(), 0><<<grid, c.threads, c.lds, P.stream>>>(
+          dg, sq, nz, seed, d_begin + j0, W1, p, s, P.table<T>(p.E), c.L);
+      else
+        kdft_gen<T, maxe<T>(), 1><<<grid, c.threads, c.lds, P.stream>>>(
+          dg, sq, nullptr, seed, d_begin + j0, W1, p, s, P.table<T>(p.E), c.L);
+      check_launch();
+    }
+    const cplx<T>* rows_in = W1;
+    RowPass r{};
+    int E = g.N[0];
+    if (g.dim == 3) {
+      // pass 2: y lines of W1[z][ky][kx], first n1 outputs -> W2[z][y][kx]
+      ColPass q{};
+      q.C = g.N[0];
+      q.E = g.N[1];
+      q.O = g.n[2];
+      q.keep = g.n[1];
+      q.in_es = g.N[0];
+      q.in_os = (int64_t)g.N[1] * g.N[0];
+      q.in_bs = w1;
+      q.out_es = g.N[0];
+      q.out_os = (int64_t)g.n[1] * g.N[0];
+      q.out_bs = w2;
+      launch_cols<T>(P, "dft_mid", true, W1, W2, q, nb);
+      rows_in = W2;
+      r.R = (int64_t)g.n[1] * g.n[2];
+      r.in_bs = w2;
+    } else if (g.dim == 2) {
+      r.R = g.n[1];
+      r.in_bs = w1;
+    }
+    if (g.dim >= 2) {
+      r.keep = g.n[0];
+      r.in_rs = g.N[0];
+      r.out_rs = g.n[0];
+      r.out_es = 1;
+      r.limit = g.domain;
+    } else {
+      r = dft_inverse_rows_1d(dg, g.n[0], g.domain);
+      E = dg.A;
+    }
+    r.out_bs = g.domain;
+    launch_rows<T, T, true, 1>(P, "dft_x", rows_in, nullptr, out, r, E, nb,
+                               f_first + 2 * (int64_t)j0, batch, transform, param);
+  }
+}
+
+// Products of the 1-D DFT field (matrix.hh:1443-1646 with
+// dftfieldbackend.hh:166-196, :244-318): zero-padded forward transform,
+// times f(lambda) / N, backward transform, the real part of the first n0
+// values.
+template <typename T>
+void matvec_dft_1d(Plan& P, int op, const T* x, T* y, int batch)
+{
+  const Geo& g = P.g;
+  if (op != PF_OP_MULTIPLY && op != PF_OP_ROOT && op != PF_OP_INVERSE_EMBED)
+    throw PfError(PF_EINVAL, "operator not available for 1-D fields");
+  const int64_t N = g.N[0];
+  DftGeom dg = dft_geom(P, sizeof(cplx<T>));
+  P.w1.ensure((size_t)N * batch * sizeof(cplx<T>));
+  P.w2.ensure((size_t)N * batch * sizeof(cplx<T>));
+  cplx<T>* W1 = P.w1.as<cplx<T>>();
+  cplx<T>* W2 = P.w2.as<cplx<T>>();
+  kdft_pad<T, T><<<grid1d(N * batch), 256, 0, P.stream>>>(x, W2, g.n[0], N, batch);
+  check_launch();
+  // forward: W2 -> (W1) -> W2, natural frequency order
+  dft_forward_1d<T>(P, dg, W2, W1, W2, batch);
+  {
+    Scope sc(&P, "dft_mult");
+    kdft_mult_1d<T><<<grid1d(N * batch), 256, 0, P.stream>>>(W2, N, g.H, P.eig.as<double>(), op,
+                                                             1.0 / (double)N, batch);
+    check_launch();
+  }
+  launch_cols<T>(P, "dft_inv_cols", true, W2, W1, dft_inverse_cols_1d(dg), batch);
+  launch_rows<T, T, true, 2>(P, "dft_inv_rows", W1, nullptr, y,
+                             dft_inverse_rows_1d(dg, g.n[0], g.n[0]), dg.A, batch, 0, batch,
+                             PF_TRANSFORM_NONE, 0.);
 }
 
 // ---------------------------------------------------------------------------
@@ -1254,8 +1666,8 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
     Scope sc(&P, "mv_r2c_x");
     const cplx<T>* tw = P.table<T>(g.N[0]);
     if (!fast_r2c<T>(fc, M, x, P.w1.as<cplx<T>>(), rows, g.Hp, g.n[0], tw, batch)) {
-      Sched s = make_sched(M);
-      Cfg c = make_cfg<T>(M, rows, g.N[0]);
+      Sched s = psched(P, M);
+      Cfg c = pcfg<T>(P, M, rows, g.N[0]);
       k_r2c_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
         x, P.w1.as<cplx<T>>(), rows, g.Hp, g.n[0], s, tw, c.L);
     }
@@ -1268,8 +1680,8 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
     if (!fast_mid<T>(fc, g.N[1], false, P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.Hp, g.N[0] / 2,
                      g.n[2], g.n[1], g.n[1], g.N[1], g.N[1], tw, batch)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
-      Sched s = make_sched(g.N[1]);
-      Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+      Sched s = psched(P, g.N[1]);
+      Cfg c = pcfg<T>(P, g.N[1], nl, g.N[1]);
       k_pass_mid<T, ME, false><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
         P.w1.as<cplx<T>>(), P.w2.as<cplx<T>>(), g.H, g.Hp, g.n[2], g.n[1], g.n[1], g.N[1], g.N[1],
         s, tw, c.L);
@@ -1284,8 +1696,8 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
     const cplx<T>* tw = P.table<T>(Nlast);
     if (!fast_fused<T>(fc, Nlast, C, gg, la, P.eig.as<double>(), op, 1.0 / (double)g.Next, tw,
                        batch, P.spec.as<double>(), P.spec_nbins)) {
-      Sched s = make_sched(Nlast);
-      Cfg c = make_cfg<T>(Nlast, nlines, Nlast);
+      Sched s = psched(P, Nlast);
+      Cfg c = pcfg<T>(P, Nlast, nlines, Nlast);
       k_fused_mult<T, ME><<<dim3(nblocks(nlines, c.L), batch), c.threads, c.lds, P.stream>>>(
         C, gg, P.eig.as<double>(), op, 1.0 / (double)g.Next, s, tw, c.L, P.spec.as<double>(),
         P.spec_nbins);
@@ -1298,8 +1710,8 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
     if (!fast_mid<T>(fc, g.N[1], true, P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.Hp, g.N[0] / 2,
                      g.n[2], g.N[1], g.N[1], g.n[1], g.n[1], tw, batch)) {
       const int64_t nl = (int64_t)g.H * g.n[2];
-      Sched s = make_sched(g.N[1]);
-      Cfg c = make_cfg<T>(g.N[1], nl, g.N[1]);
+      Sched s = psched(P, g.N[1]);
+      Cfg c = pcfg<T>(P, g.N[1], nl, g.N[1]);
       k_pass_mid<T, ME, true><<<dim3(nblocks(nl, c.L), batch), c.threads, c.lds, P.stream>>>(
         P.w2.as<cplx<T>>(), P.w1.as<cplx<T>>(), g.H, g.Hp, g.n[2], g.N[1], g.N[1], g.n[1], g.n[1],
         s, tw, c.L);
@@ -1311,8 +1723,8 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
     const cplx<T>* tw = P.table<T>(g.N[0]);
     if (!fast_c2r<T>(fc, M, P.w1.as<cplx<T>>(), y, rows, g.Hp, g.n[0], tw, PF_TRANSFORM_NONE, T(0),
                      batch)) {
-      Sched s = make_sched(M);
-      Cfg c = make_cfg<T>(M, rows, g.N[0]);
+      Sched s = psched(P, M);
+      Cfg c = pcfg<T>(P, M, rows, g.N[0]);
       k_c2r_x<T, T, ME><<<dim3(nblocks(rows, c.L), batch), c.threads, c.lds, P.stream>>>(
         P.w1.as<cplx<T>>(), y, rows, g.Hp, g.n[0], s, tw, c.L, PF_TRANSFORM_NONE, T(0));
     }
@@ -1378,7 +1790,10 @@ void matvec64(Plan& P, int op, const double* x, double* y)
 {
   if (op == PF_OP_SPECTRAL && P.spec_nbins <= 0)
     throw PfError(PF_EINVAL, "no learned multipliers loaded");
-  matvec_r2c<double>(P, op, x, y, 1);
+  if (P.g.dim == 1)
+    matvec_dft_1d<double>(P, op, x, y, 1);
+  else
+    matvec_r2c<double>(P, op, x, y, 1);
 }
 
 // Matrix::innerCG, matrix.hh:1315-1432, on device vectors.
@@ -1957,6 +2372,114 @@ void generate_r2c_noalloc(Plan& P, uint64_t seed, uint64_t first, int batch, int
   }
 }
 
+// Generation dispatch by field backend and precision (device output).
+void run_generate(Plan& P, const double* dn, uint64_t seed, uint64_t first, int batch,
+                  int transform, double param, void* dout)
+{
+  const int fb = P.d.field_backend;
+  if (P.d.precision == PF_FP32) {
+    if (fb == PF_FIELD_DFT)
+      generate_dft<float>(P, dn, seed, first, batch, transform, param, (float*)dout);
+    else if (fb == PF_FIELD_DCTDST)
+      generate_dctdst<float>(P, dn, seed, first, batch, transform, param, (float*)dout);
+    else
+      generate_r2c<float>(P, dn, seed, first, batch, transform, param, (float*)dout);
+  } else {
+    if (fb == PF_FIELD_DFT)
+      generate_dft<double>(P, dn, seed, first, batch, transform, param, (double*)dout);
+    else if (fb == PF_FIELD_DCTDST)
+      generate_dctdst<double>(P, dn, seed, first, batch, transform, param, (double*)dout);
+    else
+      generate_r2c<double>(P, dn, seed, first, batch, transform, param, (double*)dout);
+  }
+}
+
+// Whether every pass of the plan's generation has a specialised kernel (then
+// the generic kernels' tile lines and radix order do not matter).
+bool all_specialised(const Plan& P)
+{
+  if (fast_budget() == 0)
+    return false;
+  const Geo& g = P.g;
+  if (P.d.field_backend == PF_FIELD_DFT)
+    return false;
+  if (P.d.field_backend == PF_FIELD_DCTDST) {
+    DctGeom dg = dct_geom(P);
+    FastCtx q{ nullptr, dct_fast_budget() };
+    q.query = true;
+    const int Nlast = g.dim == 3 ? g.N[2] : g.N[1];
+    const int64_t xrows = (int64_t)g.n[1] * g.n[2];
+    if (P.d.precision == PF_FP32)
+      return fast_dct_gen_last<float>(q, Nlast / 2, dg, nullptr, nullptr, 0, 0, nullptr, nullptr, 1) &&
+             (g.dim != 3 || fast_dct_mid<float>(q, g.N[1] / 2, dg, nullptr, nullptr, nullptr, 1)) &&
+             fast_dct_x<float>(q, g.N[0] / 2, dg, nullptr, nullptr, xrows, nullptr, 0, 0.f, 1);
+    return fast_dct_gen_last<double>(q, Nlast / 2, dg, nullptr, nullptr, 0, 0, nullptr, nullptr, 1) &&
+           (g.dim != 3 || fast_dct_mid<double>(q, g.N[1] / 2, dg, nullptr, nullptr, nullptr, 1)) &&
+           fast_dct_x<double>(q, g.N[0] / 2, dg, nullptr, nullptr, xrows, nullptr, 0, 0., 1);
+  }
+  FastCtx q{ nullptr, fast_budget() };
+  q.query = true;
+  const GenGeom gg = gen_geom(P);
+  const LastAxis la = last_axis(g);
+  const int Nlast = g.dim == 3 ? g.N[2] : g.N[1];
+  const int64_t rows = (int64_t)g.n[1] * g.n[2];
+  auto all = [&](auto t) {
+    using T = decltype(t);
+    return fast_gen_last<T>(q, Nlast, gg, la, nullptr, nullptr, 0, 0, nullptr, nullptr, 1) &&
+           (g.dim != 3 || fast_mid<T>(q, g.N[1], true, nullptr, nullptr, g.Hp, g.N[0] / 2, g.n[2],
+                                      g.N[1], g.N[1], g.n[1], g.n[1], nullptr, 1)) &&
+           fast_c2r<T>(q, g.N[0] / 2, nullptr, nullptr, rows, g.Hp, g.n[0], nullptr, 0, T(0), 1);
+  };
+  return P.d.precision == PF_FP32 ? all(0.f) : all(0.);
+}
+
+// Autotuner cache key: device, field geometry and batch.
+std::string tune_key(const Plan& P, int batch)
+{
+  hipDeviceProp_t prop{};
+  HIPCHECK(hipGetDeviceProperties(&prop, P.device));
+  const Geo& g = P.g;
+  char k[512];
+  std::snprintf(k, sizeof k, "%s:%dcu|%s|m%d|f%d|d%d|n%dx%dx%d|N%dx%dx%d|t%d|b%d", prop.gcnArchName,
+                prop.multiProcessorCount, P.d.precision == PF_FP32 ? "fp32" : "fp64",
+                P.d.matrix_backend, P.d.field_backend, g.dim, g.n[0], g.n[1], g.n[2], g.N[0],
+                g.N[1], g.N[2], g.transposed, batch);
+  return k;
+}
+
+bool tune_lookup(const std::string& path, const std::string& key, Tuning& t, double& ms)
+{
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f)
+    return false;
+  char line[1024];
+  bool found = false;
+  while (std::fgets(line, sizeof line, f)) {
+    char k[600];
+    Tuning x;
+    double m = 0.;
+    if (std::sscanf(line, "%599s %d %d %d %d %d %d %lf", k, &x.ldiv, &x.chunks, &x.overlap,
+                    &x.lines, &x.radix, &x.sub, &m) == 8 &&
+        key == k) {
+      t = x;
+      ms = m;
+      found = true; // the last entry of a key wins
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+void tune_store(const std::string& path, const std::string& key, const Tuning& t, double ms)
+{
+  FILE* f = std::fopen(path.c_str(), "a");
+  if (!f)
+    throw PfError(PF_EINVAL, "cannot write the tuning cache " + path);
+  std::fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), t.ldiv, t.chunks, t.overlap,
+               t.lines, t.radix, t.sub, ms);
+  std::fclose(f);
+}
+
 } // namespace
 
 extern "C" {
@@ -2005,7 +2528,8 @@ int pf_plan_sizes(void* plan, pf_sizes* s)
     s->transposed = g.transposed;
     s->domain_size = g.domain;
     s->extended_size = g.Next;
-    s->eig_entries = g.eig_entries;
+    // DFT matrix: the reference's array holds every entry of the grid
+    s->eig_entries = P->d.matrix_backend == PF_MATRIX_DFT ? g.Next : g.eig_entries;
     s->noise_per_field = g.noise_per_field;
     return PF_OK;
   })
@@ -2060,14 +2584,42 @@ int pf_eigs_copy(void* plan, double* out, int64_t n)
     if (!P->eig_valid)
       compute_eigs(*P, nullptr);
     const Geo& g = P->g;
-    if (n > g.eig_entries)
+    const bool dft = P->d.matrix_backend == PF_MATRIX_DFT;
+    if (n > (dft ? g.Next : g.eig_entries))
       throw PfError(PF_EINVAL, "n exceeds the number of eigenvalues");
     std::vector<double> logical(g.eig_entries);
     HIPCHECK(hipMemcpy(logical.data(), P->eig.p, g.eig_entries * sizeof(double),
                        hipMemcpyDeviceToHost));
     // reference storage order (3D transposed: x, z, y)
     const bool tr = g.transposed;
-    if (P->d.matrix_backend == PF_MATRIX_DCT) {
+    if (dft) {
+      // every entry of the grid in the storage order of DFTMatrixBackend
+      // (last two axes swapped when transposed, dftmatrixbackend.hh:150-158);
+      // the table holds kx < H, the rest is Re X(-k)
+      const int64_t N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+      for (int64_t s = 0; s < n; s++) {
+        int64_t k0, k1, k2;
+        if (g.dim == 2 && tr) {
+          k1 = s % N1;
+          k0 = s / N1;
+          k2 = 0;
+        } else if (g.dim == 3 && tr) {
+          k0 = s % N0;
+          k2 = (s / N0) % N2;
+          k1 = s / (N0 * N2);
+        } else {
+          k0 = s % N0;
+          k1 = (s / N0) % N1;
+          k2 = s / (N0 * N1);
+        }
+        if (k0 >= g.H) {
+          k0 = N0 - k0;
+          k1 = (N1 - k1) % N1;
+          k2 = (N2 - k2) % N2;
+        }
+        out[s] = logical[k0 + g.H * (k1 + N1 * k2)];
+      }
+    } else if (P->d.matrix_backend == PF_MATRIX_DCT) {
       int perm[3] = { 0, 1, 2 };
       if (tr)
         std::swap(perm[g.dim - 2], perm[g.dim - 1]);
@@ -2109,9 +2661,12 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
       throw PfError(PF_EINVAL, "transform type not known");
     ensure_eigs(*P);
     const Geo& g = P->g;
+    const bool dftf = P->d.field_backend == PF_FIELD_DFT;
     const double* dn = nullptr;
     if (noise) {
-      const size_t nb = (size_t)batch * g.noise_per_field * sizeof(double);
+      // DFT field: one draw of noise_per_field normals fills two fields
+      const int64_t draws = dftf ? (batch + 1) / 2 : batch;
+      const size_t nb = (size_t)draws * g.noise_per_field * sizeof(double);
       P->noise.ensure(nb);
       HIPCHECK(hipMemcpyAsync(P->noise.p, noise, nb, hipMemcpyHostToDevice, P->stream));
       dn = P->noise.as<double>();
@@ -2123,25 +2678,161 @@ int pf_generate(void* plan, const double* noise, uint64_t seed, uint64_t first_r
       P->stage_out.ensure(outb);
       dout = P->stage_out.p;
     }
-    const bool dct = P->d.field_backend == PF_FIELD_DCTDST;
-    if (P->d.precision == PF_FP32) {
-      if (dct)
-        generate_dctdst<float>(*P, dn, seed, first_realization, batch, transform,
-                               transform_param, (float*)dout);
-      else
-        generate_r2c<float>(*P, dn, seed, first_realization, batch, transform, transform_param,
-                            (float*)dout);
-    } else {
-      if (dct)
-        generate_dctdst<double>(*P, dn, seed, first_realization, batch, transform,
-                                transform_param, (double*)dout);
-      else
-        generate_r2c<double>(*P, dn, seed, first_realization, batch, transform, transform_param,
-                             (double*)dout);
-    }
+    run_generate(*P, dn, seed, first_realization, batch, transform, transform_param, dout);
     if (out_mem == PF_MEM_HOST)
       HIPCHECK(hipMemcpyAsync(out, dout, outb, hipMemcpyDeviceToHost, P->stream));
     P->sync();
+    return PF_OK;
+  })
+}
+
+static void to_c(const Tuning& t, pf_tuning* o)
+{
+  o->tile_div = t.ldiv;
+  o->chunks = t.chunks;
+  o->overlap = t.overlap;
+  o->lines = t.lines;
+  o->radix_order = t.radix;
+  o->sub_batch = t.sub;
+}
+
+int pf_get_tuning(void* plan, pf_tuning* t)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    to_c(P->tune, t);
+    return PF_OK;
+  })
+}
+
+int pf_set_tuning(void* plan, const pf_tuning* t)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (t->tile_div < 0 || t->tile_div > 2 || t->chunks < 0 || t->chunks > 8 ||
+        t->overlap < -1 || t->overlap > 1 || t->lines < 0 || t->radix_order < 0 ||
+        t->radix_order > 2 || t->sub_batch < 0)
+      throw PfError(PF_EINVAL, "tuning value out of range");
+    Tuning x;
+    x.ldiv = t->tile_div;
+    x.chunks = t->chunks;
+    x.overlap = t->overlap;
+    x.lines = t->lines;
+    x.radix = t->radix_order;
+    x.sub = t->sub_batch;
+    P->tune = x;
+    return PF_OK;
+  })
+}
+
+// The autotuner: times pf_generate of `batch` device-Philox realizations
+// into `out` (device memory) under each candidate and keeps the fastest in
+// the plan. Candidates, tuned one group after the other: the tile width of
+// the y and x passes and the two-stream pipeline (off, or on with 2, 4, 8
+// chunks) for the R2C field, the realizations per sub-batch for the DCT/DST
+// and DFT fields, then the generic kernels' tile lines and radix order when
+// some pass runs on them. cache_path (optional): a text file of earlier
+// results, one line per device and field shape; a hit applies the stored
+// choice without timing, a miss appends the new result.
+int pf_autotune(void* plan, int32_t batch, uint64_t seed, int32_t transform, double param,
+                void* out, const char* cache_path, pf_tune_result* res)
+{
+  PF_GUARD({
+    Plan* P = static_cast<Plan*>(plan);
+    if (!P || !out || batch < 1)
+      throw PfError(PF_EINVAL, "pf_autotune: null plan or output, or batch < 1");
+    std::lock_guard<std::mutex> lk(P->mu);
+    ensure_eigs(*P);
+    const std::string key = tune_key(*P, batch);
+    pf_tune_result r{};
+    double cached_ms = 0.;
+    Tuning hit;
+    if (cache_path && tune_lookup(cache_path, key, hit, cached_ms)) {
+      P->tune = hit;
+      to_c(hit, &r.best);
+      r.ms = cached_ms;
+      r.from_cache = 1;
+      if (res)
+        *res = r;
+      return PF_OK;
+    }
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    auto time_it = [&](const Tuning& t) {
+      P->tune = t;
+      run_generate(*P, nullptr, seed, 0, batch, transform, param, out); // allocates, warms
+      float best = 1e30f;
+      for (int rep = 0; rep < 2; rep++) {
+        HIPCHECK(hipEventRecord(a, P->stream));
+        run_generate(*P, nullptr, seed, 0, batch, transform, param, out);
+        HIPCHECK(hipEventRecord(b, P->stream));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        best = std::min(best, ms);
+      }
+      r.candidates++;
+      return (double)best;
+    };
+    Tuning best;
+    double best_ms = time_it(best);
+    r.default_ms = best_ms;
+    auto consider = [&](const Tuning& t) {
+      const double ms = time_it(t);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    };
+    const int fb = P->d.field_backend;
+    if (fb == PF_FIELD_R2C) {
+      Tuning base = best;
+      for (int ldiv : { 1, 2 })
+        for (int ov : { 0, 1 })
+          for (int ch : { 2, 4, 8 }) {
+            if (ov == 0 && ch != 8)
+              continue;
+            if (ov == 1 && batch < 4)
+              continue;
+            Tuning t = base;
+            t.ldiv = ldiv;
+            t.overlap = ov;
+            t.chunks = ov ? ch : 0;
+            consider(t);
+          }
+    } else {
+      Tuning base = best;
+      for (int sub : { 1, 2, 4 })
+        if (sub < batch) {
+          Tuning t = base;
+          t.sub = sub;
+          consider(t);
+        }
+    }
+    if (!all_specialised(*P)) {
+      Tuning base = best;
+      for (int lines : { 0, 4, 2, 1 })
+        for (int radix : { 0, 1, 2 }) {
+          if (lines == 0 && radix == 0)
+            continue;
+          Tuning t = base;
+          t.lines = lines;
+          t.radix = radix;
+          consider(t);
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    P->tune = best;
+    to_c(best, &r.best);
+    r.ms = best_ms;
+    if (cache_path)
+      tune_store(cache_path, key, best, best_ms);
+    if (res)
+      *res = r;
     return PF_OK;
   })
 }
@@ -2227,7 +2918,12 @@ int pf_matvec(void* plan, int32_t op, const void* x, void* y, int32_t batch, int
       dx = P->stage_in.p;
       dy = P->stage_out.p;
     }
-    if (P->d.precision == PF_FP32)
+    if (g.dim == 1) {
+      if (P->d.precision == PF_FP32)
+        matvec_dft_1d<float>(*P, op, (const float*)dx, (float*)dy, batch);
+      else
+        matvec_dft_1d<double>(*P, op, (const double*)dx, (double*)dy, batch);
+    } else if (P->d.precision == PF_FP32)
       matvec_r2c<float>(*P, op, (const float*)dx, (float*)dy, batch);
     else
       matvec_r2c<double>(*P, op, (const double*)dx, (double*)dy, batch);
@@ -2576,6 +3272,9 @@ int pf_batch_create(const pf_desc* descs, int32_t n, void** batch)
       if (descs[i].periodization == PF_PERIOD_COFOLD)
         throw PfError(PF_EINVAL, "cofold fields are not supported in a batch (the cofold offset "
                                  "is reduced on the host between its two passes)");
+      if (descs[i].matrix_backend == PF_MATRIX_DFT || descs[i].field_backend == PF_FIELD_DFT)
+        throw PfError(PF_EINVAL, "DFT backends are not supported in a batch (a draw gives two "
+                                 "fields; generate them with pf_generate)");
       auto P = std::make_unique<Plan>();
       P->d = descs[i];
       P->g = make_geo(descs[i]);

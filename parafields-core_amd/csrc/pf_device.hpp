@@ -379,8 +379,10 @@ __device__ inline double cov_at(const CovParams& p, const double coord[3])
         v = 0.;
       else if (p.dim == 3)
         v = var * (1. - 1.5 * hh + 0.5 * pow(hh, 3.));
-      else
+      else if (p.dim == 2)
         v = var * (1. - 2. / 3.14159265358979323846 * (hh * sqrt(1. - pow(hh, 2.)) + asin(hh)));
+      else
+        v = var * (1. - hh);
       break;
     case PF_COV_EXPONENTIAL:
     case PF_COV_CUSTOM:

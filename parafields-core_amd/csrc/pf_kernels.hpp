@@ -77,6 +77,17 @@ __global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out
   });
 }
 
+// Covariance on the 1-D embedding grid (one long row: grid-stride over x).
+template <int COV, int PER>
+__global__ void k_eval_cov_1d(CovParams p, int64_t N0, double* out)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N0;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t idx[3] = { i, 0, 0 };
+    out[i] = embedded_cov<COV, PER>(p, idx);
+  }
+}
+
 // Cofold (matrix.hh:905-1013) on a dense grid a[n2][n1][n0] of classical
 // entries (the DCT half grid or the full R2C grid): max-norm of the wrapped
 // coordinates in units of the extensions, and whether the entry lies in the
@@ -391,6 +402,50 @@ __global__ void k_classify_r2c(cplx<double>* spec, int64_t count, int H, int N1,
     double v = *p;
     classify_one(v, thr, c);
     *p = v;
+  }
+  block_reduce_write(c, partial);
+}
+
+// DFT matrix eigenvalues of a 1-D embedding: the real parts of the forward
+// transform of kx < H, scaled by 1/N, classified and clamped like
+// fillTransformedMatrix (matrix.hh:266-291) over the full array: entry kx
+// stands for itself and, for 0 < kx < N/2, for its mirror N - kx.
+__global__ void kdft_finish_eigs_1d(const cplx<double>* X, int64_t N, int H, double inv_n,
+                                    double thr, double* eig, ClassifyPartial* partial)
+{
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < H;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    double v = X[k].x * inv_n;
+    const bool twice = k != 0 && 2 * k != N;
+    classify_one(v, thr, c);
+    if (twice) {
+      double w = X[k].x * inv_n;
+      classify_one(w, thr, c);
+    }
+    eig[k] = v;
+  }
+  block_reduce_write(c, partial);
+}
+
+// 2D/3D DFT matrix: classification and clamping of the real parts of the half
+// spectrum (natural layout [N2][N1][H]) with the multiplicity of each entry
+// in the full array (kx and N0 - kx for 0 < kx < N0/2).
+__global__ void kdft_classify(cplx<double>* spec, int64_t count, int H, int N0, double thr,
+                              ClassifyPartial* partial)
+{
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(i % H);
+    double v = spec[i].x;
+    const double v0 = v;
+    classify_one(v, thr, c);
+    if (kx != 0 && 2 * kx != N0) {
+      double w = v0;
+      classify_one(w, thr, c);
+    }
+    spec[i].x = v;
   }
   block_reduce_write(c, partial);
 }

@@ -30,8 +30,8 @@ COVARIANCES = {
     "whiteNoise": 12, "custom-iso": 13,
 }
 ANISOTROPY = {"none": 0, "axiparallel": 1, "geometric": 2}
-MATRIX_BACKENDS = {"dct": 0, "r2c": 1}
-FIELD_BACKENDS = {"r2c": 0, "dctdst": 1}
+MATRIX_BACKENDS = {"dct": 0, "r2c": 1, "dft": 2}
+FIELD_BACKENDS = {"r2c": 0, "dctdst": 1, "dft": 2}
 PRECISION = {"fp64": 0, "fp32": 1}
 PERIODIZATION = {"classical": 0, "merge": 1, "fold": 2, "cofold": 3}
 SIGMOID = {"smoothstep": 0, "smooth": 1}
@@ -116,6 +116,30 @@ class PfCgOpts(ctypes.Structure):
     ]
 
 
+class PfTuning(ctypes.Structure):
+    _fields_ = [
+        ("tile_div", ctypes.c_int32),
+        ("chunks", ctypes.c_int32),
+        ("overlap", ctypes.c_int32),
+        ("lines", ctypes.c_int32),
+        ("radix_order", ctypes.c_int32),
+        ("sub_batch", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PfTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("best", PfTuning),
+        ("ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+        ("candidates", ctypes.c_int32),
+        ("from_cache", ctypes.c_int32),
+    ]
+
+
 class PfCgStats(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int32),
@@ -128,7 +152,7 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
               covariance: str = "exponential", variance: float = 1.0,
               corr_length: Sequence[float] | float = 0.05, anisotropy: str = "none",
               factor: int = 2, periodic: bool = False, matrix: Optional[str] = None,
-              field: str = "r2c", transposed: int = -1, approximate: bool = False,
+              field: Optional[str] = None, transposed: int = -1, approximate: bool = False,
               threshold: float = 1e-14, precision: str = "fp64", cg_iterations: int = 100,
               nugget: float = 0.0, matern_nu: float = 1.0, exp_gamma: float = 1.0,
               cauchy_alpha: float = 1.0, cauchy_beta: float = 1.0,
@@ -138,7 +162,9 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
               sigmoid_end: float = 0.0, fold_max_factor: float = 0.0) -> PfDesc:
     """Descriptor with the reference's defaults. `matrix=None` picks what
     RandomField would (randomfield.hh:153-164): DCT (DefaultIsoMatrix) for
-    none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy.
+    none/axiparallel, R2C (DefaultAnisoMatrix) for geometric anisotropy, and
+    the DFT matrix and field for 1-D (matrix.hh:1660-1736); `field=None` is
+    the R2C field (DFT in 1-D).
     periodization / sigmoid / sigmoid_combine take the reference's names
     (embedding.periodization, embedding.sigmoid, stochastic.sigmoidCombine);
     merge_recursions (embedding.mergeRecursions, foldRecursions or
@@ -161,7 +187,10 @@ def make_desc(dim: int, cells: Sequence[int], extensions: Optional[Sequence[floa
     d.matern_nu, d.exp_gamma = matern_nu, exp_gamma
     d.cauchy_alpha, d.cauchy_beta = cauchy_alpha, cauchy_beta
     if matrix is None:
-        matrix = "r2c" if (anisotropy == "geometric") else "dct"
+        # 1-D: DFTMatrixBackend (matrix.hh:1660-1736)
+        matrix = "dft" if dim == 1 else ("r2c" if anisotropy == "geometric" else "dct")
+    if field is None:
+        field = "dft" if dim == 1 else "r2c"
     d.matrix_backend = MATRIX_BACKENDS[matrix]
     d.field_backend = FIELD_BACKENDS[field]
     d.transposed = transposed
@@ -239,6 +268,10 @@ def load(path: str = LIB_PATH):
                                     ctypes.POINTER(dp)]),
         "pf_last_timings": (i32, [vp, ctypes.c_char_p, i32, ctypes.POINTER(dp), i32,
                                   ctypes.POINTER(i32)]),
+        "pf_autotune": (i32, [vp, i32, u64, i32, dp, vp, ctypes.c_char_p,
+                              ctypes.POINTER(PfTuneResult)]),
+        "pf_set_tuning": (i32, [vp, ctypes.POINTER(PfTuning)]),
+        "pf_get_tuning": (i32, [vp, ctypes.POINTER(PfTuning)]),
     })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -336,7 +369,9 @@ class Plan:
         nz = None
         if noise is not None:
             nz = np.ascontiguousarray(noise, dtype=np.float64)
-            assert nz.size == batch * int(self.sizes.noise_per_field)
+            # DFT field: one draw of noise_per_field normals gives two fields
+            draws = (batch + 1) // 2 if self.desc.field_backend == FIELD_BACKENDS["dft"] else batch
+            assert nz.size == draws * int(self.sizes.noise_per_field)
         check(self.lib.pf_generate(self.h, None if nz is None else nz.ctypes.data, seed, first,
                                    batch, TRANSFORMS[transform], param, _ptr(out), _mem(out)))
         return out
@@ -365,6 +400,28 @@ class Plan:
                              ctypes.byref(st)))
         return x, {"iterations": st.iterations, "rel_residual": st.rel_residual,
                    "seconds": st.seconds}
+
+    def autotune(self, batch: int, out, seed: int = 0, transform: str = "none",
+                 param: float = 0.0, cache: Optional[str] = None) -> dict:
+        """Time the candidate launch choices of pf_generate(batch) into the
+        device tensor `out` and keep the fastest (pf_autotune)."""
+        r = PfTuneResult()
+        check(self.lib.pf_autotune(self.h, batch, seed, TRANSFORMS[transform], param, _ptr(out),
+                                   None if cache is None else cache.encode(), ctypes.byref(r)))
+        return {"best": r.best.as_dict(), "ms": r.ms, "default_ms": r.default_ms,
+                "candidates": r.candidates, "from_cache": bool(r.from_cache)}
+
+    def tuning(self) -> dict:
+        t = PfTuning()
+        check(self.lib.pf_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = PfTuning()
+        check(self.lib.pf_get_tuning(self.h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(self.lib.pf_set_tuning(self.h, ctypes.byref(t)))
 
     def value_transform(self, data, transform: str, param: float = 0.0):
         check(self.lib.pf_value_transform(self.h, TRANSFORMS[transform], param, _ptr(data),

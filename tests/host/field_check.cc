@@ -15,6 +15,7 @@
 
 using namespace parafields_amd;
 using Field = RandomField<GridTraits<double, double, 2>>;
+using Field1 = RandomField<GridTraits<double, double, 1>>; // DFT matrix and field
 
 static std::string dir;
 static int bad = 0;
@@ -134,6 +135,29 @@ int main(int argc, char** argv)
       FILE* fd = std::fopen((dir + "/dot.txt").c_str(), "w");
       std::fprintf(fd, "%.17g\n", dot);
       std::fclose(fd);
+    }
+    // 1-D: generate(seed) draws two fields; the next generate returns the
+    // spare field whatever its seed (matrix.hh:336, :411-420); the field
+    // file round trip in 1-D
+    {
+      ParameterTree c;
+      c["grid.cells"] = "300";
+      c["grid.extensions"] = "1";
+      c["stochastic.variance"] = "1";
+      c["stochastic.covariance"] = "exponential";
+      c["stochastic.corrLength"] = "0.05";
+      Field1 f(c);
+      f.generate(42u);
+      dump("d1_gen42", f.dataVector());
+      f.writeToFile(dir + "/field1");
+      Field1 g(dir + "/field1");
+      expect(g.dataVector() == f.dataVector(), "1-D field through the HDF5 file");
+      f.generate(43u);
+      dump("d1_gen43", f.dataVector());
+      f.generate(44u);
+      dump("d1_gen44", f.dataVector());
+      f.timesMatrix();
+      dump("d1_mult44", f.dataVector());
     }
   } catch (const std::exception& e) {
     std::printf("error: %s\n", e.what());

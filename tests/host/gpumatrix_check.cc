@@ -137,6 +137,11 @@ using Traits2f = parafields::RandomFieldTraits<parafields::GridTraits<double, fl
                                                GpuIsoMatrix, GpuAnisoMatrix>;
 
 static_assert(!Traits<2>::isoFieldIsReferenceR2C, "tag types are not the reference's backends");
+static_assert(Traits<1>::IsoMatrixType::MatrixBackendType::id == PF_MATRIX_DFT &&
+                Traits<1>::IsoMatrixType::FieldBackendType::id == PF_FIELD_DFT &&
+                Traits<2>::IsoMatrixType::MatrixBackendType::id == PF_MATRIX_DCT &&
+                Traits<3>::AnisoMatrixType::MatrixBackendType::id == PF_MATRIX_R2C,
+              "default backends per dimension (matrix.hh:1660-1736)");
 static_assert(!Traits<3>::anisoMatrixIsReferenceR2C, "tag types are not the reference's backends");
 static_assert(std::is_same<Traits<2>::IsoMatrixType::StochasticPartType,
                            parafields::StochasticPart<Traits<2>>>::value,
@@ -244,6 +249,23 @@ int main(int argc, char** argv)
       dump("dct_mult", (m * sp).values());
       dump("dct_root", m.multiplyRoot(sp).values());
       dump("dct_inv", m.multiplyInverse(sp).values());
+    }
+    // 1-D: the DFT matrix and field; one draw gives two fields, the second
+    // generateField returns the spare field without drawing
+    {
+      ParameterTree c = config("500", "1", "exponential", "none", "0.05");
+      auto t = std::make_shared<Traits<1>>(c);
+      GpuIsoMatrix<Traits<1>> m(t);
+      parafields::StochasticPart<Traits<1>> sp(t);
+      Rng<double> rng(11);
+      m.generateField(rng, sp);
+      dump("dft1_gen1", sp.values());
+      m.generateField(rng, sp);
+      dump("dft1_gen2", sp.values());
+      m.generateField(rng, sp); // a new draw from the same stream
+      dump("dft1_gen3", sp.values());
+      dump("dft1_mult", (m * sp).values());
+      dump("dft1_inv", m.multiplyInverse(sp).values());
     }
     // FP32 realizations with a custom covariance callable
     {

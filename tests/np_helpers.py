@@ -11,7 +11,7 @@ def textbook_cov(kind: str, h: np.ndarray, var: float = 1.0, dim: int = 2, nu: f
                  gamma: float = 1.0, alpha: float = 1.0, beta: float = 1.0) -> np.ndarray:
     """Covariance families as functions of the scaled distance h (standard
     textbook definitions: exponential, Gaussian, Matern(nu) with
-    K_nu from scipy, spherical (2D/3D), etc.)."""
+    K_nu from scipy, spherical (1D/2D/3D), etc.)."""
     h = np.asarray(h, dtype=np.float64)
     if kind == "exponential":
         return var * np.exp(-h)
@@ -29,8 +29,10 @@ def textbook_cov(kind: str, h: np.ndarray, var: float = 1.0, dim: int = 2, nu: f
         hc = np.minimum(h, 1.0)
         if dim == 3:
             v = var * (1 - 1.5 * hc + 0.5 * hc ** 3)
-        else:
+        elif dim == 2:
             v = var * (1 - 2 / np.pi * (hc * np.sqrt(1 - hc ** 2) + np.arcsin(hc)))
+        else:
+            v = var * (1 - hc)  # 1-D spherical (triangular) model
         return np.where(h > 1, 0.0, v)
     if kind == "cauchy":
         return var * (1 + h ** 2) ** -3

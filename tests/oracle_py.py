@@ -96,12 +96,15 @@ def eigs(desc, custom=None, allow_negative=False):
 
 
 def generate(desc, noise):
+    """One realization from `noise`; for the DFT field one draw: the field
+    and the spare field stacked (2, domain)."""
     s = sizes(desc)
     nz = np.ascontiguousarray(noise, dtype=np.float64)
     assert nz.size >= s.noise_per_field
-    out = np.empty(int(s.domain_size))
+    dft = desc.field_backend == pfb.FIELD_BACKENDS["dft"]
+    out = np.empty(int(s.domain_size) * (2 if dft else 1))
     _check(load().pfo_generate(ctypes.byref(desc), nz.ctypes.data, out.ctypes.data))
-    return out
+    return out.reshape(2, -1) if dft else out
 
 
 def matvec(desc, x, op="multiply"):

@@ -980,6 +980,16 @@ def test_gpumatrix_adapter_end_to_end(tmp_path):
     ga = orc.generate(da, orc.noise_cpp(5, int(orc.sizes(da).noise_per_field)))
     assert rel_max(rd("aniso_gen"), ga) < FP64_TOL
     assert rel_max(rd("aniso_mult"), orc.matvec(da, ga)) < FP64_TOL
+    d1 = desc(1, [500], covariance="exponential", corr_length=0.05, **base)
+    n1 = int(orc.sizes(d1).noise_per_field)
+    z1 = orc.noise_cpp(11, 2 * n1)
+    w1, w2 = orc.generate(d1, z1[:n1]), orc.generate(d1, z1[n1:])
+    assert rel_max(rd("dft1_gen1"), w1[0]) < FP64_TOL
+    assert rel_max(rd("dft1_gen2"), w1[1]) < FP64_TOL  # the spare field, no draw
+    assert rel_max(rd("dft1_gen3"), w2[0]) < FP64_TOL
+    assert rel_max(rd("dft1_mult"), orc.matvec(d1, w2[0])) < FP64_TOL
+    want_inv1, _ = orc.multiply_inverse(d1, w2[0])
+    assert np.linalg.norm(rd("dft1_inv") - want_inv1) / np.linalg.norm(want_inv1) < 1e-7
     dd = desc(3, [16, 12, 10], covariance="exponential", anisotropy="axiparallel",
               corr_length=[0.2, 0.1, 0.15], matrix="dct", field="dctdst", **base)
     nd = int(orc.sizes(dd).noise_per_field)
@@ -1082,3 +1092,12 @@ def test_cpp_field_files_caches_refinement(tmp_path):
     dot = float(open(out / "dot.txt").read())
     want_dot = x @ y + (z42[n:] * sd) @ (z7[n:] * sd)
     assert abs(dot - want_dot) <= 1e-10 * abs(want_dot)
+    # 1-D (DFT matrix and field): the second generate returns the spare field
+    d1 = desc(1, [300], covariance="exponential", corr_length=0.05)
+    n1 = int(orc.sizes(d1).noise_per_field)
+    w42 = orc.generate(d1, orc.noise_cpp(42, n1))
+    w44 = orc.generate(d1, orc.noise_cpp(44, n1))
+    assert rel_max(rd("d1_gen42"), w42[0]) < FP64_TOL
+    assert rel_max(rd("d1_gen43"), w42[1]) < FP64_TOL
+    assert rel_max(rd("d1_gen44"), w44[0]) < FP64_TOL
+    assert rel_max(rd("d1_mult44"), orc.matvec(d1, w44[0])) < FP64_TOL

@@ -4,6 +4,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -89,3 +90,60 @@ def test_field_files_round_trip(tmp_path):
     data = body[body.index("DATA {") + 6: body.rindex("}")]
     vals = [float(v) for v in data.replace("\n", " ").replace("}", " ").split(",") if v.strip()]
     assert vals == [i + 0.25 for i in range(12)]
+
+
+def _build_batch_asan_check(tmp_path):
+    """The library's host code (pf_api.hip, pf_fast_f32/f64.hip) compiled
+    host-only with AddressSanitizer and linked with the stand-in runtime of
+    tests/host/fake_hip.cc and the driver tests/host/batch_check.cc."""
+    import re
+    hipcc = "/opt/rocm/bin/hipcc"
+    clang = "/opt/rocm/llvm/bin/clang++"
+    csrc = os.path.join(ROOT, "parafields-core_amd", "csrc")
+    inc = os.path.join(ROOT, "include")
+    objs, procs = [], []
+    for src in ("pf_api.hip", "pf_fast_f32.hip", "pf_fast_f64.hip"):
+        obj = str(tmp_path / src.replace(".hip", ".o"))
+        objs.append(obj)
+        procs.append(subprocess.Popen(
+            [hipcc, "-x", "hip", "--offload-host-only", "-std=c++17", "-O1", "-g", "-fPIC",
+             "-Xarch_host", "-fsanitize=address", "-fno-omit-frame-pointer",
+             "-Wno-unused-command-line-argument", "-I", inc, "-c", os.path.join(csrc, src),
+             "-o", obj], stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+    for p in procs:
+        out = p.communicate()[0]
+        assert p.returncode == 0, out.decode()[-3000:]
+    # the host objects name the (absent) device code object: give it a definition
+    syms = set()
+    for obj in objs:
+        nm = subprocess.run(["nm", "-u", obj], capture_output=True, text=True).stdout
+        syms.update(re.findall(r"(__hip_fatbin\w*)", nm))
+    stub = tmp_path / "fatbin_stub.cc"
+    stub.write_text("".join(f'extern "C" {{ char {s}[64]; }}\n' for s in sorted(syms)))
+    exe = str(tmp_path / "batch_check")
+    subprocess.check_call(
+        [clang, "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
+         "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-I", inc,
+         os.path.join(ROOT, "tests", "host", "fake_hip.cc"),
+         os.path.join(ROOT, "tests", "host", "batch_check.cc"), str(stub), *objs, "-o", exe, "-lpthread"])
+    return exe
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/bin/hipcc"), reason="hipcc not available")
+def test_batch_graph_and_solver_bookkeeping_under_asan(tmp_path):
+    """No GPU: the 1000-field uneven batch of the benchmark (seed 3), one
+    autotuned batch per size class, then CG, DCT/DST and 1-D DFT plans, all
+    through the C ABI with AddressSanitizer on the library's host code and
+    the stand-in runtime checking streams, events, captures and graphs."""
+    import workloads as wl
+    exe = _build_batch_asan_check(tmp_path)
+    descs, shapes, _ = wl.uneven_batch(1000, seed=3)
+    n = len(descs)
+    (tmp_path / "descs.bin").write_bytes(b"".join(bytes(d) for d in descs))
+    cls = np.array([max(s) for s in shapes], dtype=np.int32)
+    (tmp_path / "classes.bin").write_bytes(cls.tobytes())
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([exe, str(tmp_path / "descs.bin"), str(n), str(tmp_path / "classes.bin")],
+                       capture_output=True, text=True, timeout=900, env=env)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0 and "all OK" in r.stdout, r.stdout[-3000:] + r.stderr[-6000:]

@@ -87,3 +87,39 @@ def test_dct_passes_host_build(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "all OK" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_dct_generic_passes_host_build(tmp_path):
+    """The generic (runtime-size) DCT/DST passes of pf_dctdst.hpp at the tile
+    the library gives them under an 80 KB LDS budget (8 lines of 1024, 73,792
+    bytes), the geometry of the two 1024^3 runs that faulted on the GPU: every
+    block of small shapes and sampled blocks of the 1024^3 shapes stay inside
+    their buffers."""
+    src = _host_sources(str(tmp_path))
+    exe = str(tmp_path / "dct_generic_check")
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-w", "-I", EMU, "-I", src,
+           os.path.join(EMU, "dct_generic_check.cc"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=900)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "all OK" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_dft_passes_host_build(tmp_path):
+    """The DFT field passes of pf_dft.hpp (1-D one tile and four-step, 2D and
+    3D in both storage orders, FP64 and FP32, odd first fields) and the 1-D
+    product passes, every block, against direct O(N^2) sums."""
+    src = _host_sources(str(tmp_path))
+    exe = str(tmp_path / "dft_pass_check")
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-w", "-I", EMU, "-I", src,
+           os.path.join(EMU, "dft_pass_check.cc"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "all OK" in r.stdout and r.stdout.count(" OK") >= 11

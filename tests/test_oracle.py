@@ -236,6 +236,8 @@ def linear_map(d, nsamp):
     (2, [4, 4], "gaussian", [0.2, 0.05, 0.0, 0.15], "geometric", "r2c"),
     (3, [2, 3, 2], "exponential", [0.3], "none", "dct"),
     (3, [2, 2, 2], "matern52", [0.3, 0.2, 0.1], "axiparallel", "r2c"),
+    (2, [4, 3], "gaussian", [0.2, 0.05, 0.0, 0.15], "geometric", "dft"),
+    (3, [2, 3, 2], "matern32", [0.3, 0.2, 0.25], "axiparallel", "dft"),
 ])
 def test_generate_r2c_covariance_identity(dim, cells, kind, corr, aniso, matrix):
     """L L^T = C - D, where D is the variance lost on the kx in {0, N0/2}
@@ -247,7 +249,7 @@ def test_generate_r2c_covariance_identity(dim, cells, kind, corr, aniso, matrix)
     L = linear_map(d, int(s.noise_per_field))
     C = nh.dense_cov(cells, [1.0] * dim, kind, aniso, corr)
     full = nh.full_eigs(cells, [1.0] * dim, kind, aniso, corr).real
-    full = np.maximum(full, 0.0) if matrix == "dct" else full
+    full = np.maximum(full, 0.0) if matrix in ("dct", "dft") else full
     N = [2 * c for c in cells]
     x = nh.cell_coords(cells, [1.0] * dim) / (np.array([1.0] * dim) / np.array(cells))  # int idx
     D = np.zeros_like(C)
@@ -456,6 +458,165 @@ def test_pcg_solves_dense_system(mode):
     assert rr <= 1e-10
     want = np.linalg.solve(C, b)
     assert np.linalg.norm(x - want) / np.linalg.norm(want) < 1e-7
+
+
+# --------------------------------------------------------- DFT backends --
+# DFTMatrixBackend (dftmatrixbackend.hh) and DFTFieldBackend
+# (dftfieldbackend.hh): the 1-D defaults, selectable in 2D/3D.
+
+def _dft_storage_k(s, N, tr):
+    """Frequency (k0, k1, k2) of storage index s: x fastest, the two last
+    axes swapped when transposed (transposeIfNeeded)."""
+    dim = len(N)
+    if dim == 1:
+        return (s,)
+    if dim == 2:
+        return (s // N[1], s % N[1]) if tr else (s % N[0], s // N[0])
+    if tr:
+        return (s % N[0], s // (N[0] * N[2]), (s // N[0]) % N[2])
+    return (s % N[0], (s // N[0]) % N[1], s // (N[0] * N[1]))
+
+
+def _dft_store(full, tr):
+    """numpy (k_{d-1}, ..., k0) array -> the reference storage order."""
+    return full.swapaxes(0, 1) if (tr and full.ndim > 1) else full
+
+
+DFT_CASES = [
+    (1, [12], "exponential", [0.2], "none", 0),
+    (1, [10], "spherical", [0.3], "geometric", 0),
+    (2, [5, 4], "gaussian", [0.1, 0.02, 0.0, 0.08], "geometric", 1),
+    (2, [4, 6], "exponential", [0.3], "none", 0),
+    (3, [3, 2, 4], "matern32", [0.2, 0.1, 0.15], "axiparallel", 1),
+    (3, [2, 3, 2], "exponential", [0.25], "none", 0),
+]
+
+
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,tr", DFT_CASES)
+def test_eigs_dft_vs_library(dim, cells, kind, corr, aniso, tr):
+    """Every entry of the grid: the real part of numpy's fftn of the wrapped
+    covariance / N_ext, in the reference storage order, classified and
+    clamped over all N_ext entries (matrix.hh:266-291)."""
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, matrix="dft",
+             field="dft", transposed=tr, approximate=True)
+    v, st, _ = orc.eigs(d, allow_negative=True)
+    s = orc.sizes(d)
+    assert s.eig_entries == s.extended_size == v.size
+    assert s.transposed == (tr and dim > 1)
+    full = nh.full_eigs(cells, [1.0] * dim, kind, aniso, corr).real
+    flat = np.ascontiguousarray(_dft_store(full, tr)).ravel()
+    thr = 1e-14
+    assert st["negative"] == int(np.sum(flat <= -thr))
+    assert st["zero"] == int(np.sum(np.abs(flat) < thr))
+    assert st["small"] == int(np.sum((flat >= thr) & (flat < 1e-6)))
+    np.testing.assert_allclose(v, np.maximum(flat, 0.0), atol=1e-13 * np.abs(flat).max())
+
+
+def _circulant_cov(cells, kind, aniso, corr):
+    """Covariance of the embedded model on the cells from numpy's spectrum
+    (clamped at 0): C[i, j] = sum_k lambda_k e^{2 pi i k (x_i - x_j) / N}."""
+    full = np.maximum(nh.full_eigs(cells, [1.0] * len(cells), kind, aniso, corr).real, 0.0)
+    c = np.fft.ifftn(full).real * full.size
+    x = np.stack(np.meshgrid(*[np.arange(n) for n in cells[::-1]], indexing="ij")[::-1],
+                 axis=-1).reshape(-1, len(cells))
+    N = np.array([2 * n for n in cells])
+    dx = (x[:, None, :] - x[None, :, :]) % N
+    return c[tuple(dx[..., i] for i in range(len(cells) - 1, -1, -1))]
+
+
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,tr", DFT_CASES)
+def test_generate_dft_covariance_identity(dim, cells, kind, corr, aniso, tr):
+    """Field and spare field are each distributed with the embedded covariance
+    and uncorrelated with each other: L_re L_re^T = L_im L_im^T = C,
+    L_re L_im^T = 0 (lambda (xi_1 + i xi_2) through the backward transform)."""
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, matrix="dft",
+             field="dft", transposed=tr, approximate=True)
+    n = int(orc.sizes(d).noise_per_field)
+    assert n == 2 * int(np.prod([2 * c for c in cells]))
+    cols = []
+    for j in range(n):
+        z = np.zeros(n)
+        z[j] = 1.0
+        cols.append(orc.generate(d, z))
+    Lre = np.stack([c[0] for c in cols], axis=1)
+    Lim = np.stack([c[1] for c in cols], axis=1)
+    C = _circulant_cov(cells, kind, aniso, corr)
+    np.testing.assert_allclose(Lre @ Lre.T, C, atol=1e-12)
+    np.testing.assert_allclose(Lim @ Lim.T, C, atol=1e-12)
+    np.testing.assert_allclose(Lre @ Lim.T, 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,cells,tr", [(1, [10], 0), (2, [4, 3], 0), (2, [4, 3], 1),
+                                          (3, [2, 3, 4], 0), (3, [2, 3, 4], 1)])
+def test_generate_dft_single_mode_noise_order(dim, cells, tr):
+    """A unit sample in slot 0 (1) of storage entry s gives the field
+    sqrt(l) cos(ph) (-sqrt(l) sin(ph)) and the spare field sqrt(l) sin(ph)
+    (sqrt(l) cos(ph)), ph = 2 pi k.x / N, with k the storage order of the
+    reference (transposeIfNeeded swaps the two last axes)."""
+    d = desc(dim, cells, covariance="exponential", corr_length=0.3, matrix="dft", field="dft",
+             transposed=tr)
+    N = [2 * c for c in cells]
+    n = int(orc.sizes(d).noise_per_field)
+    full = nh.full_eigs(cells, [1.0] * dim, "exponential", "none", [0.3]).real
+    x = np.stack(np.meshgrid(*[np.arange(c) for c in cells[::-1]], indexing="ij")[::-1],
+                 axis=-1).reshape(-1, dim)
+    rng = np.random.default_rng(8)
+    for s in sorted(set([0, 1, n // 2 - 1] + list(rng.integers(0, n // 2, 6)))):
+        k = _dft_storage_k(int(s), N, tr)
+        lam = full[tuple(k[::-1])]
+        ph = 2 * np.pi * sum(k[i] * x[:, i] / N[i] for i in range(dim))
+        for slot in (0, 1):
+            z = np.zeros(n)
+            z[2 * s + slot] = 1.0
+            f, spare = orc.generate(d, z)
+            a = np.sqrt(lam)
+            want_f = a * np.cos(ph) if slot == 0 else -a * np.sin(ph)
+            want_s = a * np.sin(ph) if slot == 0 else a * np.cos(ph)
+            np.testing.assert_allclose(f, want_f, atol=1e-13)
+            np.testing.assert_allclose(spare, want_s, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim,cells,kind,corr,aniso,tr", [c for c in DFT_CASES
+                                                          if c[2] != "gaussian"])
+def test_dft_matvec_is_dense_covariance_over_next(dim, cells, kind, corr, aniso, tr):
+    d = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso, matrix="dft",
+             field="dft", transposed=tr)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0] * dim, kind, aniso, corr)
+    x = np.random.default_rng(3).standard_normal(C.shape[0])
+    np.testing.assert_allclose(orc.matvec(d, x), C @ x / s.extended_size, atol=1e-14)
+    if dim > 1:
+        # the complex route of the DFT field equals the R2C route on the
+        # same (DFT) matrix
+        r = desc(dim, cells, covariance=kind, corr_length=corr, anisotropy=aniso,
+                 matrix="dft", field="r2c")
+        for op in ("multiply", "root", "inverse_embed"):
+            np.testing.assert_allclose(orc.matvec(d, x, op), orc.matvec(r, x, op),
+                                       rtol=1e-12, atol=1e-12)
+
+
+def test_dft_1d_root_and_inverse_when_periodic_and_cg():
+    d = desc(1, [64], covariance="exponential", corr_length=0.1, periodic=True)
+    assert d.matrix_backend == pfb.MATRIX_BACKENDS["dft"]
+    x = np.random.default_rng(4).standard_normal(64)
+    y = orc.matvec(d, x)
+    np.testing.assert_allclose(orc.matvec(d, orc.matvec(d, x, "root"), "root"), y, atol=1e-13)
+    np.testing.assert_allclose(orc.matvec(d, y, "inverse_embed"), x, atol=1e-10)
+    cells = [40]
+    d = desc(1, cells, covariance="exponential", corr_length=0.2)
+    s = orc.sizes(d)
+    C = nh.dense_cov(cells, [1.0], "exponential", "none", [0.2]) / s.extended_size
+    b = np.random.default_rng(5).standard_normal(40)
+    xs, it = orc.multiply_inverse(d, b)
+    want = np.linalg.solve(C, b)
+    assert np.linalg.norm(xs - want) / np.linalg.norm(want) < 1e-6
+
+
+def test_dft_backend_rules():
+    with pytest.raises(RuntimeError):
+        orc.sizes(desc(1, [8], matrix="dct", field="dft"))
+    with pytest.raises(RuntimeError):
+        orc.sizes(desc(1, [8], matrix="dft", field="r2c"))
 
 
 # ----------------------------------------------------- value transforms --
