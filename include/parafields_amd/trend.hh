@@ -7,7 +7,8 @@
 // :1148-1155, called from randomfield.hh:550-554), `generateUncorrelated`
 // from a fresh std::default_random_engine(seed) per component
 // (trend.hh:355-385), and the matrix products scale the shifts by the
-// variances (trend.hh:394-441). The image component (PNG input) is not built.
+// variances (trend.hh:394-441). The image component (trend.hh:679-836) scales
+// 1 - pixel / 255 of a PNG file (pngreader.hh) by its one coefficient.
 #pragma once
 
 #include <algorithm>
@@ -19,7 +20,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "parametertree.hh"
+#include "pngreader.hh"
 
 namespace parafields_amd {
 
@@ -27,13 +31,14 @@ template <typename RF, int dim>
 class TrendPart
 {
 public:
-  enum class Kind { Mean, Slope, Disk, Block };
+  enum class Kind { Mean, Slope, Disk, Block, Image };
 
   struct Component
   {
     Kind kind;
-    std::string name; // mean, slope, disk<k>, block<k>
+    std::string name; // mean, slope, disk<k>, block<k>, image
     std::vector<RF> mean, variance, shift;
+    std::shared_ptr<const PNGReader> image; // Kind::Image
 
     // value at location x (trend.hh:585-620)
     RF evaluate(const std::array<double, dim>& x, const std::array<double, dim>& ext) const
@@ -59,6 +64,9 @@ public:
             if (std::abs((RF)x[i] - c(i)) > 0.5 * c(dim + i))
               return 0.;
           return c(2 * dim);
+        case Kind::Image:
+          // ImageComponent::evaluate, trend.hh:798-806
+          return c(0) * (RF)image->evaluate(x, ext);
       }
       return 0.;
     }
@@ -87,8 +95,15 @@ public:
     for (int k = 0; add(config, file, "block" + std::to_string(k), Kind::Block, 2 * dim + 1);
          k++) {
     }
-    if (config.hasKey("image.mean"))
-      throw std::runtime_error("the image trend component (PNG input) is not built");
+    // the image component comes last (trend.hh:1015-1032); it needs 2D and
+    // one parameter (trend.hh:722-735)
+    if (config.hasKey("image.mean")) {
+      if (dim != 2)
+        throw std::runtime_error("image trend components require dim == 2");
+      add(config, file, "image", Kind::Image, 1);
+      comps_.back().image =
+        std::make_shared<const PNGReader>(config.get<std::string>("image.filename"));
+    }
   }
 
   // TrendComponent::writeToFile / TrendPart::writeToFile (trend.hh:656-675,
@@ -214,8 +229,10 @@ private:
     if (c.mean.size() != c.variance.size())
       throw std::runtime_error("trend component size does not match");
     if (c.mean.size() != n)
-      throw std::runtime_error("trend component " + name + " must contain " + std::to_string(n) +
-                               " parameters");
+      throw std::runtime_error(kind == Kind::Image
+                                 ? std::string("Image component must only contain one parameter")
+                                 : "trend component " + name + " must contain " +
+                                     std::to_string(n) + " parameters");
     c.shift.assign(n, RF(0));
     if (file) {
       const auto coef = file->get<std::vector<RF>>(name);

@@ -147,3 +147,63 @@ def test_batch_graph_and_solver_bookkeeping_under_asan(tmp_path):
                        capture_output=True, text=True, timeout=900, env=env)
     print(r.stdout[-3000:])
     assert r.returncode == 0 and "all OK" in r.stdout, r.stdout[-3000:] + r.stderr[-6000:]
+
+
+def _write_png(path, pixels, channels):
+    """8-bit PNG (grayscale or RGB) from rows of byte values, written with
+    zlib: signature, IHDR, one IDAT with filter byte 0 per row, IEND."""
+    import struct
+    import zlib
+
+    def chunk(kind, data):
+        return (struct.pack(">I", len(data)) + kind + data +
+                struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF))
+
+    h = len(pixels)
+    w = len(pixels[0]) // channels
+    color = {1: 0, 3: 2}[channels]
+    raw = b"".join(b"\x00" + bytes(row) for row in pixels)
+    png = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, color, 0, 0, 0))
+           + chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+    with open(path, "wb") as f:
+        f.write(png)
+
+
+def _png_link_args(tmp_path):
+    """libpng from /opt/conda through a directory holding only it and zlib
+    (conda's own libstdc++ stays off the search path)."""
+    d = tmp_path / "pnglib"
+    d.mkdir(exist_ok=True)
+    for name in ("libpng16.so.16", "libz.so.1"):
+        link = d / name
+        if not link.exists():
+            link.symlink_to(os.path.join("/opt/conda/lib", name))
+    return (["-DPARAFIELDS_AMD_HAVE_PNG", "-I", "/opt/conda/include"],
+            [str(d / "libpng16.so.16"), "-Wl,-rpath," + str(d)])
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/conda/include/png.h"), reason="libpng not present")
+def test_image_trend_component(tmp_path):
+    """Image trend component (trend.hh:679-836, pngreader.hh): at 48 points of
+    a 2 x 1 domain the value is 0.5 + 2 (1 - p / 255) with p the pixel at row
+    h - 1 - int(y / 1 * h), column int(x / 2 * w) (the red channel of an RGB
+    image); generate draws its shift last and the .trend file keeps it."""
+    rng = np.random.default_rng(4)
+    h, w = 5, 7
+    gray = rng.integers(0, 256, size=(h, w))
+    rgb = rng.integers(0, 256, size=(h, w, 3))
+    _write_png(str(tmp_path / "g.png"), [list(map(int, r)) for r in gray], 1)
+    _write_png(str(tmp_path / "c.png"), [list(map(int, r.ravel())) for r in rgb], 3)
+    inc, libs = _png_link_args(tmp_path)
+    exe = str(tmp_path / "image_check")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), *inc,
+                           os.path.join(ROOT, "tests", "host", "image_check.cc"), "-o", exe, *libs])
+    r = subprocess.run([exe, str(tmp_path / "g.png"), str(tmp_path / "c.png"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0 and "all OK" in r.stdout, r.stdout + r.stderr
+    v = np.loadtxt(str(tmp_path / "values.txt"))
+    for x, y, g, c in v:
+        row, col = h - 1 - int(y / 1.0 * h), int(x / 2.0 * w)
+        assert g == pytest.approx(0.5 + 2 * (1 - gray[row, col] / 255.), abs=1e-15)
+        assert c == pytest.approx(0.5 + 2 * (1 - rgb[row, col, 0] / 255.), abs=1e-15)
