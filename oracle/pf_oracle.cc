@@ -1261,6 +1261,21 @@ void generate_dctdst(const pf_desc& d, const Geo& g, const Eigs& e, const double
 // used by matvec op 3: bin of the folded position |k/(N/2)|/sqrt(dim).
 thread_local std::vector<double> g_spec_table;
 
+// Multiplier of an eigenvalue for the products: lambda (operator*), sqrt
+// (multiplyRoot), 1 / lambda (multiplyInverseExtended), or op 4: the PCG
+// embedding-inverse preconditioner 1 / max(lambda, g_inv_floor).
+double g_inv_floor = 0.;
+double op_factor(int op, double m)
+{
+  if (op == PF_OP_MULTIPLY)
+    return m;
+  if (op == PF_OP_ROOT)
+    return std::sqrt(m);
+  if (op == 4)
+    return 1. / std::max(m, g_inv_floor);
+  return 1. / m;
+}
+
 // Backward complex transform of the full grid (dftfieldbackend.hh:166-196)
 // followed by extendedFieldToField of component `comp` (:280-318): the first
 // n_i entries per axis of the real (0) or imaginary (1) part.
@@ -1323,7 +1338,7 @@ void matvec_dft(const pf_desc& d, const Geo& g, const Eigs& e, int op, const dou
     for (Index k1 = 0; k1 < N1; k1++)
       for (Index k0 = 0; k0 < N0; k0++) {
         const double m = eig_at(d, g, e, k0, k1, k2);
-        const double f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+        const double f = op_factor(op, m);
         spec[k0 + N0 * (k1 + N1 * k2)] *= f;
       }
   dft_backward_extract(g, spec, y, nullptr);
@@ -1365,7 +1380,7 @@ void matvec_r2c(const pf_desc& d, const Geo& g, const Eigs& e, int op, const dou
           f = g_spec_table[radial_bin(g, k0, k1, k2, (int)g_spec_table.size())];
         else {
           const double m = eig_at(d, g, e, k0, k1, k2);
-          f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+          f = op_factor(op, m);
         }
         spec[k0 + g.H * (k1 + g.N[1] * k2)] *= f;
       }
@@ -1482,7 +1497,7 @@ void matvec_dctdst(const pf_desc& d, const Geo& g, const Eigs& e, int op, const 
         const Index k0 = c % P[0], k1 = (c / P[0]) % P[1], k2 = c / (P[0] * P[1]);
         f = g_spec_table[radial_bin(g, k0, k1, k2, (int)g_spec_table.size())];
       } else
-        f = op == PF_OP_MULTIPLY ? m : op == PF_OP_ROOT ? std::sqrt(m) : 1. / m;
+        f = op_factor(op, m);
       a[c] = a[c] / (double)g.Next * f;
     }
     for (int ax = 0; ax < g.dim; ax++)
@@ -1972,11 +1987,14 @@ int pfo_pcg(const pf_desc* d, int mode, int maxit, double rel_tol, const double*
       return en->status;
     }
     const Index n = en->g.domain;
+    g_inv_floor = d->nugget / (double)en->g.Next;
     std::vector<double> bb(b, b + n), xx;
     std::function<void(const std::vector<double>&, std::vector<double>&)> prec =
       [&](const std::vector<double>& r, std::vector<double>& z) {
         z.resize(r.size());
-        matvec(*en, mode == PF_CG_LEARNED_PCG ? 3 : PF_OP_INVERSE_EMBED, r.data(), z.data());
+        // embedding inverse floored at nugget / N_ext when there is a nugget
+        const int op = mode == PF_CG_LEARNED_PCG ? 3 : d->nugget > 0. ? 4 : PF_OP_INVERSE_EMBED;
+        matvec(*en, op, r.data(), z.data());
       };
     int it = pcg(*en, bb, xx, mode == PF_CG_PLAIN ? nullptr : &prec, maxit, rel_tol, relres);
     std::copy(xx.begin(), xx.end(), x);

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -261,6 +262,8 @@ struct Plan
   DevBuf w1, w2, w3, noise, stage_in, stage_out, partial, scratch;
   DevBuf spec;       // learned radial multipliers exp(L_b) (FP64)
   int spec_nbins = 0;
+  DevBuf floor;      // PF_OP_INVERSE_FLOOR: the eigenvalue floor (one FP64)
+  double floor_value = 0.;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tw; // (precision bytes, n) -> table
   bool profiling = false;
   // set during pf_generate_noalloc: a lookup that would allocate (a missing
@@ -1619,7 +1622,8 @@ template <typename T>
 void matvec_dft_1d(Plan& P, int op, const T* x, T* y, int batch)
 {
   const Geo& g = P.g;
-  if (op != PF_OP_MULTIPLY && op != PF_OP_ROOT && op != PF_OP_INVERSE_EMBED)
+  if (op != PF_OP_MULTIPLY && op != PF_OP_ROOT && op != PF_OP_INVERSE_EMBED &&
+      op != PF_OP_INVERSE_FLOOR)
     throw PfError(PF_EINVAL, "operator not available for 1-D fields");
   const int64_t N = g.N[0];
   DftGeom dg = dft_geom(P, sizeof(cplx<T>));
@@ -1634,7 +1638,8 @@ void matvec_dft_1d(Plan& P, int op, const T* x, T* y, int batch)
   {
     Scope sc(&P, "dft_mult");
     kdft_mult_1d<T><<<grid1d(N * batch), 256, 0, P.stream>>>(W2, N, g.H, P.eig.as<double>(), op,
-                                                             1.0 / (double)N, batch);
+                                                             1.0 / (double)N, batch,
+                                                             P.floor_value);
     check_launch();
   }
   launch_cols<T>(P, "dft_inv_cols", true, W2, W1, dft_inverse_cols_1d(dg), batch);
@@ -1694,12 +1699,13 @@ void matvec_r2c(Plan& P, int op, const T* x, T* y, int batch)
     const int64_t nlines = dim == 3 ? (int64_t)g.H * g.N[1] : g.H;
     Scope sc(&P, "mv_fused_mult");
     const cplx<T>* tw = P.table<T>(Nlast);
+    const double* tab = op == PF_OP_INVERSE_FLOOR ? P.floor.as<double>() : P.spec.as<double>();
     if (!fast_fused<T>(fc, Nlast, C, gg, la, P.eig.as<double>(), op, 1.0 / (double)g.Next, tw,
-                       batch, P.spec.as<double>(), P.spec_nbins)) {
+                       batch, tab, P.spec_nbins)) {
       Sched s = psched(P, Nlast);
       Cfg c = pcfg<T>(P, Nlast, nlines, Nlast);
       k_fused_mult<T, ME><<<dim3(nblocks(nlines, c.L), batch), c.threads, c.lds, P.stream>>>(
-        C, gg, P.eig.as<double>(), op, 1.0 / (double)g.Next, s, tw, c.L, P.spec.as<double>(),
+        C, gg, P.eig.as<double>(), op, 1.0 / (double)g.Next, s, tw, c.L, tab,
         P.spec_nbins);
     }
     check_launch();
@@ -1878,9 +1884,19 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
     *relres = 0.;
     return 0;
   }
+  // embedding-inverse preconditioner: 1 / max(lambda, nugget / N_ext), so
+  // that eigenvalues the clamp set to zero do not make it infinite
+  const bool floored = mode == PF_CG_EMBED_PCG && P.d.nugget > 0.;
+  if (floored) {
+    P.floor_value = P.d.nugget / (double)P.g.Next;
+    P.floor.ensure(sizeof(double));
+    HIPCHECK(hipMemcpyAsync(P.floor.p, &P.floor_value, sizeof(double), hipMemcpyHostToDevice,
+                            P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+  }
   auto prec = [&](const double* rr, double* zz) {
     if (mode == PF_CG_EMBED_PCG)
-      matvec64(P, PF_OP_INVERSE_EMBED, rr, zz);
+      matvec64(P, floored ? PF_OP_INVERSE_FLOOR : PF_OP_INVERSE_EMBED, rr, zz);
     else if (mode == PF_CG_LEARNED_PCG)
       matvec64(P, PF_OP_SPECTRAL, rr, zz);
   };
@@ -1915,6 +1931,11 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
     HIPCHECK(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, P.stream));
     HIPCHECK(hipStreamSynchronize(P.stream));
     it = h.it;
+    if (!std::isfinite(h.rr) || !std::isfinite(h.rz)) {
+      // the recurrence broke down (an infinite or undefined preconditioner)
+      rr = std::numeric_limits<double>::quiet_NaN();
+      break;
+    }
     if (h.done || it >= maxit || since_refresh >= kRefresh) {
       const double rr2 = true_residual(); // r = b - A x
       rr = std::sqrt(rr2 / bb);

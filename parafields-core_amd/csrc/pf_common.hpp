@@ -71,6 +71,10 @@ __device__ __forceinline__ int64_t eig_index(const GenGeom& g, int kx, int k1, i
 // Operator index of the learned spectral preconditioner in the fused matvec
 // pass (after PF_OP_MULTIPLY / PF_OP_ROOT / PF_OP_INVERSE_EMBED).
 constexpr int PF_OP_SPECTRAL = 3;
+// Inverse of the embedding with the eigenvalues floored at table[0]: the
+// embedding-inverse preconditioner of (P)CG, 1 / max(lambda, nugget / N_ext),
+// defined where the clamp left zero eigenvalues.
+constexpr int PF_OP_INVERSE_FLOOR = 4;
 
 // Radial bin of a (folded) spectral position: r = |k / (N/2)| / sqrt(dim).
 __device__ __forceinline__ int radial_bin(int a, int b, int c, int h0, int h1, int h2, int dim,

@@ -284,18 +284,19 @@ __global__ void kdft_pad(const TI* in, cplx<T>* out, int64_t n, int64_t N, int b
 }
 
 // 1-D product step in natural frequency order: X[k] *= f(lambda_k) / N with
-// f = lambda (multiply), sqrt(lambda) (root) or 1 / lambda (inverse of the
-// embedding); lambda from the FP64 half table (k >= H reads N - k).
+// f = lambda (multiply), sqrt(lambda) (root), 1 / lambda (inverse of the
+// embedding) or 1 / max(lambda, floor) (op 4, the floored inverse of the
+// preconditioner); lambda from the FP64 half table (k >= H reads N - k).
 template <typename T>
 __global__ void kdft_mult_1d(cplx<T>* X, int64_t N, int H, const double* eig, int op,
-                             double scale, int batch)
+                             double scale, int batch, double floor)
 {
   const int64_t tot = N * batch;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t k = i % N;
     const double lam = eig[k < H ? k : N - k];
-    double f = op == 0 ? lam : op == 1 ? sqrt(lam) : 1. / lam;
+    double f = op == 0 ? lam : op == 1 ? sqrt(lam) : op == 4 ? 1. / fmax(lam, floor) : 1. / lam;
     f *= scale;
     X[i] = { (T)((double)X[i].x * f), (T)((double)X[i].y * f) };
   }

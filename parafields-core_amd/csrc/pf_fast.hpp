@@ -823,7 +823,10 @@ __global__ void __launch_bounds__(NT) kf_fused_mult(cplx<T>* C, GenGeom g, LastA
         f = spectral_factor(g, kx, k1, k2, spec, nbins);
       else {
         const double m = eig[eig_index(g, kx, k1, k2)];
-        f = op == PF_OP_MULTIPLY ? m : (op == PF_OP_ROOT ? sqrt(m) : 1. / m);
+        f = op == PF_OP_MULTIPLY ? m
+            : op == PF_OP_ROOT   ? sqrt(m)
+            : op == PF_OP_INVERSE_FLOOR ? 1. / fmax(m, spec[0])
+                                 : 1. / m;
       }
       f *= inv_next;
       buf[e * L + l] = cscale(buf[e * L + l], (T)f);

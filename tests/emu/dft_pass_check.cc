@@ -232,7 +232,7 @@ static void matvec_case(const char* tag, int n0, int cap, int nb, double tol) {
   cols<T>("kdft_cols (forward)", false, W2, W1, p, nb);
   RowPass r{}; r.R = g.A; r.keep = g.B; r.in_rs = g.B; r.in_bs = N; r.out_rs = 1; r.out_es = g.A; r.out_bs = N; r.limit = N;
   rows<T, false, 0>("kdft_rows (forward)", W1, W2, nullptr, r, g.B, nb, 0, 0);
-  launch("kdft_mult_1d", dim3(4, 1), 64, [&] { kdft_mult_1d<T>(W2, N, H, eig, 0, 1.0 / N, nb); });
+  launch("kdft_mult_1d", dim3(4, 1), 64, [&] { kdft_mult_1d<T>(W2, N, H, eig, 0, 1.0 / N, nb, 0.); });
   ColPass q{}; q.C = g.A; q.E = g.B; q.O = 1; q.keep = g.B; q.in_es = g.A; q.in_bs = N; q.out_es = g.A; q.out_bs = N; q.twN = g.B > 1 ? N : 0;
   cols<T>("kdft_cols (inverse)", true, W2, W1, q, nb);
   RowPass s{}; s.R = g.B; s.keep = g.A; s.in_rs = g.A; s.in_bs = N; s.out_rs = 1; s.out_es = g.B; s.out_bs = n0; s.limit = n0;
