@@ -13,6 +13,12 @@ roofline of the dominant kernel against 8 TB/s and against the VALU issue
 peak (its launch time without the two-stream overlap), the FP64 per-realization GPU
 path on the same grid, and the FP64 CPU oracle on a bounded sample (one
 realization) as the CPU baseline.
+
+The other configurations run after the headline workload, each in a child
+process (`--leg NAME`), and land under `other_configs`: the 128^3 covariance
+inverse (plain CG against embedding, spectral and learned PCG), the 1024^3
+DCT/DST field (8 realizations, sign transform) and the uneven batch of 1000
+2D fields (one captured graph). `--no-legs` skips them.
 """
 from __future__ import annotations
 
@@ -85,11 +91,12 @@ def measured_traffic(kernel, fields_per_launch):
     return int((rec["fetch"] + rec["write"]) * fields_per_launch)
 
 
-# VALU issue peak of MI355X without packed FP32 math (which this build keeps
-# off): one wave64 VALU instruction per CU per cycle, 256 CUs at 2.4 GHz
-# (= 78.6 TFLOP/s of FP32 FMA; the 157.3 TFLOP/s vector figure counts packed
-# FP32 instructions).
-VALU_PEAK = 256 * 2.4e9
+# VALU issue peak of MI355X: gfx950 issues wave64 FP32 add / mul / FMA
+# instructions (not packed) at two per SIMD every 4 cycles (1.80 measured for
+# independent FMA chains): 256 CUs x 4 SIMDs x 2 / 4 cycles x 2.4 GHz. The
+# transcendentals (v_log, v_sin, v_cos, v_sqrt) take about 4 and the 64-bit
+# integer multiply-add of Philox about 3 of those half-slots.
+VALU_PEAK = 256 * 4 * 2 / 4 * 2.4e9
 
 
 def valu_per_field(kernel):
@@ -273,28 +280,24 @@ def run_cpu_baseline(r0, gpu_fields=None, reps=3, single_thread=True):
                       f"setup ({eig_s:.1f} s) excluded"}
 
 
-def run_uneven(nfields=1000, seed=3, reps=3):
+def run_uneven(nfields=1000, seed=3, reps=3, branches=16):
     """Config 3: nfields 2D fields, sides log-uniform on {64..1024} (2^k and
     3*2^k), Gaussian covariance with full geometric anisotropy, R2C matrix
     with per-field FP64 eigenvalues, FP32 realizations, whole batch captured
-    in one hipGraph and replayed."""
+    in one hipGraph on `branches` forked streams and replayed (one capture;
+    the branch-count autotuner is not run here)."""
     import torch
     sides = wl.UNEVEN_SIDES
     descs, shapes, falg = wl.uneven_batch(nfields, seed)
     alg = sum(falg)
     t0 = time.perf_counter()
     b = pfb.Batch(descs)
+    b.set_branches(branches)
     out = torch.empty(int(b.offsets[-1]), dtype=torch.float32, device="cuda")
     b.generate(seed=11, out=out, use_graph=True)   # eager preparation + capture + first launch
     setup = time.perf_counter() - t0
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        b.generate(seed=11, out=out, use_graph=True)
-    serial = (time.perf_counter() - t0) / reps
-    best, tuned_ms = b.autotune(seed=11, out=out)      # one replay per branch count
-    b.generate(seed=11, out=out, use_graph=True)       # capture for the chosen count
-    torch.cuda.synchronize()
+    print("bench: uneven batch captured", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(reps):
         b.generate(seed=11, out=out, use_graph=True)
@@ -316,12 +319,11 @@ def run_uneven(nfields=1000, seed=3, reps=3):
     b.close()
     del out
     torch.cuda.empty_cache()
-    torch.cuda.empty_cache()
     return {"fields": nfields, "fields_per_s": round(nfields / el, 1),
             "ms_per_batch": round(el * 1e3, 2), "eager_ms_per_batch": round(eager * 1e3, 2),
-            "serial_graph_ms_per_batch": round(serial * 1e3, 2), "branches": best,
-            "alg_GBps": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / HBM_PEAK, 4),
-            "setup_s": round(setup, 2), "per_size_class": per_size,
+            "branches": branches, "alg_GBps": round(alg / el / 1e9, 1),
+            "frac_of_hbm": round(alg / el / HBM_PEAK, 4), "setup_s": round(setup, 2),
+            "per_size_class": per_size,
             "note": "per-field FP64 eigenvalues + one FP32 realization each, one graph launch; "
                     "per_size_class: fields keyed by their larger side, device time of each "
                     "field in one eager serial pass (HIP events around every field)"}
@@ -421,6 +423,29 @@ def run_dctdst(batch=8, steps=1):
             "eig_negative": st["negative"]}
 
 
+LEGS = {"uneven_batch_2d": run_uneven, "cg_vs_learned_pcg": run_cg, "dctdst_1024": run_dctdst}
+
+
+def run_leg_child(name, timeout_s):
+    """One leg of another configuration in a child process (`bench.py --leg
+    name`), so that whatever happens to it leaves the headline line intact;
+    the child's last stdout line is its JSON result."""
+    import subprocess
+    print(f"bench: {name} leg (child process)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--leg", name],
+                           stdout=subprocess.PIPE, timeout=timeout_s, text=True)
+    except subprocess.TimeoutExpired:
+        return {"error": f"leg did not finish in {timeout_s} s"}
+    lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"leg exited with status {r.returncode}"}
+    res = json.loads(lines[-1])
+    res["leg_wall_s"] = round(time.perf_counter() - t0, 1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,16 +459,17 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a 1 GiB device-to-device copy of known size (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
-    ap.add_argument("--extra", action="store_true",
-                    help="also run the CG leg")
-    ap.add_argument("--uneven", action="store_true",
-                    help="also run the 1000-field uneven-batch leg (off by default: one run of "
-                         "this batch's autotuner ended in a host segmentation fault whose cause "
-                         "has not been shown)")
-    ap.add_argument("--config5", action="store_true",
-                    help="also run the 1024^3 DCT/DST leg (about a minute: the 2048^3 embedding's "
-                         "eigenvalues and 8 realizations)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the other configurations (uneven batch, CG, 1024^3 DCT/DST), "
+                         "which otherwise run after the headline workload, each in a child "
+                         "process")
+    ap.add_argument("--leg", choices=sorted(LEGS),
+                    help="run one other configuration in this process and print its JSON "
+                         "result (what the child processes of the default run do)")
     a = ap.parse_args()
+    if a.leg:
+        print(json.dumps(LEGS[a.leg]()), flush=True)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine holds the whole extended grid on one "
               "device and has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -493,20 +519,12 @@ def main():
     r0 = stats.pop("realization")
     fp64_fps, field64 = (None, None) if a.no_fp64 else run_gpu_fp64_single(3, r0)
     extra = {}
-    legs = []
-    if a.uneven:
-        legs += [("uneven_batch_2d", run_uneven)]
-    if a.extra:
-        legs += [("cg_vs_learned_pcg", run_cg)]
-    if a.config5:
-        legs += [("dctdst_1024", run_dctdst)]
-    if legs:
-        for name, fn in legs:
-            try:
-                print(f"bench: {name} leg", file=sys.stderr, flush=True)
-                extra[name] = fn()
-            except Exception as e:  # a failing leg is reported, not hidden
-                extra[name] = {"error": repr(e)}
+    if not a.no_legs:
+        import torch
+        torch.cuda.empty_cache()  # the child processes take the device memory
+        for name, limit in (("cg_vs_learned_pcg", 420), ("dctdst_1024", 420),
+                            ("uneven_batch_2d", 300)):
+            extra[name] = run_leg_child(name, limit)
     cpu = None if a.no_cpu else run_cpu_baseline(r0, {"fp32_batched": field32,
                                                       "fp64_per_realization": field64},
                                                  single_thread=not a.no_cpu_single)
