@@ -135,6 +135,11 @@ int main(int argc, char** argv) {
   bad += run_case<16>("3D transposed r2c N=16", 3, 1, 0, 32, 16, 8, 2, 1 << 30);
   bad += run_case<32>("3D x-fastest r2c nlast=N", 3, 0, 0, 32, 32, 32, 2, 1 << 30);
   bad += run_case<64>("2D r2c nlast=N", 2, 0, 0, 64, 1, 64, 2, 1 << 30);
+  // the 512-point tile of the 256^3 / 512^3 ensemble (16 lines, 512 threads)
+  // on small x and middle extents
+  bad += run_case<512>("3D transposed dct N=512", 3, 1, 1, 32, 16, 256, 2, 1 << 30);
+  bad += run_case<512>("3D x-fastest r2c N=512", 3, 0, 0, 32, 16, 256, 1, 1 << 30);
+  bad += run_case<512>("2D r2c N=512", 2, 0, 0, 64, 1, 256, 2, 1 << 30);
   printf(bad ? "FAILURES %d\n" : "all OK\n", bad);
   return bad;
 }
