@@ -1872,11 +1872,13 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
   double* Ax = vAx.p(n);
   const unsigned gb = std::min<unsigned>(1024, grid1d(n));
   DevBuf dsc, dpart;
-  dsc.ensure(sizeof(CgScalars) + sizeof(double));
-  dpart.ensure(1024 * sizeof(double));
+  dsc.ensure(sizeof(CgScalars));
+  dpart.ensure(3 * 1024 * sizeof(double));
   CgScalars* sc = dsc.as<CgScalars>();
-  double* rz_new = reinterpret_cast<double*>(sc + 1);
-  double* part = dpart.as<double>();
+  // block partials of p . Ap, r . r and r . z
+  double* pap_part = dpart.as<double>();
+  double* rr_part = pap_part + 1024;
+  double* rz_part = rr_part + 1024;
   HIPCHECK(hipMemsetAsync(x, 0, n * sizeof(double), P.stream));
   HIPCHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
   const double bb = dot(P, b, b, n);
@@ -1910,6 +1912,9 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
     check_launch();
     return dot(P, r, r, n);
   };
+  // iteration i (the device's count) reads r . z from slot i % 2 and its p
+  // update writes the next value to the other slot (CgScalars::rzs)
+  int par = 0;
   {
     // start: z = M^{-1} r, p = z, rz = r . z
     if (mode != PF_CG_PLAIN)
@@ -1917,7 +1922,7 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
     HIPCHECK(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
     CgScalars h{};
     h.rr = bb;
-    h.rz = mode == PF_CG_PLAIN ? bb : dot(P, r, z, n);
+    h.rzs[par] = mode == PF_CG_PLAIN ? bb : dot(P, r, z, n);
     h.bb = bb;
     h.tol2 = rel_tol * rel_tol * bb;
     h.done = h.rr <= h.tol2 ? 1 : 0;
@@ -1931,7 +1936,10 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
     HIPCHECK(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, P.stream));
     HIPCHECK(hipStreamSynchronize(P.stream));
     it = h.it;
-    if (!std::isfinite(h.rr) || !std::isfinite(h.rz)) {
+    // the slot of the next iteration; an iteration that set done did not
+    // write it, and its own r . z is in the other slot
+    par = it & 1;
+    if (!std::isfinite(h.rr) || !std::isfinite(h.rzs[h.done ? par ^ 1 : par])) {
       // the recurrence broke down (an infinite or undefined preconditioner)
       rr = std::numeric_limits<double>::quiet_NaN();
       break;
@@ -1949,39 +1957,33 @@ int pcg(Plan& P, const double* b, double* x, int64_t n, int mode, int maxit, dou
       if (h.done) {
         // the iteration that set `done` stopped before its direction update:
         // finish it from the true residual, p = z + (r . z / rz) p
-        h.done = 0;
-        put(h);
-        HIPCHECK(hipMemcpyAsync(rz_new, &rzn, sizeof rzn, hipMemcpyHostToDevice, P.stream));
-        HIPCHECK(hipStreamSynchronize(P.stream));
-        k_cg_update_p<<<gb, 256, 0, P.stream>>>(z, p, n, rz_new, sc);
-        k_cg_commit_rz<<<1, 64, 0, P.stream>>>(sc, rz_new);
+        k_xpby<<<gb, 256, 0, P.stream>>>(z, rzn / h.rzs[par ^ 1], p, n);
         check_launch();
-      } else {
-        // periodic replacement inside the run: the next step size uses the
-        // replaced residual's r . z
-        h.rz = rzn;
-        put(h);
+        h.done = 0;
       }
+      // the next step size uses the replaced residual's r . z
+      h.rzs[par] = rzn;
+      put(h);
       since_refresh = 0;
       continue;
     }
     const int steps = std::min(kCheck, maxit - it);
     for (int k = 0; k < steps; k++) {
+      // once an iteration sets done, the rest of this round of kernels skips
+      // (all test sc->done) and the next round starts from the device's count
       matvec64(P, PF_OP_MULTIPLY, p, Ap);
-      k_cg_dot<<<gb, 256, 0, P.stream>>>(p, Ap, n, part, sc);
-      k_cg_finish_pap<<<1, 256, 0, P.stream>>>(part, (int)gb, sc);
-      k_cg_update_xr<<<gb, 256, 0, P.stream>>>(p, Ap, x, r, n, part, sc);
-      k_cg_finish_rr<<<1, 256, 0, P.stream>>>(part, (int)gb, sc);
-      if (mode == PF_CG_PLAIN) {
-        k_cg_finish_rz<<<1, 256, 0, P.stream>>>(nullptr, 0, sc, rz_new);
-      } else {
+      k_cg_dot<<<gb, 256, 0, P.stream>>>(p, Ap, n, pap_part, sc);
+      k_cg_update_xr<<<gb, 256, 0, P.stream>>>(p, Ap, x, r, n, pap_part, (int)gb, rr_part, sc,
+                                               par);
+      if (mode != PF_CG_PLAIN) {
         prec(r, z);
-        k_cg_dot<<<gb, 256, 0, P.stream>>>(r, z, n, part, sc);
-        k_cg_finish_rz<<<1, 256, 0, P.stream>>>(part, (int)gb, sc, rz_new);
+        k_cg_dot<<<gb, 256, 0, P.stream>>>(r, z, n, rz_part, sc);
       }
-      k_cg_update_p<<<gb, 256, 0, P.stream>>>(z, p, n, rz_new, sc);
-      k_cg_commit_rz<<<1, 64, 0, P.stream>>>(sc, rz_new);
+      k_cg_update_p<<<gb, 256, 0, P.stream>>>(z, p, n, rr_part,
+                                              mode == PF_CG_PLAIN ? nullptr : rz_part, (int)gb,
+                                              sc, par);
       check_launch();
+      par ^= 1;
     }
     since_refresh += steps;
   }

@@ -768,8 +768,15 @@ __global__ void __launch_bounds__(NT) kf_r2c_x(const TI* in, cplx<T>* out, int64
 }
 
 // Fused last-axis pass of the matvec: forward DFT of n_last leading points,
-// times f(k)/N_ext, inverse DFT, first n_last points back in place.
-template <typename T, int N, int L, int NT>
+// times f(k)/N_ext, inverse DFT, first n_last points back in place. The
+// forward transform's first stage reads the tile's lines straight from C
+// (rows e >= n_last are zero) and the inverse transform's last stage writes
+// rows j < n_last straight back (HALF: n_last <= N/2, so that stage computes
+// the lower half of its outputs only); the tile passes through LDS in
+// between. Element (e, l) of the tile is at e*plane + l*lstep from a
+// block-uniform base (32-bit offsets: the launcher checks that a
+// realization's span stays below 4 GiB).
+template <typename T, int N, int L, int NT, bool HALF>
 __global__ void __launch_bounds__(NT) kf_fused_mult(cplx<T>* C, GenGeom g, LastAxis a,
                                                     const double* eig, int op, double inv_next,
                                                     const cplx<T>* tw, const double* spec,
@@ -790,59 +797,61 @@ __global__ void __launch_bounds__(NT) kf_fused_mult(cplx<T>* C, GenGeom g, LastA
       return;
   }
   const int64_t plane = (int64_t)a.Hp * a.Nmid;
-  C += (int64_t)blockIdx.z * plane * a.nlast;
+  cplx<T>* cb = C + (int64_t)blockIdx.z * plane * a.nlast + (int64_t)km0 * a.Hp + kx0;
+  constexpr uint32_t CB = sizeof(cplx<T>);
+  const uint32_t pb = (uint32_t)plane * CB, lb = nyq ? (uint32_t)a.Hp * CB : CB;
+  const int lmax = nyq ? a.Nmid - km0 : L; // live columns of the tile
+  char* cbb = reinterpret_cast<char*>(cb);
+  auto load = [&](int e, int l) -> cplx<T> {
+    cplx<T> v = { 0, 0 };
+    if (e < a.nlast && l < lmax)
+      v = *reinterpret_cast<const cplx<T>*>(cbb + ((uint32_t)e * pb + (uint32_t)l * lb));
+    return v;
+  };
+  auto store = [&](int j, int l, const cplx<T>& v) {
+    if (j < a.nlast && l < lmax)
+      *reinterpret_cast<cplx<T>*>(cbb + ((uint32_t)j * pb + (uint32_t)l * lb)) = v;
+  };
+  tfft_io<T, N, L, NT, 0, false, 1>(buf, tw, (int)threadIdx.x, load, TileLds{});
   using TL = TileLoop<L * N, NT>;
   {
-    cplx<T> v[TL::IT];
+    // the eigenvalues (or bin multipliers) of all entries a thread scales are
+    // fetched before the first product, with a compile-time trip count, so
+    // their loads are in flight together instead of one latency per entry
+    double m[TL::IT];
 #pragma unroll
     for (int it = 0; it < TL::IT; it++) {
       const int idx = TL::idx(it);
       const int l = idx % L, e = idx / L;
       const int kx = nyq ? kx0 : kx0 + l;
       const int km = nyq ? km0 + l : km0;
-      v[it] = { 0, 0 };
-      if (TL::live(it) && e < a.nlast && km < a.Nmid)
-        v[it] = C[e * plane + (int64_t)km * a.Hp + kx];
-    }
-#pragma unroll
-    for (int it = 0; it < TL::IT; it++)
-      if (TL::live(it))
-        buf[TL::idx(it)] = v[it];
-  }
-  __syncthreads();
-  tfft<T, N, L, NT, 0, false, 1>(buf, tw);
-  for (int idx = threadIdx.x; idx < L * N; idx += NT) {
-    const int l = idx % L, e = idx / L;
-    const int kx = nyq ? kx0 : kx0 + l;
-    const int km = nyq ? km0 + l : km0;
-    if (km < a.Nmid) {
-      int k1, k2;
-      line_k(g, kx, km, e, k1, k2);
-      double f;
-      if (op == PF_OP_SPECTRAL)
-        f = spectral_factor(g, kx, k1, k2, spec, nbins);
-      else {
-        const double m = eig[eig_index(g, kx, k1, k2)];
-        f = op == PF_OP_MULTIPLY ? m
-            : op == PF_OP_ROOT   ? sqrt(m)
-            : op == PF_OP_INVERSE_FLOOR ? 1. / fmax(m, spec[0])
-                                 : 1. / m;
+      m[it] = 0.;
+      if (TL::live(it) && km < a.Nmid) {
+        int k1, k2;
+        line_k(g, kx, km, e, k1, k2);
+        m[it] = op == PF_OP_SPECTRAL ? spectral_factor(g, kx, k1, k2, spec, nbins)
+                                     : eig[eig_index(g, kx, k1, k2)];
       }
-      f *= inv_next;
-      buf[e * L + l] = cscale(buf[e * L + l], (T)f);
+    }
+#pragma unroll
+    for (int it = 0; it < TL::IT; it++) {
+      const int idx = TL::idx(it);
+      const int l = idx % L;
+      const int km = nyq ? km0 + l : km0;
+      if (TL::live(it) && km < a.Nmid) {
+        double f = m[it];
+        if (op != PF_OP_SPECTRAL)
+          f = op == PF_OP_MULTIPLY ? f
+              : op == PF_OP_ROOT   ? sqrt(f)
+              : op == PF_OP_INVERSE_FLOOR ? 1. / fmax(f, spec[0])
+                                   : 1. / f;
+        f *= inv_next;
+        buf[idx] = cscale(buf[idx], (T)f);
+      }
     }
   }
   __syncthreads();
-  tfft<T, N, L, NT, 0, true, 1>(buf, tw);
-#pragma unroll
-  for (int it = 0; it < TL::IT; it++) {
-    const int idx = TL::idx(it);
-    const int l = idx % L, j = idx / L;
-    const int kx = nyq ? kx0 : kx0 + l;
-    const int km = nyq ? km0 + l : km0;
-    if (TL::live(it) && j < a.nlast && km < a.Nmid)
-      C[j * plane + (int64_t)km * a.Hp + kx] = buf[idx];
-  }
+  tfft_io<T, N, L, NT, 0, true, 1, HALF>(buf, tw, (int)threadIdx.x, TileLds{}, store);
 }
 
 } // namespace pfamd

@@ -229,19 +229,30 @@ bool fast_r2c(const FastCtx& c, int M, const T* in, cplx<T>* out, int64_t rows, 
   }
 }
 
+// The fused pass runs two transforms per tile: 8 values per thread and stage
+// (twice the threads of the single-transform passes) keep more waves in
+// flight through its LDS stages.
 template <typename T, int N>
 bool fused_n(const FastCtx& c, cplx<T>* C, const GenGeom& g, const LastAxis& a, const double* eig,
              int op, double inv_next, const cplx<T>* tw, int batch, const double* spec, int nbins)
 {
-  constexpr int L = fast_L<T>(N), NT = fast_NT<T>(N, L);
+  constexpr int L = fast_L<T>(N);
+  constexpr int NT = (L * N / 8 + 63) / 64 * 64 < 64 ? 64 : (L * N / 8 + 63) / 64 * 64;
   const size_t lds = (size_t)L * N * sizeof(cplx<T>);
   if (a.N0h % L != 0 || lds > c.lds_budget)
+    return false;
+  // 32-bit element offsets within one realization's span
+  if ((int64_t)a.Hp * a.Nmid * a.nlast * (int64_t)sizeof(cplx<T>) >= ((int64_t)1 << 32))
     return false;
   if (c.query)
     return true;
   dim3 grid(a.N0h / L + 1, a.Nmid, batch);
-  kf_fused_mult<T, N, L, NT><<<grid, NT, lds, c.stream>>>(C, g, a, eig, op, inv_next, tw, spec,
-                                                          nbins);
+  if (2 * a.nlast <= N)
+    kf_fused_mult<T, N, L, NT, true><<<grid, NT, lds, c.stream>>>(C, g, a, eig, op, inv_next, tw,
+                                                                  spec, nbins);
+  else
+    kf_fused_mult<T, N, L, NT, false><<<grid, NT, lds, c.stream>>>(C, g, a, eig, op, inv_next,
+                                                                   tw, spec, nbins);
   return true;
 }
 

@@ -934,8 +934,8 @@ __global__ void k_energy_partial(const double* it, const double* ait, const doub
 // ---------------------------------------------------------------------------
 struct CgScalars
 {
-  double rz;     // r . z (z = M^{-1} r; r . r for plain CG)
-  double pAp;    // p . A p
+  double rzs[2]; // r . z (z = M^{-1} r; r . r for plain CG): iteration k reads slot k % 2 and
+                 // the p update of iteration k writes the next value to slot (k + 1) % 2
   double rr;     // r . r
   double bb;     // b . b
   double tol2;   // (rel_tol)^2 * b . b
@@ -969,6 +969,21 @@ __device__ __forceinline__ double reduce_partials_dev(const double* partial, int
   return block_sum(s);
 }
 
+// Sum of the nb partials a previous kernel wrote, the same value in every
+// thread of every block: each block of the consuming kernel reduces them
+// itself in one fixed order (no one-block kernel in between).
+__device__ __forceinline__ double all_reduce_partials(const double* partial, int nb)
+{
+  __shared__ double tot;
+  const double t = reduce_partials_dev(partial, nb);
+  if (threadIdx.x == 0)
+    tot = t;
+  __syncthreads();
+  const double v = tot;
+  __syncthreads(); // block_sum's scratch and tot may be reused after this
+  return v;
+}
+
 // partial = a . b over the vector (skipped once the solve is done)
 __global__ void k_cg_dot(const double* a, const double* b, int64_t n, double* partial,
                          const CgScalars* sc)
@@ -984,23 +999,15 @@ __global__ void k_cg_dot(const double* a, const double* b, int64_t n, double* pa
     partial[blockIdx.x] = t;
 }
 
-// pAp = sum of the partials (one block)
-__global__ void k_cg_finish_pap(const double* partial, int nb, CgScalars* sc)
-{
-  if (sc->done)
-    return;
-  const double t = reduce_partials_dev(partial, nb);
-  if (threadIdx.x == 0)
-    sc->pAp = t;
-}
-
-// x += alpha p, r -= alpha Ap (alpha = rz / pAp), partial sums of r . r
+// x += alpha p, r -= alpha Ap with alpha = rz / pAp (pAp from the nb partials
+// of p . Ap, rz from slot par), partial sums of r . r into rr_part
 __global__ void k_cg_update_xr(const double* p, const double* Ap, double* x, double* r, int64_t n,
-                               double* partial, const CgScalars* sc)
+                               const double* pap_part, int nb, double* rr_part,
+                               const CgScalars* sc, int par)
 {
   if (sc->done)
     return;
-  const double alpha = sc->rz / sc->pAp;
+  const double alpha = sc->rzs[par] / all_reduce_partials(pap_part, nb);
   double s = 0.;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -1011,52 +1018,38 @@ __global__ void k_cg_update_xr(const double* p, const double* Ap, double* x, dou
   }
   const double t = block_sum(s);
   if (threadIdx.x == 0)
-    partial[blockIdx.x] = t;
+    rr_part[blockIdx.x] = t;
 }
 
-// rr = r . r; counts the iteration and sets done when rr <= tol^2 b.b
-__global__ void k_cg_finish_rr(const double* partial, int nb, CgScalars* sc)
+// End of an iteration: rr from the partials of r . r. Every block decides
+// alike: with rr <= tol^2 b.b the solve is done and p is left as it is;
+// otherwise p = z + beta p with beta = rz_new / rz (rz_new from the partials
+// of r . z, or rr for plain CG when rz_part is null). Block 0 records rr and
+// the iteration, sets done, or stores rz_new in the other slot (the slot the
+// blocks read stays untouched during the kernel).
+__global__ void k_cg_update_p(const double* z, double* p, int64_t n, const double* rr_part,
+                              const double* rz_part, int nb, CgScalars* sc, int par)
 {
   if (sc->done)
     return;
-  const double t = reduce_partials_dev(partial, nb);
-  if (threadIdx.x == 0) {
-    sc->rr = t;
+  const double rr = all_reduce_partials(rr_part, nb);
+  const bool stop = rr <= sc->tol2;
+  const double rz_new = stop ? 0. : rz_part ? all_reduce_partials(rz_part, nb) : rr;
+  const double rz = sc->rzs[par];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rr = rr;
     sc->it += 1;
-    if (t <= sc->tol2)
+    if (stop)
       sc->done = 1;
+    else
+      sc->rzs[par ^ 1] = rz_new;
   }
-}
-
-// p = z + beta p with beta = rz_new / rz; rz_new from the partials of r . z
-// (plain CG: rz_new = rr, no partials). One kernel per iteration updates p;
-// the first block publishes rz_new after every block has read the old rz, so
-// the scalar is finished by a separate one-block kernel.
-__global__ void k_cg_finish_rz(const double* partial, int nb, CgScalars* sc, double* rz_new)
-{
-  if (sc->done)
+  if (stop)
     return;
-  const double t = partial ? reduce_partials_dev(partial, nb) : sc->rr;
-  if (threadIdx.x == 0)
-    *rz_new = t;
-}
-
-__global__ void k_cg_update_p(const double* z, double* p, int64_t n, const double* rz_new,
-                              const CgScalars* sc)
-{
-  if (sc->done)
-    return;
-  const double beta = *rz_new / sc->rz;
+  const double beta = rz_new / rz;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
-}
-
-__global__ void k_cg_commit_rz(CgScalars* sc, const double* rz_new)
-{
-  if (sc->done || threadIdx.x != 0)
-    return;
-  sc->rz = *rz_new;
 }
 
 } // namespace pfamd

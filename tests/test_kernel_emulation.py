@@ -123,3 +123,18 @@ def test_dft_passes_host_build(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "all OK" in r.stdout and r.stdout.count(" OK") >= 11
+
+def test_fused_product_pass_host_build(tmp_path):
+    """The fused last-axis pass of the covariance product (kf_fused_mult),
+    every block of 3D (both storage orders) and 2D shapes, FP64 and FP32,
+    multiply / root / inverse-floor, against direct O(N^2) sums."""
+    src = _host_sources(str(tmp_path))
+    exe = str(tmp_path / "fused_pass_check")
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-w", "-I", EMU, "-I", src,
+           os.path.join(EMU, "fused_pass_check.cc"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "all OK" in r.stdout
