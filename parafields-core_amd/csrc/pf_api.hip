@@ -2000,6 +2000,9 @@ struct Mlp
   std::vector<int64_t> woff, boff;
   int64_t nparams = 0;
   DevBuf params, grads, m, v, x, y, d0, d1, partial;
+  // weight gradients: block partials over kGradSplit interleaved sample sets
+  static constexpr int kGradSplit = 32;
+  DevBuf gpart;
   std::vector<DevBuf> Z, A;
   int t = 0;
   hipStream_t stream = nullptr;
@@ -2028,6 +2031,10 @@ struct Mlp
     d0.ensure((size_t)B * wmax * sizeof(float));
     d1.ensure((size_t)B * wmax * sizeof(float));
     partial.ensure(1024 * sizeof(double));
+    int64_t gmax = 0;
+    for (int l = 0; l < nl(); l++)
+      gmax = std::max<int64_t>(gmax, (int64_t)dims[l] * dims[l + 1] + dims[l + 1]);
+    gpart.ensure((size_t)kGradSplit * gmax * sizeof(float));
     cap = B;
   }
   void forward(int B)
@@ -2058,8 +2065,11 @@ struct Mlp
         k_mlp_gelu_back<<<grid1d((int64_t)B * b), 256, 0, stream>>>(D, Z[l].as<float>(),
                                                                     (int64_t)B * b);
       const float* in = l == 0 ? x.as<float>() : A[l - 1].as<float>();
-      k_mlp_grad_w<<<(unsigned)((a * b + b + 255) / 256), 256, 0, stream>>>(
-        D, in, grads.as<float>() + woff[l], grads.as<float>() + boff[l], B, a, b);
+      const int nout = a * b + b, nsplit = std::max(1, std::min(kGradSplit, B / 16));
+      k_mlp_grad_part<<<dim3((unsigned)nsplit, (unsigned)((nout + 255) / 256)), 256, 0, stream>>>(
+        D, in, gpart.as<float>(), B, a, b);
+      k_mlp_grad_sum<<<(unsigned)((nout + 255) / 256), 256, 0, stream>>>(
+        gpart.as<float>(), nsplit, grads.as<float>() + woff[l], grads.as<float>() + boff[l], a, b);
       if (l > 0) {
         k_mlp_back_in<<<(unsigned)(((int64_t)B * a + 255) / 256), 256, 0, stream>>>(
           D, params.as<float>() + woff[l], Dn, B, a, b);
@@ -3124,10 +3134,12 @@ int pf_spectral_targets(void* plan, int32_t nbins, double floor_value, double* l
     HIPCHECK(hipMemsetAsync(sums.p, 0, nbins * sizeof(double), P->stream));
     HIPCHECK(hipMemsetAsync(counts.p, 0, nbins * sizeof(unsigned long long), P->stream));
     const int64_t total = (int64_t)g.P[0] * g.P[1] * g.P[2];
-    k_radial_sums<<<grid1d(total), 256, 0, P->stream>>>(P->eig.as<double>(), g.P[0], g.P[1],
-                                                        g.P[2], g.dim, nbins, floor_value,
-                                                        sums.as<double>(),
-                                                        counts.as<unsigned long long>());
+    const size_t lds = (size_t)nbins * (sizeof(double) + sizeof(unsigned long long));
+    if (lds > 65536)
+      throw PfError(PF_EINVAL, "nbins too large for the per-block bins");
+    k_radial_sums<<<std::min<unsigned>(1024, grid1d(total)), 256, lds, P->stream>>>(
+      P->eig.as<double>(), g.P[0], g.P[1], g.P[2], g.dim, nbins, floor_value, sums.as<double>(),
+      counts.as<unsigned long long>());
     check_launch();
     std::vector<double> hs(nbins);
     std::vector<unsigned long long> hc(nbins);

@@ -64,24 +64,39 @@ __global__ void k_mlp_gelu_back(float* D, const float* Z, int64_t n)
     D[i] *= gelu_grad(Z[i]);
 }
 
-// gW[j][i] = sum_s Dz[s][j] X[s][i]; gb[j] = sum_s Dz[s][j]
-__global__ void k_mlp_grad_w(const float* Dz, const float* X, float* gW, float* gb, int B, int a,
-                             int b)
+// Weight and bias gradients of one layer, gW[j][i] = sum_s Dz[s][j] X[s][i]
+// and gb[j] = sum_s Dz[s][j], in two steps: block (split q, output tile) sums
+// the samples s = q, q + nsplit, ... of its outputs t (t < a b: weight j i,
+// t = a b + j: bias j) into part[q][t]; k_mlp_grad_sum adds the nsplit
+// partials of each output in one fixed order.
+__global__ void k_mlp_grad_part(const float* Dz, const float* X, float* part, int B, int a, int b)
+{
+  const int t = blockIdx.y * blockDim.x + threadIdx.x;
+  const int q = blockIdx.x, nsplit = gridDim.x;
+  const int nout = a * b + b;
+  if (t >= nout)
+    return;
+  const bool w = t < a * b;
+  const int j = w ? t / a : t - a * b, i = w ? t % a : 0;
+  float acc = 0.f;
+  for (int s = q; s < B; s += nsplit)
+    acc += Dz[(int64_t)s * b + j] * (w ? X[(int64_t)s * a + i] : 1.f);
+  part[(int64_t)q * nout + t] = acc;
+}
+
+__global__ void k_mlp_grad_sum(const float* part, int nsplit, float* gW, float* gb, int a, int b)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < a * b) {
-    const int j = t / a, i = t % a;
-    float acc = 0.f;
-    for (int s = 0; s < B; s++)
-      acc += Dz[(int64_t)s * b + j] * X[(int64_t)s * a + i];
+  const int nout = a * b + b;
+  if (t >= nout)
+    return;
+  float acc = 0.f;
+  for (int q = 0; q < nsplit; q++)
+    acc += part[(int64_t)q * nout + t];
+  if (t < a * b)
     gW[t] = acc;
-  } else if (t < a * b + b) {
-    const int j = t - a * b;
-    float acc = 0.f;
-    for (int s = 0; s < B; s++)
-      acc += Dz[(int64_t)s * b + j];
-    gb[j] = acc;
-  }
+  else
+    gb[t - a * b] = acc;
 }
 
 // Din[s][i] = sum_j Dz[s][j] W[j][i]
@@ -113,19 +128,35 @@ __global__ void k_mlp_adam(float* p, const float* g, float* m, float* v, int64_t
 }
 
 // Per-bin sums of max(lambda, floor) and counts over the DCT-half eigenvalue
-// array [P2][P1][P0] (every folded spectral position once).
+// array [P2][P1][P0] (every folded spectral position once). Each block keeps
+// its bins in LDS (dynamic shared memory of nbins sums and nbins counts) and
+// adds them to the global bins once at its end.
 __global__ void k_radial_sums(const double* eig, int P0, int P1, int P2, int dim, int nbins,
                               double floor_v, double* sums, unsigned long long* counts)
 {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* lsum = reinterpret_cast<double*>(smem);
+  unsigned long long* lcnt = reinterpret_cast<unsigned long long*>(lsum + nbins);
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+    lsum[i] = 0.;
+    lcnt[i] = 0ull;
+  }
+  __syncthreads();
   const int64_t total = (int64_t)P0 * P1 * P2;
   for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < total;
        f += (int64_t)gridDim.x * blockDim.x) {
     const int a = (int)(f % P0), b = (int)((f / P0) % P1), c = (int)(f / ((int64_t)P0 * P1));
     const int bin = radial_bin(a, b, c, P0 - 1, P1 - 1, P2 > 1 ? P2 - 1 : 1, dim, nbins);
     const double v = eig[f] > floor_v ? eig[f] : floor_v;
-    atomicAdd(&sums[bin], v);
-    atomicAdd(&counts[bin], 1ull);
+    atomicAdd(&lsum[bin], v);
+    atomicAdd(&lcnt[bin], 1ull);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+    if (lcnt[i] > 0) {
+      atomicAdd(&sums[i], lsum[i]);
+      atomicAdd(&counts[i], lcnt[i]);
+    }
 }
 
 } // namespace pfamd
