@@ -2227,7 +2227,8 @@ struct Batch
       constexpr int C = decltype(c)::value;
       const dim3 grid(row_blocks(rows, (int)N0), nf);
       with_period(P.d.periodization, [&](auto q) {
-        k_eval_cov_full_group<C, decltype(q)::value><<<grid, 256, 0, s>>>(
+        // groups are 2D (groupable)
+        k_eval_cov_full_group<C, decltype(q)::value, 2><<<grid, 256, 0, s>>>(
           G.cov.as<CovParams>(), (int)N0, (int)N1, 1, real, N0 * rows);
       });
     });

@@ -473,11 +473,15 @@ __global__ void k_finish_r2c_eigs(const cplx<double>* spec, int Hp, int H, int64
 // Field f = blockIdx.y of a group: its own covariance parameters, threshold
 // and arrays at stride `stride` (entries) from the group's base.
 
-template <int COV, int PER>
-__global__ void k_eval_cov_full_group(const CovParams* cps, int N0, int N1, int N2, double* out,
-                                      int64_t stride)
+template <int COV, int PER, int DIM>
+__global__ void k_eval_cov_full_group(const CovParams* __restrict__ cps, int N0, int N1, int N2,
+                                      double* __restrict__ out, int64_t stride)
 {
-  const CovParams& p = cps[blockIdx.y]; // read in place (uniform address): no local copy
+  // a local copy with the dimension fixed at compile time: every loop over
+  // the axes unrolls, every parameter index is static, and the parameters
+  // (uniform) stay in scalar registers instead of being loaded per point
+  CovParams p = cps[blockIdx.y];
+  p.dim = DIM;
   out += (int64_t)blockIdx.y * stride;
   for_rows((int64_t)N1 * N2, N0, [&](int64_t row, int i) {
     const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)N1),
