@@ -467,9 +467,6 @@ def main():
                     help="run one other configuration in this process and print its JSON "
                          "result (what the child processes of the default run do)")
     a = ap.parse_args()
-    if a.leg:
-        print(json.dumps(LEGS[a.leg]()), flush=True)
-        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine holds the whole extended grid on one "
               "device and has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -484,6 +481,9 @@ def main():
         print("bench: copy probe of 1073741824 bytes x3 done", file=sys.stderr, flush=True)
         del src, dst
         torch.cuda.empty_cache()
+    if a.leg:
+        print(json.dumps(LEGS[a.leg]()), flush=True)
+        return
     elapsed, per_kernel, st, eig_s, stats, field32 = run_gpu(a.steps, a.warmup, a.batch)
     fields = a.steps * a.batch
     fps = fields / elapsed

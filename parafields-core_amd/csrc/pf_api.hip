@@ -730,16 +730,127 @@ pf_eig_stats reduce_partials(Plan& P, int nb)
   return s;
 }
 
+// Whether the 2D R2C eigenvalues can come from the even part of the
+// covariance: specialised kernels for the r2c along x of rows 0 .. N1/2 and
+// for the real y pass (kf_eig_c2r_y) of N1/2 points.
+bool even2d_ok(const Geo& g)
+{
+  if (g.dim != 2 || g.N[0] % 2 != 0 || g.N[1] % 2 != 0 || fast_budget() == 0)
+    return false;
+  FastCtx q{ nullptr, fast_budget() };
+  q.query = true;
+  const int N0 = g.N[0], N1 = g.N[1];
+  return fast_r2c<double>(q, N0 / 2, nullptr, nullptr, N1 / 2 + 1, g.Hp, N0, nullptr, 1) &&
+         fast_eig_c2r_y<float>(q, N1 / 2, nullptr, g.Hp, (int)g.H, N0 / 2, 0., 0., nullptr,
+                               nullptr, nullptr, nullptr, nullptr, 1, nullptr);
+}
+
+// Classification partials per realization of kf_eig_c2r_y for this shape.
+int even2d_blocks(const Geo& g)
+{
+  FastCtx q{ nullptr, fast_budget() };
+  q.query = true;
+  int nb = 0;
+  fast_eig_c2r_y<float>(q, g.N[1] / 2, nullptr, g.Hp, (int)g.H, g.N[0] / 2, 0., 0., nullptr,
+                        nullptr, nullptr, nullptr, nullptr, 1, &nb);
+  return nb;
+}
+
+// R2C matrix, 2D, full real-part check, from the even part e of the
+// embedded covariance (k_eval_cov_even2d): e on rows 0 .. N1/2, r2c of those
+// rows along x, then one real pass along y that scales, classifies and
+// writes the eigenvalues and the square-root table (kf_eig_c2r_y). The
+// eigenvalues are the real part of the embedding's DFT, as in the full
+// transform; the passes move about a third of its bytes.
+bool compute_eigs_even2d(Plan& P, const double* custom_host, ClassifyPartial* dev_stats,
+                         const CovParams& cp, double inv_next)
+{
+  const Geo& g = P.g;
+  const pf_desc& d = P.d;
+  const int64_t H = g.H, N0 = g.N[0], N1 = g.N[1], hr = N1 / 2 + 1;
+  const int64_t count = H * N1;
+  P.scratch.ensure((size_t)N0 * hr * sizeof(double) + (size_t)g.Hp * hr * sizeof(cplx<double>));
+  double* real = P.scratch.as<double>();
+  cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + N0 * hr);
+  if (custom_host) {
+    // custom values arrive as the padded 2H x N1 real array: its even part
+    std::vector<double> even(N0 * hr);
+    for (int64_t j = 0; j < hr; j++) {
+      const int64_t jm = j ? N1 - j : 0;
+      for (int64_t i = 0; i < N0; i++) {
+        const int64_t im = i ? N0 - i : 0;
+        even[j * N0 + i] = 0.5 * (custom_host[j * 2 * H + i] + custom_host[jm * 2 * H + im]);
+      }
+    }
+    HIPCHECK(hipMemcpyAsync(real, even.data(), even.size() * sizeof(double),
+                            hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+  } else {
+    Scope sc(&P, "eig_cov");
+    with_cov(cp.covariance, [&](auto c) {
+      constexpr int C = decltype(c)::value;
+      with_period(cp.period, [&](auto q) {
+        k_eval_cov_even2d<C, decltype(q)::value><<<row_blocks(hr, (int)N0), 256, 0, P.stream>>>(
+          cp, (int)N0, (int)N1, real);
+      });
+    });
+    check_launch();
+  }
+  const FastCtx fc{ P.stream, fast_budget() };
+  {
+    Scope sc(&P, "eig_r2c_x");
+    if (!fast_r2c<double>(fc, (int)(N0 / 2), real, spec, hr, g.Hp, (int)N0,
+                          P.table<double>((int)N0), 1))
+      throw PfError(PF_EHIP, "even-part eigenvalue r2c launch rejected");
+    check_launch();
+  }
+  P.eig.ensure(count * sizeof(double));
+  const int64_t ne = g.eig_entries;
+  int nb = even2d_blocks(g);
+  P.partial.ensure((size_t)std::max(nb, 1024) * sizeof(ClassifyPartial));
+  {
+    Scope sc(&P, "eig_classify");
+    const cplx<double>* tw = P.table<double>((int)N1);
+    bool ok;
+    if (d.precision == PF_FP32) {
+      P.sq32.ensure(ne * sizeof(float));
+      ok = fast_eig_c2r_y<float>(fc, (int)(N1 / 2), spec, g.Hp, (int)H, (int)(N0 / 2), inv_next,
+                                 d.threshold, nullptr, P.eig.as<double>(), P.sq32.as<float>(),
+                                 P.partial.as<ClassifyPartial>(), tw, 1, &nb);
+    } else {
+      P.sq64.ensure(ne * sizeof(double));
+      ok = fast_eig_c2r_y<double>(fc, (int)(N1 / 2), spec, g.Hp, (int)H, (int)(N0 / 2), inv_next,
+                                  d.threshold, nullptr, P.eig.as<double>(), P.sq64.as<double>(),
+                                  P.partial.as<ClassifyPartial>(), tw, 1, &nb);
+    }
+    if (!ok)
+      throw PfError(PF_EHIP, "even-part eigenvalue y-pass launch rejected");
+    check_launch();
+  }
+  if (dev_stats)
+    k_reduce_partials<<<1, 64, 0, P.stream>>>(P.partial.as<ClassifyPartial>(), nb, dev_stats);
+  else {
+    P.st = reduce_partials(P, nb);
+    P.st.entries = count;
+    P.scratch.release();
+  }
+  return true;
+}
+
 // R2C matrix, 2D, full real-part check (r2c_check_all): covariance on the
 // padded grid, compile-time r2c along x and forward DFT along y (in place,
 // row pitch Hp), then one fused pass for scaling, classification, the real
 // part and the square-root table. Returns false (nothing changed but the
-// scratch contents) when a line length has no specialised kernel.
+// scratch contents) when a line length has no specialised kernel. Shapes the
+// even-part kernels cover take compute_eigs_even2d (except cofold, whose
+// offset pass works on the full grid).
 bool compute_eigs_r2c2d_fast(Plan& P, const double* custom_host, ClassifyPartial* dev_stats,
                              const CovParams& cp, double inv_next, int nbp)
 {
   const Geo& g = P.g;
   const pf_desc& d = P.d;
+  if (d.periodization != PF_PERIOD_COFOLD && even2d_ok(g))
+    return compute_eigs_even2d(P, custom_host, dev_stats, cp, inv_next);
   const int64_t H = g.H, N0 = g.N[0], N1 = g.N[1], rows = N1;
   const int64_t count = H * rows;
   P.scratch.ensure((size_t)N0 * rows * sizeof(double) +
@@ -2107,9 +2218,10 @@ bool groupable(const Plan& P)
   const int N0 = g.N[0], N1 = g.N[1];
   const GenGeom gg = gen_geom(P);
   const LastAxis la = last_axis(g);
-  return fast_r2c<double>(q, N0 / 2, nullptr, nullptr, N1, g.Hp, N0, nullptr, 1) &&
-         fast_mid<double>(q, N1, false, nullptr, nullptr, g.Hp, N0 / 2, 1, N1, N1, N1, N1,
-                          nullptr, 1) &&
+  return (even2d_ok(g) ||
+          (fast_r2c<double>(q, N0 / 2, nullptr, nullptr, N1, g.Hp, N0, nullptr, 1) &&
+           fast_mid<double>(q, N1, false, nullptr, nullptr, g.Hp, N0 / 2, 1, N1, N1, N1, N1,
+                            nullptr, 1))) &&
          fast_gen_last<float>(q, N1, gg, la, nullptr, nullptr, 0, 0, nullptr, nullptr, 1) &&
          fast_c2r<float>(q, N0 / 2, nullptr, nullptr, g.n[1], g.Hp, g.n[0], nullptr, 0, 0.f, 1);
 }
@@ -2207,9 +2319,11 @@ struct Batch
     }
   }
   // One group on stream s: FP64 eigenvalues of all its fields (covariance,
-  // r2c along x, forward DFT along y, scale/classify/root, per-field
-  // reduction into stats), then one FP32 realization of each (pass G along
-  // y, c2r along x into the field's place in the output).
+  // r2c along x, y pass, scale/classify/root, per-field reduction into
+  // stats), then one FP32 realization of each (pass G along y, c2r along x
+  // into the field's place in the output). The eigenvalues take the path of
+  // a single plan of the shape: from the even part of the covariance where
+  // its kernels exist (compute_eigs_even2d), else from the full grid.
   void enqueue_group(FieldGroup& G, hipStream_t s, uint64_t seed, uint64_t first, int transform,
                      double param, void* out)
   {
@@ -2217,38 +2331,56 @@ struct Batch
     P.stream = s;
     const Geo& g = P.g;
     const int nf = (int)G.fields.size();
+    const bool even = even2d_ok(g);
     const int64_t H = g.H, N0 = g.N[0], N1 = g.N[1], rows = N1, count = H * rows;
+    const int64_t hr = even ? N1 / 2 + 1 : rows; // covariance rows evaluated
     const double inv_next = 1.0 / (double)g.Next;
-    G.scratch.ensure((size_t)nf * (N0 * rows * sizeof(double) + g.Hp * rows * sizeof(cplx<double>)));
+    G.scratch.ensure((size_t)nf * (N0 * hr * sizeof(double) + g.Hp * hr * sizeof(cplx<double>)));
     double* real = G.scratch.as<double>();
-    cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + nf * N0 * rows);
+    cplx<double>* spec = reinterpret_cast<cplx<double>*>(real + nf * N0 * hr);
     with_cov(P.d.covariance, [&](auto c) {
       // the fields of a group share one periodization (part of the group key)
       constexpr int C = decltype(c)::value;
-      const dim3 grid(row_blocks(rows, (int)N0), nf);
+      const dim3 grid(row_blocks(hr, (int)N0), nf);
       with_period(P.d.periodization, [&](auto q) {
         // groups are 2D (groupable)
-        k_eval_cov_full_group<C, decltype(q)::value, 2><<<grid, 256, 0, s>>>(
-          G.cov.as<CovParams>(), (int)N0, (int)N1, 1, real, N0 * rows);
+        if (even)
+          k_eval_cov_even2d_group<C, decltype(q)::value><<<grid, 256, 0, s>>>(
+            G.cov.as<CovParams>(), (int)N0, (int)N1, real, N0 * hr);
+        else
+          k_eval_cov_full_group<C, decltype(q)::value, 2><<<grid, 256, 0, s>>>(
+            G.cov.as<CovParams>(), (int)N0, (int)N1, 1, real, N0 * rows);
       });
     });
     check_launch();
     const FastCtx fc{ s, fast_budget() };
-    if (!fast_r2c<double>(fc, (int)(N0 / 2), real, spec, rows, g.Hp, (int)N0,
-                          P.table<double>((int)N0), nf) ||
-        !fast_mid<double>(fc, (int)N1, false, spec, spec, g.Hp, (int)(N0 / 2), 1, (int)N1,
-                          (int)N1, (int)N1, (int)N1, P.table<double>((int)N1), nf))
-      throw PfError(PF_EHIP, "grouped eigenvalue launch rejected");
-    check_launch();
-    const int nbp = 1024;
-    const unsigned nb = std::min<unsigned>(nbp, grid1d(count));
     G.sq.ensure((size_t)nf * count * sizeof(float)); // the FP64 eigenvalues are not kept
-    G.partial.ensure((size_t)nf * nb * sizeof(ClassifyPartial));
-    k_finish_r2c_eigs_group<float><<<dim3(nb, nf), 256, 0, s>>>(
-      spec, g.Hp, (int)H, rows, inv_next, G.thr.as<double>(), nullptr, G.sq.as<float>(),
-      G.partial.as<ClassifyPartial>());
-    check_launch();
-    k_reduce_partials_group<<<nf, 256, 0, s>>>(G.partial.as<ClassifyPartial>(), (int)nb,
+    if (!fast_r2c<double>(fc, (int)(N0 / 2), real, spec, hr, g.Hp, (int)N0,
+                          P.table<double>((int)N0), nf))
+      throw PfError(PF_EHIP, "grouped eigenvalue launch rejected");
+    int nb;
+    if (even) {
+      nb = even2d_blocks(g);
+      G.partial.ensure((size_t)nf * nb * sizeof(ClassifyPartial));
+      if (!fast_eig_c2r_y<float>(fc, (int)(N1 / 2), spec, g.Hp, (int)H, (int)(N0 / 2), inv_next,
+                                 0., G.thr.as<double>(), nullptr, G.sq.as<float>(),
+                                 G.partial.as<ClassifyPartial>(), P.table<double>((int)N1), nf,
+                                 &nb))
+        throw PfError(PF_EHIP, "grouped eigenvalue launch rejected");
+      check_launch();
+    } else {
+      if (!fast_mid<double>(fc, (int)N1, false, spec, spec, g.Hp, (int)(N0 / 2), 1, (int)N1,
+                            (int)N1, (int)N1, (int)N1, P.table<double>((int)N1), nf))
+        throw PfError(PF_EHIP, "grouped eigenvalue launch rejected");
+      check_launch();
+      nb = (int)std::min<unsigned>(1024, grid1d(count));
+      G.partial.ensure((size_t)nf * nb * sizeof(ClassifyPartial));
+      k_finish_r2c_eigs_group<float><<<dim3(nb, nf), 256, 0, s>>>(
+        spec, g.Hp, (int)H, rows, inv_next, G.thr.as<double>(), nullptr, G.sq.as<float>(),
+        G.partial.as<ClassifyPartial>());
+      check_launch();
+    }
+    k_reduce_partials_group<<<nf, 256, 0, s>>>(G.partial.as<ClassifyPartial>(), nb,
                                                G.fidx.as<int64_t>(), stats.as<ClassifyPartial>());
     check_launch();
     GenGeom gg = gen_geom(P);

@@ -68,6 +68,32 @@ __device__ __forceinline__ int64_t eig_index(const GenGeom& g, int kx, int k1, i
   return kx + (int64_t)g.H * (k1 + (int64_t)g.N1 * k2);
 }
 
+// Eigenvalue classification (matrix.hh:266-291): counts of small, zero and
+// negative eigenvalues against the threshold, and the smallest value; the
+// value is clamped to 0 when negative.
+struct ClassifyPartial
+{
+  long long small, zero, negative;
+  double smallest;
+};
+
+__device__ __forceinline__ void classify_one(double& v, double thr, ClassifyPartial& c)
+{
+  if (v < c.smallest)
+    c.smallest = v;
+  if (v < 1e-6) {
+    if (v < thr) {
+      if (v > -thr)
+        c.zero++;
+      else
+        c.negative++;
+    } else
+      c.small++;
+  }
+  if (v < 0.)
+    v = 0.;
+}
+
 // Operator index of the learned spectral preconditioner in the fused matvec
 // pass (after PF_OP_MULTIPLY / PF_OP_ROOT / PF_OP_INVERSE_EMBED).
 constexpr int PF_OP_SPECTRAL = 3;

@@ -715,6 +715,97 @@ __global__ void __launch_bounds__(NT) kf_c2r_x(const cplx<T>* in, TO* out, int64
                               blockIdx.y);
 }
 
+// Eigenvalues of a 2D R2C embedding from the x half spectrum X of the even
+// part of its covariance (rows j = 0 .. M of the N1 = 2M rows, row pitch Hp
+// complex; realization f at f * Hp * (M + 1)). Along y every column kx of X
+// is Hermitian, X[N1 - j] = conj X[j] with rows 0 and M real, so its forward
+// DFT along y is real: the inverse DFT of the conjugated column, computed as
+// one M-point complex FFT of the packed column (the packing of c2r_x_tile),
+// whose entry m holds outputs 2m and 2m + 1. Each eigenvalue is scaled by
+// inv_next, classified and clamped (classify_one) and stored as FP64 (eig,
+// optional) and as its square root (sq), both [N1][H] per realization. One
+// classification partial per block at partial[f * gridDim.x + tile].
+// Tile bx < N0h / L: columns bx*L .. bx*L + L - 1; tile N0h / L: the Nyquist
+// column N0h alone. thrs: per-realization thresholds (nullptr: thr0).
+template <typename TS, int M, int L, int NT>
+__global__ void __launch_bounds__(NT) kf_eig_c2r_y(const cplx<double>* spec, int Hp, int H, int N0h,
+                                                   double inv_next, double thr0,
+                                                   const double* thrs, double* eig, TS* sq,
+                                                   ClassifyPartial* partial,
+                                                   const cplx<double>* tw)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<double>* buf = reinterpret_cast<cplx<double>*>(smem);
+  // 4 FP64 columns are half a 128-byte line: neighbouring tiles run on one XCD
+  int bx = blockIdx.x, f = blockIdx.y, bz = 0;
+  if constexpr (L * sizeof(cplx<double>) < 128)
+    xcd_block(bx, f, bz);
+  const int kx0 = bx < N0h / L ? bx * L : N0h;
+  const int lmax = bx < N0h / L ? L : 1; // live columns of the tile
+  const double t = thrs ? thrs[f] : thr0;
+  const int64_t cnt = (int64_t)H * (2 * M);
+  const char* sb = reinterpret_cast<const char*>(spec + (int64_t)f * Hp * (M + 1) + kx0);
+  TS* sqf = sq + f * cnt + kx0;
+  double* eigf = eig ? eig + f * cnt + kx0 : nullptr;
+  const uint32_t eb = (uint32_t)Hp * (uint32_t)sizeof(cplx<double>);
+  auto load = [&](int e, int l) -> cplx<double> {
+    cplx<double> z = { 0, 0 };
+    if (l < lmax) {
+      const uint32_t lo = (uint32_t)l * (uint32_t)sizeof(cplx<double>);
+      cplx<double> xk = *reinterpret_cast<const cplx<double>*>(sb + ((uint32_t)e * eb + lo));
+      cplx<double> xm = *reinterpret_cast<const cplx<double>*>(sb + ((uint32_t)(M - e) * eb + lo));
+      if (e == 0) {
+        xk.y = 0;
+        xm.y = 0;
+      }
+      // packing of conj X: s = conj X[e] + X[M-e], d = conj X[e] - X[M-e]
+      const cplx<double> s = { xk.x + xm.x, xm.y - xk.y };
+      const cplx<double> d = { xk.x - xm.x, -xk.y - xm.y };
+      const cplx<double> td = cmul(cconj(tw[e]), d);
+      z = { s.x - td.y, s.y + td.x };
+    }
+    return z;
+  };
+  ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
+  auto store = [&](int m, int l, const cplx<double>& v) {
+    if (l < lmax) {
+      double a = v.x * inv_next, b = v.y * inv_next;
+      classify_one(a, t, c);
+      classify_one(b, t, c);
+      const uint32_t o0 = (uint32_t)(2 * m) * (uint32_t)H + (uint32_t)l, o1 = o0 + (uint32_t)H;
+      if (eigf) {
+        eigf[o0] = a;
+        eigf[o1] = b;
+      }
+      sqf[o0] = (TS)sqrt(a);
+      sqf[o1] = (TS)sqrt(b);
+    }
+  };
+  tfft_io<double, M, L, NT, 0, true, 2>(buf, tw, (int)threadIdx.x, load, store);
+  // block sum of the partials through the tile's LDS (free after the barrier)
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    c.small += __shfl_down(c.small, off);
+    c.zero += __shfl_down(c.zero, off);
+    c.negative += __shfl_down(c.negative, off);
+    c.smallest = fmin(c.smallest, __shfl_down(c.smallest, off));
+  }
+  ClassifyPartial* red = reinterpret_cast<ClassifyPartial*>(smem);
+  if (threadIdx.x % 64 == 0)
+    red[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ClassifyPartial r = red[0];
+    for (int w = 1; w < NT / 64; w++) {
+      r.small += red[w].small;
+      r.zero += red[w].zero;
+      r.negative += red[w].negative;
+      r.smallest = fmin(r.smallest, red[w].smallest);
+    }
+    partial[(int64_t)f * gridDim.x + bx] = r;
+  }
+}
+
 // r2c along x of zero-padded real rows (n0 of N0 values), see k_r2c_x.
 template <typename T, typename TI, int M, int L, int NT>
 __global__ void __launch_bounds__(NT) kf_r2c_x(const TI* in, cplx<T>* out, int64_t rows, int Hp,

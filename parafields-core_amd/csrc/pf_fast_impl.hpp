@@ -229,6 +229,49 @@ bool fast_r2c(const FastCtx& c, int M, const T* in, cplx<T>* out, int64_t rows, 
   }
 }
 
+template <typename TS, int M>
+bool eig_c2r_y_n(const FastCtx& c, const cplx<double>* spec, int Hp, int H, int N0h,
+                 double inv_next, double thr0, const double* thrs, double* eig, TS* sq,
+                 ClassifyPartial* partial, const cplx<double>* tw, int batch, int* blocks)
+{
+  // full-width tiles: 4 lines at M = 1024 (64 KB); half-width tiles (more
+  // blocks per CU, 32-byte row segments) measured slower
+  constexpr int L = fast_L<double>(M);
+  constexpr int NT = fast_NT<double>(M, L);
+  const size_t lds = (size_t)L * M * sizeof(cplx<double>);
+  // 32-bit offsets: input rows within one realization's span, outputs
+  // within one realization's [2M][H] tables
+  if (N0h % L != 0 || lds > c.lds_budget || H > Hp ||
+      (int64_t)(M + 1) * Hp * (int64_t)sizeof(cplx<double>) >= ((int64_t)1 << 32) ||
+      (int64_t)2 * M * H >= ((int64_t)1 << 31))
+    return false;
+  const int nb = N0h / L + 1;
+  if (blocks)
+    *blocks = nb;
+  if (c.query)
+    return true;
+  kf_eig_c2r_y<TS, M, L, NT><<<dim3(nb, batch), NT, lds, c.stream>>>(
+    spec, Hp, H, N0h, inv_next, thr0, thrs, eig, sq, partial, tw);
+  return true;
+}
+
+template <typename TS>
+bool fast_eig_c2r_y(const FastCtx& c, int M, const cplx<double>* spec, int Hp, int H, int N0h,
+                    double inv_next, double thr0, const double* thrs, double* eig, TS* sq,
+                    ClassifyPartial* partial, const cplx<double>* tw, int batch, int* blocks)
+{
+  switch (M) {
+#define X(n)                                                                                       \
+  case n:                                                                                          \
+    return eig_c2r_y_n<TS, n>(c, spec, Hp, H, N0h, inv_next, thr0, thrs, eig, sq, partial, tw,    \
+                              batch, blocks);
+    PF_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
 // The fused pass runs two transforms per tile: 8 values per thread and stage
 // (twice the threads of the single-transform passes) keep more waves in
 // flight through its LDS stages.
@@ -412,6 +455,9 @@ bool fast_dct_x(const FastCtx& c, int M, const DctGeom& g, const T* Hh, T* out, 
                             const cplx<T>*, int, T, int);                                          \
   template bool fast_r2c<T>(const FastCtx&, int, const T*, cplx<T>*, int64_t, int, int,            \
                             const cplx<T>*, int);                                                  \
+  template bool fast_eig_c2r_y<T>(const FastCtx&, int, const cplx<double>*, int, int, int, double,  \
+                                  double, const double*, double*, T*, ClassifyPartial*,            \
+                                  const cplx<double>*, int, int*);                                 \
   template bool fast_fused<T>(const FastCtx&, int, cplx<T>*, const GenGeom&, const LastAxis&,      \
                               const double*, int, double, const cplx<T>*, int, const double*,      \
                               int);                                                                \

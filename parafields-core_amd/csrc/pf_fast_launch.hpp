@@ -36,6 +36,14 @@ bool fast_c2r(const FastCtx& c, int M, const cplx<T>* in, T* out, int64_t rows, 
 template <typename T>
 bool fast_r2c(const FastCtx& c, int M, const T* in, cplx<T>* out, int64_t rows, int H, int n0,
               const cplx<T>* tw, int batch);
+// Eigenvalues of a 2D R2C embedding from the x half spectrum of its even
+// part (rows 0 .. M of N1 = 2M): y pass, scaling, classification and roots
+// (kf_eig_c2r_y; sq of type T). *blocks: classification partials per
+// realization (set also in query mode).
+template <typename T>
+bool fast_eig_c2r_y(const FastCtx& c, int M, const cplx<double>* spec, int Hp, int H, int N0h,
+                    double inv_next, double thr0, const double* thrs, double* eig, T* sq,
+                    ClassifyPartial* partial, const cplx<double>* tw, int batch, int* blocks);
 template <typename T>
 bool fast_fused(const FastCtx& c, int N, cplx<T>* C, const GenGeom& g, const LastAxis& a,
                 const double* eig, int op, double inv_next, const cplx<T>* tw, int batch,

@@ -77,6 +77,37 @@ __global__ void k_eval_cov_full(CovParams p, int N0, int N1, int N2, double* out
   });
 }
 
+// Even part of the 2D embedded covariance, e(i, j) = (c(i, j) + c(-i, -j)) / 2
+// with indices mod (N0, N1) (c as in k_eval_cov_full). The real part of the
+// embedding's DFT is the DFT of e (the odd part transforms to an imaginary
+// array), and e is point-symmetric, so its rows j = 0 .. N1/2 determine it.
+// Classical embedding: c(-i, -j) is c(i, j) up to the rounding of the wrapped
+// coordinates, except on the Nyquist column 2i = N0, where -i wraps onto i
+// and the mirror is c(i, -j), another value under a rotated anisotropy; so
+// only that column is averaged. Rows 0 and N1/2 need no mirror: the y pass
+// keeps the real part of their x spectra, the DFT of their even part in x.
+// Merge and fold embeddings average every entry.
+template <int COV, int PER>
+__device__ __forceinline__ double cov_even_2d(const CovParams& p, int N0, int N1, int i, int j)
+{
+  const int64_t a[3] = { i, j, 0 };
+  const double v = embedded_cov<COV, PER>(p, a);
+  if (PER == 0 && !(2 * i == N0 && j != 0 && 2 * j != N1))
+    return v;
+  const int64_t b[3] = { i ? N0 - i : 0, j ? N1 - j : 0, 0 };
+  return 0.5 * (v + embedded_cov<COV, PER>(p, b));
+}
+
+// e on rows 0 .. N1/2 of the 2D embedding: out[j][i], N1/2 + 1 rows of N0.
+template <int COV, int PER>
+__global__ void k_eval_cov_even2d(CovParams p, int N0, int N1, double* out)
+{
+  p.dim = 2;
+  for_rows(N1 / 2 + 1, N0, [&](int64_t row, int i) {
+    out[row * N0 + i] = cov_even_2d<COV, PER>(p, N0, N1, i, (int)row);
+  });
+}
+
 // Covariance on the 1-D embedding grid (one long row: grid-stride over x).
 template <int COV, int PER>
 __global__ void k_eval_cov_1d(CovParams p, int64_t N0, double* out)
@@ -286,29 +317,6 @@ __global__ void k_c2c_strided(cplx<T>* a, Sched s, const cplx<T>* tw, int64_t st
   }
 }
 
-struct ClassifyPartial
-{
-  long long small, zero, negative;
-  double smallest;
-};
-
-__device__ __forceinline__ void classify_one(double& v, double thr, ClassifyPartial& c)
-{
-  if (v < c.smallest)
-    c.smallest = v;
-  if (v < 1e-6) {
-    if (v < thr) {
-      if (v > -thr)
-        c.zero++;
-      else
-        c.negative++;
-    } else
-      c.small++;
-  }
-  if (v < 0.)
-    v = 0.;
-}
-
 // Block-wide sum of the classification records; the result is valid in
 // thread 0.
 __device__ ClassifyPartial block_reduce(ClassifyPartial c)
@@ -487,6 +495,20 @@ __global__ void k_eval_cov_full_group(const CovParams* __restrict__ cps, int N0,
     const int64_t idx[3] = { i, (int64_t)((uint32_t)row % (uint32_t)N1),
                              (int64_t)((uint32_t)row / (uint32_t)N1) };
     out[row * N0 + i] = embedded_cov<COV, PER>(p, idx);
+  });
+}
+
+// k_eval_cov_even2d per field (group of 2D fields): field f = blockIdx.y
+// with its own parameters, rows at out + f * stride.
+template <int COV, int PER>
+__global__ void k_eval_cov_even2d_group(const CovParams* __restrict__ cps, int N0, int N1,
+                                        double* __restrict__ out, int64_t stride)
+{
+  CovParams p = cps[blockIdx.y];
+  p.dim = 2;
+  out += (int64_t)blockIdx.y * stride;
+  for_rows(N1 / 2 + 1, N0, [&](int64_t row, int i) {
+    out[row * N0 + i] = cov_even_2d<COV, PER>(p, N0, N1, i, (int)row);
   });
 }
 

@@ -33,6 +33,22 @@ inline int __builtin_amdgcn_ds_bpermute(int addr, int val)
   __syncthreads();
   return r;
 }
+// __shfl_down: lane i reads `v` of lane i + off of the same wave (its own
+// value where that lane lies outside the wave); every thread of the block
+// takes part.
+extern unsigned long long emu_shfl[1024];
+template <typename T>
+inline T __shfl_down(T v, unsigned off)
+{
+  static_assert(sizeof(T) <= 8, "64-bit lanes");
+  std::memcpy(&emu_shfl[threadIdx.x], &v, sizeof(T));
+  __syncthreads();
+  T r = v;
+  if ((threadIdx.x & 63u) + off < 64u)
+    std::memcpy(&r, &emu_shfl[threadIdx.x + off], sizeof(T));
+  __syncthreads();
+  return r;
+}
 // the hardware transcendentals (log2, sin/cos of revolutions, sqrt)
 inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
 inline float __builtin_amdgcn_sinf(float x) { return std::sin(6.283185307179586f * x); }

@@ -124,6 +124,8 @@ def test_dft_passes_host_build(tmp_path):
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "all OK" in r.stdout and r.stdout.count(" OK") >= 11
 
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
 def test_fused_product_pass_host_build(tmp_path):
     """The fused last-axis pass of the covariance product (kf_fused_mult),
     every block of 3D (both storage orders) and 2D shapes, FP64 and FP32,
@@ -138,3 +140,23 @@ def test_fused_product_pass_host_build(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "all OK" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_even_part_eigenvalue_y_pass_host_build(tmp_path):
+    """The real y pass of the 2D R2C eigenvalues (kf_eig_c2r_y): from the x
+    half spectrum of rows 0 .. N1/2 of a point-symmetric array, every block
+    of two realizations (own thresholds), FP32 and FP64 roots, with and
+    without the FP64 eigenvalues, N1 from 32 to 2048, against direct 2D DFTs;
+    classification counts summed over the block partials match exactly."""
+    src = _host_sources(str(tmp_path))
+    exe = str(tmp_path / "eig_y_check")
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-w", "-I", EMU, "-I", src,
+           os.path.join(EMU, "eig_y_check.cc"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "all OK" in r.stdout
+    assert r.stdout.count(" OK") >= 21
