@@ -25,7 +25,15 @@ def uneven_batch(nfields: int = 1000, seed: int = 3, sides=UNEVEN_SIDES, precisi
     geometric anisotropy A = R(theta) diag(L, L/r), theta ~ U[0, pi),
     r ~ U[1, 8], L ~ U[0.02, 0.2]; R2C matrix with the full real-part check
     (the reference's interleaved check would turn Gaussian fields into NaN).
-    Returns (descs, shapes, algorithmic bytes per field)."""
+    Returns (descs, shapes, algorithmic bytes per field).
+
+    Algorithmic bytes of a field (embedding N0 = 2 n0, N1 = 2 n1, H = n0 + 1,
+    h = n1 + 1 rows of the even part): covariance rows written (8 N0 h),
+    read by the r2c along x (8 N0 h) which writes the half spectrum
+    (16 H h); the real y pass reads it (16 H h) and writes the FP32 root
+    table (4 H N1); the generation pass reads the table (4 H N1) and writes
+    the y-pruned FP32 spectrum (8 H n1); the x c2r reads it (8 H n1) and
+    writes the field (4 n0 n1)."""
     rng = np.random.default_rng(seed)
     descs, shapes, falg = [], [], []
     for _ in range(nfields):
@@ -37,8 +45,8 @@ def uneven_batch(nfields: int = 1000, seed: int = 3, sides=UNEVEN_SIDES, precisi
         descs.append(pfb.make_desc(2, [n0, n1], [1.0, 1.0], covariance="gaussian",
                                    anisotropy="geometric", corr_length=list(A.ravel()),
                                    approximate=True, r2c_check_all=True, precision=precision))
-        H, N1 = n0 + 1, 2 * n1
-        falg.append(48 * H * N1 + 16 * H * n1 + 4 * n0 * n1)
+        N0, N1, H, h = 2 * n0, 2 * n1, n0 + 1, n1 + 1
+        falg.append(16 * N0 * h + 32 * H * h + 8 * H * N1 + 16 * H * n1 + 4 * n0 * n1)
     return descs, shapes, falg
 
 
