@@ -35,7 +35,10 @@ constexpr int pad_L(int M)
 template <typename T>
 constexpr int fast_NT(int N, int L)
 {
-  const int me = 16; // values per thread and stage (FP32: 512 threads for a 16 x 512 tile)
+  // values per thread and stage: 16 for FP32 (512 threads for a 16 x 512
+  // tile), 8 for FP64, whose values take twice the registers (more waves in
+  // flight per tile)
+  const int me = sizeof(T) == 4 ? 16 : 8;
   int nt = (L * N + me - 1) / me;
   nt = ((nt + 63) / 64) * 64;
   return nt < 64 ? 64 : nt;

@@ -27,7 +27,7 @@ namespace pfamd { alignas(16) unsigned char smem[160 * 1024]; }
 using namespace pfamd;
 // tile shape the launcher picks for FP64 (fast_L / fast_NT in pf_fast_impl.hpp)
 constexpr int fast_L64(int N) { int L = 8; while (L > 1 && L * N > 4096) L /= 2; return L; }
-constexpr int fast_NT64(int N, int L) { int nt = (L * N + 15) / 16; nt = (nt + 63) / 64 * 64; return nt < 64 ? 64 : nt; }
+constexpr int fast_NT64(int N, int L) { int nt = (L * N + 7) / 8; nt = (nt + 63) / 64 * 64; return nt < 64 ? 64 : nt; }
 struct Buf { const char* name; char* lo; char* hi; };
 static std::vector<Buf> bufs;
 static const size_t GUARD = (size_t)1 << 30;
