@@ -251,6 +251,35 @@ def test_eigs_r2c_check_all_match_oracle(dim, cells, kw, path, monkeypatch):
     assert rel_max(p.generate(1, noise=noise)[0], orc.generate(d, noise)) < tol
 
 
+@pytest.mark.parametrize("cells", [[24, 16], [48, 96]])
+def test_eigs_custom_values_even_part_matches_full_transform(cells, monkeypatch):
+    """Custom covariance values (pf_eigs_custom, the padded 2H x N1 array) on
+    a 2D R2C plan with the full real-part check, for an arbitrary array that
+    is not point-symmetric: the even-part path (values symmetrised on the
+    host, half the rows, real y pass) gives the eigenvalues of numpy's 2D FFT
+    (real part / N_ext, negatives clamped), and the eigenvalues and
+    classification of the full-grid generic kernels (PF_NO_FAST=1)."""
+    d = desc(2, cells, covariance="exponential", corr_length=0.1, matrix="r2c",
+             r2c_check_all=True, approximate=True)
+    N0, N1 = 2 * cells[0], 2 * cells[1]
+    H = N0 // 2 + 1
+    c = np.random.default_rng(5).standard_normal((N1, N0))
+    c[0, 0] += 40.0
+    vals = np.zeros((N1, 2 * H))
+    vals[:, :N0] = c
+    p = pfb.Plan(d)
+    st = p.eigs_custom(vals.ravel())
+    got = p.eig_values().reshape(N1, H)
+    want = np.maximum(np.fft.fft2(c).real[:, :H] / (N0 * N1), 0.0)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+    monkeypatch.setenv("PF_NO_FAST", "1")
+    pg = pfb.Plan(d)
+    stg = pg.eigs_custom(vals.ravel())
+    assert np.abs(got - pg.eig_values().reshape(N1, H)).max() <= 1e-13 * np.abs(want).max()
+    for k in ("small", "zero", "negative"):
+        assert st[k] == stg[k], (k, st, stg)
+
+
 def test_negative_eigenvalue_error():
     d = desc(2, [16, 16], covariance="gaussian", corr_length=0.5)
     p = pfb.Plan(d)
