@@ -748,24 +748,6 @@ __global__ void __launch_bounds__(NT) kf_eig_c2r_y(const cplx<double>* spec, int
   TS* sqf = sq + f * cnt + kx0;
   double* eigf = eig ? eig + f * cnt + kx0 : nullptr;
   const uint32_t eb = (uint32_t)Hp * (uint32_t)sizeof(cplx<double>);
-  auto load = [&](int e, int l) -> cplx<double> {
-    cplx<double> z = { 0, 0 };
-    if (l < lmax) {
-      const uint32_t lo = (uint32_t)l * (uint32_t)sizeof(cplx<double>);
-      cplx<double> xk = *reinterpret_cast<const cplx<double>*>(sb + ((uint32_t)e * eb + lo));
-      cplx<double> xm = *reinterpret_cast<const cplx<double>*>(sb + ((uint32_t)(M - e) * eb + lo));
-      if (e == 0) {
-        xk.y = 0;
-        xm.y = 0;
-      }
-      // packing of conj X: s = conj X[e] + X[M-e], d = conj X[e] - X[M-e]
-      const cplx<double> s = { xk.x + xm.x, xm.y - xk.y };
-      const cplx<double> d = { xk.x - xm.x, -xk.y - xm.y };
-      const cplx<double> td = cmul(cconj(tw[e]), d);
-      z = { s.x - td.y, s.y + td.x };
-    }
-    return z;
-  };
   ClassifyPartial c = { 0, 0, 0, 1.7976931348623157e308 };
   auto store = [&](int m, int l, const cplx<double>& v) {
     if (l < lmax) {
@@ -781,7 +763,51 @@ __global__ void __launch_bounds__(NT) kf_eig_c2r_y(const cplx<double>* spec, int
       sqf[o1] = (TS)sqrt(b);
     }
   };
-  tfft_io<double, M, L, NT, 0, true, 2>(buf, tw, (int)threadIdx.x, load, store);
+  {
+    // the packed column Z[k] from X[k] and X[M - k]: one thread loads each
+    // row pair (k, M - k) once and writes Z[k] and Z[M - k] to the tile
+    using TP = TileLoop<L * (M / 2 + 1), NT>;
+    cplx<double> xa[TP::IT], xb[TP::IT];
+#pragma unroll
+    for (int it = 0; it < TP::IT; it++) {
+      const int idx = TP::idx(it);
+      const int l = idx % L, k = idx / L;
+      xa[it] = { 0, 0 };
+      xb[it] = { 0, 0 };
+      if (TP::live(it) && l < lmax) {
+        const uint32_t lo = (uint32_t)l * (uint32_t)sizeof(cplx<double>);
+        xa[it] = *reinterpret_cast<const cplx<double>*>(sb + ((uint32_t)k * eb + lo));
+        xb[it] = *reinterpret_cast<const cplx<double>*>(sb + ((uint32_t)(M - k) * eb + lo));
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < TP::IT; it++) {
+      if (!TP::live(it))
+        continue;
+      const int idx = TP::idx(it);
+      const int l = idx % L, k = idx / L;
+      cplx<double> xk = xa[it], xm = xb[it];
+      if (k == 0) {
+        xk.y = 0;
+        xm.y = 0;
+      }
+      {
+        // packing of conj X: s = conj X[k] + X[M-k], d = conj X[k] - X[M-k]
+        const cplx<double> sm = { xk.x + xm.x, xm.y - xk.y };
+        const cplx<double> d = { xk.x - xm.x, -xk.y - xm.y };
+        const cplx<double> td = cmul(cconj(tw[k]), d);
+        buf[tix<L, 0>(k, l)] = { sm.x - td.y, sm.y + td.x };
+      }
+      if (k != 0 && 2 * k != M) {
+        const cplx<double> sm = { xm.x + xk.x, xk.y - xm.y };
+        const cplx<double> d = { xm.x - xk.x, -xm.y - xk.y };
+        const cplx<double> td = cmul(cconj(tw[M - k]), d);
+        buf[tix<L, 0>(M - k, l)] = { sm.x - td.y, sm.y + td.x };
+      }
+    }
+    __syncthreads();
+    tfft_io<double, M, L, NT, 0, true, 2>(buf, tw, (int)threadIdx.x, TileLds{}, store);
+  }
   // block sum of the partials through the tile's LDS (free after the barrier)
   __syncthreads();
   for (int off = 32; off > 0; off >>= 1) {
