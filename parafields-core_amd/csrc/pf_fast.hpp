@@ -159,13 +159,18 @@ __device__ __forceinline__ void tstage(cplx<T>* buf, const cplx<T>* tw, int tid,
           v[it][r] = in(j + r * NR, l);
       }
       if (NS > 1) {
+        // one table load per butterfly: w^r = w^(r/2) * w^(r - r/2) from w
+        // (at most three products deep for radix 8) instead of R - 1 loads
+        cplx<T> t[R];
+        t[1] = tw[k * TSTEP];
+        if (INV)
+          t[1].y = -t[1].y;
 #pragma unroll
-        for (int r = 1; r < R; r++) {
-          cplx<T> t = tw[r * k * TSTEP];
-          if (INV)
-            t.y = -t.y;
-          v[it][r] = cmul(v[it][r], t);
-        }
+        for (int r = 2; r < R; r++)
+          t[r] = cmul(t[r / 2], t[r - r / 2]);
+#pragma unroll
+        for (int r = 1; r < R; r++)
+          v[it][r] = cmul(v[it][r], t[r]);
       }
       Butterfly<T, R, INV>::run(v[it]);
     }
