@@ -415,6 +415,18 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
         return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(sqt) +
                                            (uint32_t)i * (uint32_t)sizeof(T));
     };
+    // the eigenvalues at offsets i and i + 1 in one load (element-aligned)
+    auto sq_pair = [sqt](auto i, T& x, T& y) {
+      const char* p;
+      if constexpr (IDX64)
+        p = reinterpret_cast<const char*>(sqt + i);
+      else
+        p = reinterpret_cast<const char*>(sqt) + (uint32_t)i * (uint32_t)sizeof(T);
+      T v[2];
+      __builtin_memcpy(v, p, sizeof(v));
+      x = v[0];
+      y = v[1];
+    };
     const IX base0 = (IX)((int64_t)a.H * ((int64_t)km0 * a.sB) + kx0);
     const int par0 = (int)(base0 & 1), rs = (int)(rowstep & 1);
     const int tid = (int)threadIdx.x;
@@ -432,7 +444,10 @@ __device__ __forceinline__ void gen_last_tile(const GenGeom& g, const LastAxis& 
       for (int it = 0; it < ITM; it++) {
         if (N % ES == 0 || e < N) {
           const IX row = (g.dct && e >= Pe) ? dn : up;
-          const T sa = sq_at(row + (IX)lac), sb = sq_at(row + (IX)lb);
+          // columns lac and lac + 1 (= lb, or column 1 when la = -1 and lb = 0)
+          T sa, sp;
+          sq_pair(row + (IX)lac, sa, sp);
+          const T sb = la < 0 ? sa : sp;
           const uint64_t blk = (uint64_t)base >> 1;
           uint32_t c[4];
           philox_block<IDX64>(blk, real, sk0, sk1, c);
