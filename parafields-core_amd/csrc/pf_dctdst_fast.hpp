@@ -23,6 +23,28 @@ namespace pfamd {
 
 constexpr int DCT_PAD = -1; // tile layout tix<L, -1>: pitch L, permuted rows
 
+// 16 bytes of consecutive reals at a 16-byte aligned address (rows of the
+// work arrays start on 128-byte boundaries and interior tiles at multiples of
+// L >= 4 columns): one vector access per lane instead of 16 / sizeof(T).
+template <typename T>
+struct Vec16
+{
+  static constexpr int V = 16 / (int)sizeof(T);
+  T v[V];
+};
+template <typename T>
+__device__ __forceinline__ Vec16<T> load16(const T* p)
+{
+  Vec16<T> r;
+  __builtin_memcpy(&r, __builtin_assume_aligned(p, 16), 16);
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void store16(T* p, const Vec16<T>& r)
+{
+  __builtin_memcpy(__builtin_assume_aligned(p, 16), &r, 16);
+}
+
 // X_m of line l into the tile (m <= M): row m, except that X_0 and X_M
 // share row 0 as (Re X_0, Re X_M).
 template <typename T, int M, int L>
@@ -274,6 +296,46 @@ __global__ void __launch_bounds__(NT, 4) kdf_mid(DctGeom g, const T* G, T* Hh, c
     const T* ge = Gb + (int64_t)tx * gsize + (int64_t)z0 * gplane + kx0;
     const T* go = ge + 2 * gsize;
     const uint32_t zstep = edge ? (uint32_t)gplane : 1u; // step of l
+    if (!edge) {
+      // interior tile: a row's L columns are contiguous, Vec16 per lane; two
+      // batches of loads keep the staging registers within the budget
+      constexpr int V = Vec16<T>::V;
+      static_assert(L % V == 0, "whole vectors per tile row");
+      using TV = TileLoop<(L / V) * (M + 1), NT>;
+      constexpr int HV = (TV::IT + 1) / 2;
+#pragma unroll
+      for (int h = 0; h < TV::IT; h += HV) {
+        Vec16<T> ve[HV], vo[HV];
+#pragma unroll
+        for (int i = 0; i < HV; i++) {
+          const int it = h + i;
+          const int idx = TV::idx(it, tid);
+          const int l0 = (idx % (L / V)) * V, m = idx / (L / V);
+          if (it < TV::IT && TV::live(it, tid) && z0 < g.n2) {
+            const uint32_t o = (uint32_t)m * (uint32_t)g.P0p + (uint32_t)l0;
+            ve[i] = load16(ge + o);
+            vo[i] = load16(go + o);
+          } else {
+#pragma unroll
+            for (int u = 0; u < V; u++) {
+              ve[i].v[u] = 0;
+              vo[i].v[u] = 0;
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < HV; i++) {
+          const int it = h + i;
+          if (it >= TV::IT || !TV::live(it, tid))
+            continue;
+          const int idx = TV::idx(it, tid);
+          const int l0 = (idx % (L / V)) * V, m = idx / (L / V);
+#pragma unroll
+          for (int u = 0; u < V; u++)
+            put_x<T, M, L>(buf, m, l0 + u, { ve[i].v[u], -vo[i].v[u] });
+        }
+      }
+    } else {
     // two batches of loads (all of a batch in flight at once) keep the
     // staging registers within a 4-waves-per-SIMD budget
     constexpr int HB = (TL::IT + 1) / 2;
@@ -304,20 +366,39 @@ __global__ void __launch_bounds__(NT, 4) kdf_mid(DctGeom g, const T* G, T* Hh, c
         put_x<T, M, L>(buf, m, l, { ve[i], -vo[i] });
       }
     }
+    }
     __syncthreads();
     c2r_tile_ct<T, M, L, NT>(buf, tw, tid);
     T* hb = Hb + (int64_t)tx * hsize + (int64_t)z0 * hplane + kx0;
-    const uint32_t hstep = edge ? (uint32_t)hplane : 1u;
-    using TO = TileLoop<L * M, NT>;
+    if (!edge) {
+      constexpr int V = Vec16<T>::V;
+      using TV = TileLoop<(L / V) * M, NT>;
 #pragma unroll
-    for (int it = 0; it < TO::IT; it++) {
-      const int idx = TO::idx(it, tid);
-      const int l = idx % L, y = idx / L;
-      const int kx = edge ? kx0 : kx0 + l;
-      const int z = edge ? z0 + l : z0;
-      if (TO::live(it, tid) && y < g.n1 && z < g.n2) {
-        const cplx<T> v = buf[tix<L, DCT_PAD>(y >> 1, l)];
-        hb[(uint32_t)y * (uint32_t)g.P0p + (uint32_t)l * hstep] = (y & 1) ? v.y : v.x;
+      for (int it = 0; it < TV::IT; it++) {
+        const int idx = TV::idx(it, tid);
+        const int l0 = (idx % (L / V)) * V, y = idx / (L / V);
+        if (TV::live(it, tid) && y < g.n1 && z0 < g.n2) {
+          Vec16<T> w;
+#pragma unroll
+          for (int u = 0; u < V; u++) {
+            const cplx<T> v = buf[tix<L, DCT_PAD>(y >> 1, l0 + u)];
+            w.v[u] = (y & 1) ? v.y : v.x;
+          }
+          store16(hb + ((uint32_t)y * (uint32_t)g.P0p + (uint32_t)l0), w);
+        }
+      }
+    } else {
+      const uint32_t hstep = (uint32_t)hplane;
+      using TO = TileLoop<L * M, NT>;
+#pragma unroll
+      for (int it = 0; it < TO::IT; it++) {
+        const int idx = TO::idx(it, tid);
+        const int l = idx % L, y = idx / L;
+        const int z = z0 + l;
+        if (TO::live(it, tid) && y < g.n1 && z < g.n2) {
+          const cplx<T> v = buf[tix<L, DCT_PAD>(y >> 1, l)];
+          hb[(uint32_t)y * (uint32_t)g.P0p + (uint32_t)l * hstep] = (y & 1) ? v.y : v.x;
+        }
       }
     }
     __syncthreads();
